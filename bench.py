@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Time-to-solution benchmark of the MI355X SaP solver.
+
+Workload (default): synthetic banded system N=1,000,000, half-bandwidth
+K=200, off-diagonals U[-1,1), diagonal 0.8 x row off-diagonal abs-sum (not
+diagonally dominant), x_true ~ U[-1,1), b = A x_true.  SaP-C (truncated
+SPIKE, LU_UL) with P=64 partitions, FP32 factors and sweeps, FP64
+BiCGStab(2) with FP64 refinement, stopping at relative residual 1e-10.
+
+One step = setup() + solve() from scratch with A and b resident in HBM
+(the reference's Stats.timeSetup + Stats.timeSolve).  The plain path (FP64
+SaP-D, same P, no refinement) is timed beside it for the speedup.
+
+Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-to-solution @ rel. residual 1e-10, N=1M K=200 banded; speedup vs plain"
+
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, GB/s
+FP32_MFMA_PEAK_TFLOPS = 157.3   # MI355X dense FP32 matrix
+FP32_VALU_PEAK_TFLOPS = 78.6    # MI355X FP32 vector without packed math
+
+WORKLOADS = {
+    # name: (n, k, dominance, P, seed_A, seed_x)
+    "c1": (1_000_000, 200, 0.8, 64, 10, 11),
+    "c0": (200_000, 50, 1.2, 16, 0, 1),
+}
+
+
+def band_nnz(n, k):
+    return n * (2 * k + 1) - k * (k + 1)
+
+
+def part_sizes(n, P):
+    q, r = divmod(n, P)
+    return [q + 1 if p < r else q for p in range(P)]
+
+
+def factor_nnz(n, k, P):
+    # stored entries of the partition-diagonal blocks of the band
+    return sum(ln * (2 * k + 1) - k * (k + 1) for ln in part_sizes(n, P))
+
+
+def algorithmic(name, n, k, P, sp, spike):
+    """(bytes or flops, unit, bound) per launch of a kernel family."""
+    if name == "spmv":
+        return 8 * band_nnz(n, k) + 16 * n, "GB/s", "hbm"
+    if name == "sweep_full":
+        extra = 2 * k * (P - 1) * sp if spike else 0
+        return sp * factor_nnz(n, k, P) + 16 * n + extra, "GB/s", "hbm"
+    if name == "sweep_interface":
+        # L of LU (partitions 0..P-2) and U of UL (1..P-1), plus D and v
+        sizes = part_sizes(n, P)
+        half = sum(ln * (k + 1) - k * (k + 1) // 2 for ln in sizes[:-1]) + \
+            sum(ln * (k + 1) - k * (k + 1) // 2 for ln in sizes[1:])
+        return sp * half + 8 * (2 * n), "GB/s", "hbm"
+    if name == "band_factor":
+        # 2 n_p k^2 flops per LU or UL of a partition
+        sizes = part_sizes(n, P)
+        jobs = (sizes[:-1] + sizes[1:]) if spike else sizes
+        return sum(2.0 * ln * k * k for ln in jobs), "TFLOP/s", "mfma"
+    return None, None, None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported; this build runs on one GPU "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import saplibrary_amd as sap
+
+    n, k, dom, P, sa, sx = WORKLOADS[args.workload]
+    spike = args.workload == "c1"
+    print(f"[bench] workload {args.workload}: n={n} k={k} d={dom} P={P}", flush=True)
+
+    A = sap.BandedMatrix.generate(n, k, dom, sa)
+    xt = sap.gen_vector(n, sx)
+    b = torch.empty_like(xt)
+    sap.band_spmv(A, xt, b)
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+
+    def options(plain):
+        if plain:
+            return sap.Options(precondType=sap.Block, factMethod=sap.LU_UL, precValueType=sap.PREC_DOUBLE,
+                               relTol=1e-10, profile=1)
+        return sap.Options(precondType=sap.Spike if spike else sap.Block, factMethod=sap.LU_UL,
+                           precValueType=sap.PREC_FLOAT, relTol=1e-10, refine=8, profile=1)
+
+    def run(plain, steps, warmup, label):
+        # one solver object: buffers allocated by the warm-up setup are reused
+        s = sap.Solver(P, options(plain))
+        times, last = [], None
+        for it in range(warmup + steps):
+            if it == warmup:
+                s.reset_profile()
+            x.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s.setup(A)
+            ok = s.solve(None, b, x)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            st = s.getStats()
+            if it >= warmup:
+                times.append((t1 - t0) * 1e3)
+                last = (s, st, ok)
+            print(f"[bench] {label} step {it}: {(t1 - t0) * 1e3:.2f} ms ok={ok} "
+                  f"its={st.numIterations} rel={st.relResidualNorm:.2e} setup={st.timeSetup:.2f} "
+                  f"solve={st.timeSolve:.2f}", flush=True)
+        return times, last
+
+    times, (solver, st, ok) = run(False, args.steps, args.warmup, "SaP-C fp32" if spike else "SaP-D fp32")
+    prof = solver.profile()
+    ms = sum(times) / len(times)
+
+    # independent checks: true residual and forward error against x_true
+    r = torch.empty_like(b)
+    sap.band_spmv(A, x, r)
+    rel = float(torch.linalg.norm(b - r) / torch.linalg.norm(b))
+    ferr = float(torch.linalg.norm(x - xt) / torch.linalg.norm(xt))
+
+    # dominant kernel family (per step) and its roofline
+    steps_profiled = args.steps
+    fam = max(prof.items(), key=lambda kv: kv[1][1])
+    name, (launches, total_ms) = fam
+    algo, unit, bound = algorithmic(name, n, k, P, 4, spike)
+    roof = None
+    if algo is not None and launches > 0:
+        avg_s = total_ms / launches / 1e3
+        if unit == "GB/s":
+            achieved = algo / avg_s / 1e9
+            peak = HBM_PEAK_GBS
+        else:
+            achieved = algo / avg_s / 1e12
+            peak = FP32_MFMA_PEAK_TFLOPS
+        roof = {"kernel": name, "bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": unit,
+                "frac": round(achieved / peak, 4), "traffic": None,
+                "avg_launch_ms": round(total_ms / launches, 4), "launches_per_step": launches / steps_profiled,
+                "algorithmic_per_launch": algo}
+    kernels = {kname: {"launches_per_step": round(l / steps_profiled, 2),
+                       "ms_per_step": round(t / steps_profiled, 3)} for kname, (l, t) in prof.items()}
+
+    plain_ms = None
+    if not args.no_plain:
+        ptimes, (_, pst, pok) = run(True, max(1, args.steps), args.warmup, "plain fp64 SaP-D")
+        plain_ms = sum(ptimes) / len(ptimes)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.workload, args.cpu_threads)
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32 preconditioner + f64 Krylov",
+        "data": "synthetic (xoshiro256** generator, seeds %d/%d)" % (sa, sx),
+        "config": {"workload": f"banded N={n} K={k} diag={dom}x off-diag abs-sum; "
+                               f"{'SaP-C LU_UL' if spike else 'SaP-D'} P={P}; BiCGStab(2) + FP64 refinement; "
+                               f"rel tol 1e-10",
+                   "n": n, "k": k, "partitions": st.numPartitions},
+        "converged": bool(ok),
+        "rel_residual": rel,
+        "forward_error": ferr,
+        "iterations": st.numIterations,
+        "setup_ms": round(st.timeSetup, 3),
+        "solve_ms": round(st.timeSolve, 3),
+        "speedup_vs_plain": round(plain_ms / ms, 3) if plain_ms else None,
+        "plain_ms": round(plain_ms, 3) if plain_ms else None,
+        "kernels": kernels,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(workload, threads):
+    """The CPU oracle (a restatement of the reference algorithm, OpenMP over
+    partitions) timed on the host cores for the same workload."""
+    if threads > 0:
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+    import numpy as np
+    from oracle import pyoracle as O
+
+    n, k, dom, P, sa, sx = WORKLOADS[workload]
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    band = O.gen_banded(n, k, dom, sa)
+    b = O.band_spmv(band, O.gen_vector(n, sx))
+    S = O.Solver(P, rel_tol=1e-10, fact="LU_UL", precond="Spike" if workload == "c1" else "Block", refine=8)
+    t0 = time.perf_counter()
+    rc = S.setup(band)
+    ok, _ = S.solve(b)
+    t1 = time.perf_counter()
+    st = S.stats()
+    print(f"[bench] cpu oracle: {(t1 - t0):.2f} s ok={ok} rc={rc} its={st['num_iterations']}", flush=True)
+    del band, b
+    return {"value": round((t1 - t0) * 1e3, 1), "unit": "ms", "cores": cores, "kind": "port",
+            "sample": f"the full {workload} workload, one setup+solve, OMP_NUM_THREADS={cores}",
+            "converged": bool(ok), "rel_residual": st["rel_residual_norm"]}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,207 @@
+/*
+ * sap_c.h -- C ABI of the MI355X-native SaP (SPIKE) solver.
+ *
+ * One entry point per member of the reference's public surface
+ * (spikegpu/SaPLibrary sap/solver.h), with plain pointers and sizes:
+ *
+ *   sap::Options / Options::Options()      -> sap_options / sap_default_options
+ *                                              (sap/solver.h:52-81, :232-256)
+ *   sap::Stats                             -> sap_stats        (sap/solver.h:89-140)
+ *   Solver(int numPartitions, Options)     -> sap_create       (sap/solver.h:157-158, :302-328)
+ *   ~Solver()                              -> sap_destroy      (sap/solver.h:159-168)
+ *   Solver::setup(BandedMatrix)            -> sap_setup_banded (sap/solver.h:339-404,
+ *                                              sap/precond.h:2103-2233)
+ *   Solver::solve(spmv, b, x)              -> sap_solve        (sap/solver.h:477-562)
+ *   Solver::getStats()                     -> sap_get_stats    (sap/solver.h:182)
+ *   Solver::getMonitorCode()               -> sap_monitor_code (sap/solver.h:183-188)
+ *   Solver::getMonitorMessage()            -> sap_monitor_message (sap/solver.h:189-194)
+ *   MVBanded::operator()(v, Av)            -> sap_band_spmv    (sap/spmv.h:27-74)
+ *   Precond::operator()(v, z)              -> sap_precond_apply (sap/precond.h:1370-1386)
+ *
+ * Conventions
+ *   - All array arguments named d_* are device pointers owned by the caller.
+ *     The solver copies what it keeps, except the banded matrix given to
+ *     sap_setup_banded: like the reference's MVBanded functor (sap/spmv.h:73)
+ *     the built-in SpMV keeps a non-owning pointer to it, so it must stay
+ *     alive until the last sap_solve that uses the built-in SpMV.
+ *   - Banded input: row-major band, A(i,j) at d_band[i*(2k+1) + k + (j-i)]
+ *     (sap/banded_matrix.h:74-92).
+ *   - Return codes mirror sap::system_error::Reason (sap/exception.h:16-23):
+ *     0 OK, -1 zero pivot, -2 negative DB weight, -3 illegal update,
+ *     -4 illegal solve, -5 singular matrix; plus -6 out of device memory,
+ *     -7 HIP runtime error, -8 bad argument, -9 unsupported option.
+ *     sap_solve returns 1 (converged) or 0 (not converged) when it runs.
+ *   - No exception crosses this boundary.  A solver object is not
+ *     thread-safe; all work is queued on one HIP stream (sap_set_stream).
+ */
+#ifndef SAP_C_H
+#define SAP_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SAP_OK                 0
+#define SAP_ERR_ZERO_PIVOT    -1
+#define SAP_ERR_NEG_DB_WEIGHT -2
+#define SAP_ERR_ILLEGAL_UPDATE -3
+#define SAP_ERR_ILLEGAL_SOLVE -4
+#define SAP_ERR_SINGULAR      -5
+#define SAP_ERR_OOM           -6
+#define SAP_ERR_HIP           -7
+#define SAP_ERR_BADARG        -8
+#define SAP_ERR_UNSUPPORTED   -9
+
+/* sap::KrylovSolverType (sap/common.h:71-82), same order */
+enum { SAP_BiCGStab_C = 0, SAP_GMRES_C, SAP_CG_C, SAP_CR_C,
+       SAP_BiCGStab1, SAP_BiCGStab2, SAP_BiCGStab, SAP_MINRES };
+/* sap::FactorizationMethod (sap/common.h:84-87) */
+enum { SAP_LU_UL = 0, SAP_LU_only = 1 };
+/* sap::PreconditionerType (sap/common.h:92-96) */
+enum { SAP_Spike = 0, SAP_Block = 1, SAP_None = 2 };
+/* the PrecValueType template argument of sap::Solver */
+enum { SAP_PREC_FLOAT = 0, SAP_PREC_DOUBLE = 1 };
+
+/* Field-for-field mirror of sap::Options (sap/solver.h:52-81) followed by
+ * the options this implementation adds. */
+typedef struct sap_options {
+    int    solverType;          /* default SAP_BiCGStab2                           */
+    int    maxNumIterations;    /* default 100                                     */
+    double relTol;              /* default 1e-6                                    */
+    double absTol;              /* default 0                                       */
+    int    testDB;
+    int    isSPD;
+    int    saveMem;
+    int    performReorder;      /* default 1                                       */
+    int    performDB;           /* default 1                                       */
+    int    dbFirstStageOnly;
+    int    applyScaling;        /* default 1                                       */
+    int    maxBandwidth;        /* default INT_MAX                                 */
+    int    gpuCount;            /* default 1; only 1 is supported                  */
+    double dropOffFraction;     /* default 0                                       */
+    int    factMethod;          /* default SAP_LU_only                             */
+    int    precondType;         /* default SAP_Spike                               */
+    int    safeFactorization;   /* default 0                                       */
+    int    variableBandwidth;   /* default 1                                       */
+    int    trackReordering;     /* default 0                                       */
+    int    useBCR;              /* default 0; only 0 is supported                  */
+    int    ilu_level;           /* default -1; only -1 is supported                */
+    /* ---- additions ---- */
+    int    precValueType;       /* SAP_PREC_FLOAT (default) or SAP_PREC_DOUBLE     */
+    int    refine;              /* outer FP64 refinement steps around the Krylov
+                                   solve; 0 (default) runs the reference's single
+                                   right-preconditioned solve                       */
+    double refineTol;           /* relative tolerance of each inner solve (1e-6)   */
+    int    profile;             /* 1: time every kernel family with HIP events     */
+} sap_options;
+
+/* Field-for-field mirror of sap::Stats (sap/solver.h:89-140), times in
+ * milliseconds as the reference reports them, followed by additions. */
+typedef struct sap_stats {
+    double timeSetup;
+    double timeUpdate;
+    double timeSolve;
+    double time_DB;
+    double time_DB_pre;
+    double time_DB_first;
+    double time_DB_second;
+    double time_DB_post;
+    double d_p1;
+    double d_p1_ori;
+    double diag_dom;
+    double diag_dom_ori;
+    double time_reorder;
+    double time_dropOff;
+    double time_cpu_assemble;
+    double time_transfer;
+    double time_toBanded;
+    double time_offDiags;
+    double time_bandLU;
+    double time_bandUL;
+    double time_fullLU;
+    double time_assembly;
+    double time_shuffle;
+    double time_bcr_lu;
+    double time_bcr_sweep_deflation;
+    double time_bcr_mat_mul_deflation;
+    double time_bcr_sweep_inflation;
+    double time_bcr_mv_inflation;
+    int    bandwidthReorder;
+    int    bandwidthDB;
+    int    bandwidth;
+    double nuKf;
+    double flops_LU;
+    int    numPartitions;
+    double actualDropOff;
+    float  numIterations;
+    double rhsNorm;
+    double residualNorm;
+    double relResidualNorm;
+    int    actual_nnz;
+    /* ---- additions ---- */
+    int    numApplies;          /* preconditioner applications in the last solve  */
+    int    numSpmv;             /* SpMV calls in the last solve                    */
+    int    numRefine;           /* outer refinement steps in the last solve        */
+} sap_stats;
+
+/* Kernel-family timing collected when sap_options.profile = 1. */
+typedef struct sap_kernel_profile {
+    const char* name;
+    int         launches;
+    double      total_ms;
+} sap_kernel_profile;
+
+typedef struct sap_solver sap_solver;
+
+/* y = A*v for a user-supplied operator (the SPMV functor of sap/spmv.h:38-56,
+ * examples/sequence/driver_seq.cu:82-96).  Called on the solver's stream. */
+typedef void (*sap_spmv_fn)(void* ctx, const double* d_v, double* d_Av, int n);
+
+void        sap_default_options(sap_options* opts);
+sap_solver* sap_create(int numPartitions, const sap_options* opts);
+void        sap_destroy(sap_solver* s);
+int         sap_set_stream(sap_solver* s, void* hip_stream);
+
+int         sap_setup_banded(sap_solver* s, int n, int k, const double* d_band);
+
+/* Solves A x = b.  d_x holds the initial guess on entry and the solution on
+ * return.  spmv == NULL uses the built-in banded SpMV of the setup matrix.
+ * nrhs > 1 solves the columns of b/x (leading dimension n) one after another. */
+int         sap_solve(sap_solver* s, sap_spmv_fn spmv, void* ctx,
+                      const double* d_b, double* d_x, int nrhs);
+
+int         sap_get_stats(const sap_solver* s, sap_stats* out);
+int         sap_monitor_code(const sap_solver* s);
+const char* sap_monitor_message(const sap_solver* s);
+const char* sap_last_error(const sap_solver* s);
+
+/* z = M^{-1} v through the preconditioner built by the last setup. */
+int         sap_precond_apply(sap_solver* s, const double* d_v, double* d_z);
+
+/* y = A x for a row-major FP64 band on the given HIP stream (NULL: default). */
+int         sap_band_spmv(int n, int k, const double* d_band, const double* d_x,
+                          double* d_y, void* hip_stream);
+
+/* Kernel-family timing since the last sap_reset_profile (profile = 1). */
+int         sap_get_profile(const sap_solver* s, sap_kernel_profile* out, int max_entries);
+void        sap_reset_profile(sap_solver* s);
+
+/* Diagnostics for tests: copies of the factored band (column-major,
+ * (2k+1) x n, as FP64), of the K x K spike tips (2 k^2 (P-1)) and of the
+ * Schur blocks S_i = I - W_i V_i after factorization (k^2 (P-1)); host out. */
+int         sap_get_factors(const sap_solver* s, double* h_out, int64_t count);
+int         sap_get_spikes(const sap_solver* s, double* h_wv, double* h_schur, int64_t count);
+int         sap_num_partitions(const sap_solver* s);
+
+/* Synthetic workload generators (xoshiro256** per row, seeded by splitmix64):
+ * off-diagonals U[-1,1), diagonal = dominance * row off-diagonal abs-sum. */
+int         sap_gen_banded(int64_t n, int k, double dominance, uint64_t seed,
+                           double* d_band, void* hip_stream);
+int         sap_gen_vector(int64_t n, uint64_t seed, double* d_x, void* hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SAP_C_H */

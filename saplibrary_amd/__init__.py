@@ -1,0 +1,332 @@
+"""MI355X-native SaP (SPIKE) solver -- Python binding of libsap_amd.so.
+
+The compute path is the HIP library built from saplibrary_amd/csrc (C ABI
+in include/sap_c.h).  This module mirrors the reference's C++ interface
+(spikegpu/SaPLibrary sap/solver.h: ``Options``, ``Stats``,
+``Solver(numPartitions, opts)`` with ``setup``/``solve``/``getStats``/
+``getMonitorCode``/``getMonitorMessage``; sap/banded_matrix.h:
+``BandedMatrix``; sap/spmv.h: ``MVBanded``) over device tensors.  PyTorch is
+only used for device memory; it is imported first so the library binds to
+the HIP runtime already loaded in the process.
+
+There is no CPU fallback: every entry point raises if the library is
+missing or the call fails.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (device memory; must be loaded before the HIP library)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsap_amd.so")
+
+# sap::KrylovSolverType (sap/common.h:71-82)
+BiCGStab_C, GMRES_C, CG_C, CR_C, BiCGStab1, BiCGStab2, BiCGStab, MINRES = range(8)
+# sap::FactorizationMethod (sap/common.h:84-87)
+LU_UL, LU_only = 0, 1
+# sap::PreconditionerType (sap/common.h:92-96)
+Spike, Block, NoPrecond = 0, 1, 2
+PREC_FLOAT, PREC_DOUBLE = 0, 1
+
+# sap::system_error::Reason (sap/exception.h:16-23) plus the ABI's own codes
+ERRORS = {
+    -1: "Zero_pivoting",
+    -2: "Negative_DB_weight",
+    -3: "Illegal_update",
+    -4: "Illegal_solve",
+    -5: "Matrix_singular",
+    -6: "Out_of_memory",
+    -7: "HIP_error",
+    -8: "Bad_argument",
+    -9: "Unsupported",
+}
+
+
+class SapError(RuntimeError):
+    """Counterpart of sap::system_error; ``reason`` is the negative code."""
+
+    def __init__(self, code, message=""):
+        self.reason = code
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+
+
+class Options(ctypes.Structure):
+    """sap::Options (sap/solver.h:52-81) followed by this library's additions."""
+
+    _fields_ = [
+        ("solverType", ctypes.c_int),
+        ("maxNumIterations", ctypes.c_int),
+        ("relTol", ctypes.c_double),
+        ("absTol", ctypes.c_double),
+        ("testDB", ctypes.c_int),
+        ("isSPD", ctypes.c_int),
+        ("saveMem", ctypes.c_int),
+        ("performReorder", ctypes.c_int),
+        ("performDB", ctypes.c_int),
+        ("dbFirstStageOnly", ctypes.c_int),
+        ("applyScaling", ctypes.c_int),
+        ("maxBandwidth", ctypes.c_int),
+        ("gpuCount", ctypes.c_int),
+        ("dropOffFraction", ctypes.c_double),
+        ("factMethod", ctypes.c_int),
+        ("precondType", ctypes.c_int),
+        ("safeFactorization", ctypes.c_int),
+        ("variableBandwidth", ctypes.c_int),
+        ("trackReordering", ctypes.c_int),
+        ("useBCR", ctypes.c_int),
+        ("ilu_level", ctypes.c_int),
+        ("precValueType", ctypes.c_int),
+        ("refine", ctypes.c_int),
+        ("refineTol", ctypes.c_double),
+        ("profile", ctypes.c_int),
+    ]
+
+    def __init__(self, **kw):
+        super().__init__()
+        lib().sap_default_options(ctypes.byref(self))
+        for key, val in kw.items():
+            if not hasattr(self, key):
+                raise AttributeError(key)
+            setattr(self, key, val)
+
+
+class Stats(ctypes.Structure):
+    """sap::Stats (sap/solver.h:89-140), times in ms, plus additions."""
+
+    _fields_ = [(f, ctypes.c_double) for f in (
+        "timeSetup", "timeUpdate", "timeSolve", "time_DB", "time_DB_pre", "time_DB_first", "time_DB_second",
+        "time_DB_post", "d_p1", "d_p1_ori", "diag_dom", "diag_dom_ori", "time_reorder", "time_dropOff",
+        "time_cpu_assemble", "time_transfer", "time_toBanded", "time_offDiags", "time_bandLU", "time_bandUL",
+        "time_fullLU", "time_assembly", "time_shuffle", "time_bcr_lu", "time_bcr_sweep_deflation",
+        "time_bcr_mat_mul_deflation", "time_bcr_sweep_inflation", "time_bcr_mv_inflation")] + [
+        ("bandwidthReorder", ctypes.c_int),
+        ("bandwidthDB", ctypes.c_int),
+        ("bandwidth", ctypes.c_int),
+        ("nuKf", ctypes.c_double),
+        ("flops_LU", ctypes.c_double),
+        ("numPartitions", ctypes.c_int),
+        ("actualDropOff", ctypes.c_double),
+        ("numIterations", ctypes.c_float),
+        ("rhsNorm", ctypes.c_double),
+        ("residualNorm", ctypes.c_double),
+        ("relResidualNorm", ctypes.c_double),
+        ("actual_nnz", ctypes.c_int),
+        ("numApplies", ctypes.c_int),
+        ("numSpmv", ctypes.c_int),
+        ("numRefine", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class KernelProfile(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("launches", ctypes.c_int), ("total_ms", ctypes.c_double)]
+
+
+SPMV_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int)
+
+# every symbol include/sap_c.h declares
+EXPORTS = (
+    "sap_default_options", "sap_create", "sap_destroy", "sap_set_stream", "sap_setup_banded", "sap_solve",
+    "sap_get_stats", "sap_monitor_code", "sap_monitor_message", "sap_last_error", "sap_precond_apply",
+    "sap_band_spmv", "sap_get_profile", "sap_reset_profile", "sap_get_factors", "sap_get_spikes",
+    "sap_num_partitions", "sap_gen_banded", "sap_gen_vector",
+)
+
+_lib = None
+
+
+def lib():
+    """Loads libsap_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C saplibrary_amd`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, i64, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+    L.sap_default_options.argtypes = [ctypes.POINTER(Options)]
+    L.sap_default_options.restype = None
+    L.sap_create.argtypes = [i, ctypes.POINTER(Options)]
+    L.sap_create.restype = vp
+    L.sap_destroy.argtypes = [vp]
+    L.sap_destroy.restype = None
+    L.sap_set_stream.argtypes = [vp, vp]
+    L.sap_setup_banded.argtypes = [vp, i, i, vp]
+    L.sap_solve.argtypes = [vp, SPMV_FN, vp, vp, vp, i]
+    L.sap_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.sap_monitor_code.argtypes = [vp]
+    L.sap_monitor_message.argtypes = [vp]
+    L.sap_monitor_message.restype = ctypes.c_char_p
+    L.sap_last_error.argtypes = [vp]
+    L.sap_last_error.restype = ctypes.c_char_p
+    L.sap_precond_apply.argtypes = [vp, vp, vp]
+    L.sap_band_spmv.argtypes = [i, i, vp, vp, vp, vp]
+    L.sap_get_profile.argtypes = [vp, ctypes.POINTER(KernelProfile), i]
+    L.sap_reset_profile.argtypes = [vp]
+    L.sap_reset_profile.restype = None
+    L.sap_get_factors.argtypes = [vp, ctypes.POINTER(d), i64]
+    L.sap_get_spikes.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(d), i64]
+    L.sap_num_partitions.argtypes = [vp]
+    L.sap_gen_banded.argtypes = [i64, i, d, ctypes.c_uint64, vp, vp]
+    L.sap_gen_vector.argtypes = [i64, ctypes.c_uint64, vp, vp]
+    _lib = L
+    return L
+
+
+def _dev_f64(t, name):
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+        raise TypeError(f"{name} must be a contiguous float64 device tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _check(code, handle=None):
+    if code < 0:
+        msg = lib().sap_last_error(handle).decode() if handle else ""
+        raise SapError(code, msg)
+    return code
+
+
+class BandedMatrix:
+    """Row-major FP64 band on the device (sap/banded_matrix.h:25-95):
+    A(i, j) at band[i, k + j - i]."""
+
+    def __init__(self, band):
+        if band.dim() != 2 or (band.shape[1] - 1) % 2:
+            raise ValueError("band must be n x (2k+1)")
+        _dev_f64(band, "band")
+        self.band = band
+        self.num_rows = band.shape[0]
+        self.m_k = (band.shape[1] - 1) // 2
+
+    @staticmethod
+    def generate(n, k, dominance, seed, device="cuda"):
+        band = torch.empty((n, 2 * k + 1), dtype=torch.float64, device=device)
+        _check(lib().sap_gen_banded(n, k, dominance, seed, ctypes.c_void_p(band.data_ptr()), None))
+        return BandedMatrix(band)
+
+
+def gen_vector(n, seed, device="cuda"):
+    x = torch.empty(n, dtype=torch.float64, device=device)
+    _check(lib().sap_gen_vector(n, seed, ctypes.c_void_p(x.data_ptr()), None))
+    return x
+
+
+def band_spmv(A, x, y):
+    """y = A x for a BandedMatrix (the MVBanded functor, sap/spmv.h:27-74)."""
+    _check(lib().sap_band_spmv(A.num_rows, A.m_k, _dev_f64(A.band, "A"), _dev_f64(x, "x"), _dev_f64(y, "y"), None))
+    return y
+
+
+class MVBanded:
+    """SPMV functor over a BandedMatrix (sap/spmv.h:27-74)."""
+
+    def __init__(self, A):
+        self.A = A
+        self.num_rows = A.num_rows
+
+    def __call__(self, v, Av):
+        band_spmv(self.A, v, Av)
+
+
+class Solver:
+    """sap::Solver<Array, PrecValueType> (sap/solver.h:153-225) over the C ABI."""
+
+    def __init__(self, numPartitions, opts=None):
+        self.opts = opts if opts is not None else Options()
+        h = lib().sap_create(numPartitions, ctypes.byref(self.opts))
+        if not h:
+            raise SapError(-9, "sap_create rejected the options")
+        self.h = ctypes.c_void_p(h)
+        self._A = None
+        self._keep = None
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.sap_destroy(h)
+            self.h = None
+
+    def setup(self, A):
+        """Builds the preconditioner (sap/solver.h:339-404).  Raises SapError."""
+        if not isinstance(A, BandedMatrix):
+            raise TypeError("setup() takes a BandedMatrix")
+        _check(lib().sap_setup_banded(self.h, A.num_rows, A.m_k, _dev_f64(A.band, "A")), self.h)
+        self._A = A
+        return True
+
+    def solve(self, spmv, b, x):
+        """Solves A x = b (sap/solver.h:477-562); x holds the initial guess.
+        spmv: None (built-in banded SpMV of the setup matrix), an MVBanded of
+        that matrix, or any callable f(v, Av) on device tensors."""
+        n = b.shape[0]
+        if spmv is None or (isinstance(spmv, MVBanded) and spmv.A is self._A):
+            fn = SPMV_FN()
+        else:
+            def _cb(ctx, pv, pav, nn):
+                v = _wrap(pv, nn, b.device)
+                av = _wrap(pav, nn, b.device)
+                spmv(v, av)
+            fn = SPMV_FN(_cb)
+        self._keep = fn
+        rc = lib().sap_solve(self.h, fn, None, _dev_f64(b, "b"), _dev_f64(x, "x"), 1 if b.dim() == 1 else b.shape[1])
+        _check(rc, self.h)
+        return rc == 1
+
+    def precond_apply(self, v, z):
+        """z = M^{-1} v (Precond::operator(), sap/precond.h:1370-1386)."""
+        _check(lib().sap_precond_apply(self.h, _dev_f64(v, "v"), _dev_f64(z, "z")), self.h)
+        return z
+
+    def getStats(self):
+        st = Stats()
+        _check(lib().sap_get_stats(self.h, ctypes.byref(st)), self.h)
+        return st
+
+    def getMonitorCode(self):
+        return lib().sap_monitor_code(self.h)
+
+    def getMonitorMessage(self):
+        return lib().sap_monitor_message(self.h).decode()
+
+    @property
+    def numPartitions(self):
+        return lib().sap_num_partitions(self.h)
+
+    def profile(self):
+        m = lib().sap_get_profile(self.h, None, 0)
+        _check(m, self.h)
+        arr = (KernelProfile * max(m, 1))()
+        lib().sap_get_profile(self.h, arr, m)
+        return {arr[j].name.decode(): (arr[j].launches, arr[j].total_ms) for j in range(m)}
+
+    def reset_profile(self):
+        lib().sap_reset_profile(self.h)
+
+    def factors(self):
+        import numpy as np
+        n, k = self._A.num_rows, self._A.m_k
+        out = np.empty((n, 2 * k + 1))
+        _check(lib().sap_get_factors(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size), self.h)
+        return out
+
+    def spikes(self):
+        """(WV, S): spike tips (P-1, 2, k, k) stored RHS-major and the factored
+        Schur blocks (P-1, k, k) stored column-major."""
+        import numpy as np
+        k = self._A.m_k
+        P = self.numPartitions
+        wv = np.empty((max(P - 1, 0), 2, k, k))
+        s = np.empty((max(P - 1, 0), k, k))
+        _check(lib().sap_get_spikes(self.h, wv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                    s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), wv.size), self.h)
+        return wv, s
+
+
+def _wrap(ptr, n, device):
+    """A float64 device tensor view of n doubles at ptr (for SPMV callbacks)."""
+    class _Arr:
+        def __init__(self, p, nn):
+            self.__cuda_array_interface__ = {"shape": (nn,), "typestr": "<f8", "data": (p, False), "version": 2}
+    return torch.as_tensor(_Arr(ptr, n), device=device)

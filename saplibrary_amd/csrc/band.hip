@@ -1,0 +1,184 @@
+// Band-format kernels: workload generator, FP64 banded SpMV, and the
+// row-major FP64 band -> column-major preconditioner band conversion.
+#include "sap_internal.hpp"
+
+namespace sap_amd {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// xoshiro256** (Blackman & Vigna), one stream per row seeded by splitmix64.
+// Same definition as the CPU generator used by the tests.
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t splitmix64(uint64_t& x)
+{
+    uint64_t z = (x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ inline uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+struct Xoshiro {
+    uint64_t s0, s1, s2, s3;
+    __device__ Xoshiro(uint64_t seed, uint64_t stream)
+    {
+        uint64_t x = seed ^ (stream * 0xD1B54A32D192ED03ULL);
+        s0 = splitmix64(x); s1 = splitmix64(x); s2 = splitmix64(x); s3 = splitmix64(x);
+    }
+    __device__ inline uint64_t next()
+    {
+        const uint64_t result = rotl64(s1 * 5, 7) * 9;
+        const uint64_t t = s1 << 17;
+        s2 ^= s0; s3 ^= s1; s1 ^= s2; s0 ^= s3; s2 ^= t;
+        s3 = rotl64(s3, 45);
+        return result;
+    }
+    __device__ inline double uniform_pm1() { return (double)(next() >> 11) * 0x1.0p-52 - 1.0; }
+};
+
+// One thread per row; the row's random values are drawn in column order.
+__global__ void gen_banded_kernel(int64_t n, int k, double dom, uint64_t seed, double* __restrict__ band)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int w = 2 * k + 1;
+    Xoshiro rng(seed, (uint64_t)i + 1);
+    double* row = band + i * w;
+    double s = 0.0;
+    for (int d = 0; d < w; ++d) {
+        if (d == k) continue;
+        const int64_t j = i + d - k;
+        if (j < 0 || j >= n) { row[d] = 0.0; continue; }
+        const double v = rng.uniform_pm1();
+        row[d] = v;
+        s += fabs(v);
+    }
+    row[k] = dom * s;
+}
+
+__global__ void gen_vector_kernel(int64_t n, uint64_t seed, double* __restrict__ x)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Xoshiro rng(seed, (uint64_t)i + 1);
+    x[i] = rng.uniform_pm1();
+}
+
+// ---------------------------------------------------------------------------
+// y = A x on the row-major FP64 band (the MVBanded functor, sap/spmv.h:27-74).
+// G lanes cooperate on one row: lane l sums diagonals l, l+G, ... in order,
+// then a fixed shuffle tree combines the G partial sums.
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ __launch_bounds__(256) void band_spmv_kernel(int64_t n, int k, const double* __restrict__ band,
+                                                        const double* __restrict__ x, double* __restrict__ y)
+{
+    const int w = 2 * k + 1;
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    double s = 0.0;
+    if (row < n) {
+        const double* r = band + row * w;
+        const int dlo = (int)(row < k ? k - row : 0);
+        const int64_t dhi64 = n - 1 - row + k;
+        const int dhi = (int)(dhi64 < w - 1 ? dhi64 : w - 1);
+        const double* xr = x + row - k;
+        for (int d = dlo + lane; d <= dhi; d += G) s = fma(r[d], xr[d], s);
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_down(s, off, G);
+    if (lane == 0 && row < n) y[row] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Column-major band from the row-major input: B[c*(2k+1) + d] = A(c+d-k, c),
+// cast to the preconditioner type (bandedMatrixTranspose,
+// sap/device/data_transfer.cuh:732-770).  The tile of TC columns x TD
+// diagonals is staged through LDS so both the reads (along input rows) and
+// the writes (along output columns) are contiguous.
+// ---------------------------------------------------------------------------
+constexpr int TC = 64;
+constexpr int TD = 32;
+
+template <class T>
+__global__ __launch_bounds__(256) void band_to_colmajor_kernel(int64_t n, int k, const double* __restrict__ in,
+                                                               T* __restrict__ out)
+{
+    __shared__ double tile[TC + TD][TD + 1];
+    const int w = 2 * k + 1;
+    const int64_t c0 = (int64_t)blockIdx.x * TC;
+    const int d0 = blockIdx.y * TD;
+    // input rows r = c + d - k for c in [c0, c0+TC), d in [d0, d0+TD):
+    // r in [c0 + d0 - k, c0 + d0 - k + TC + TD - 1), input offset o = 2k - d.
+    const int64_t rbase = c0 + d0 - k;
+    for (int e = threadIdx.x; e < (TC + TD - 1) * TD; e += blockDim.x) {
+        const int rr = e / TD;           // row within the box
+        const int oo = e % TD;           // offset index: o = 2k - d0 - (TD - 1) + oo
+        const int64_t r = rbase + rr;
+        const int o = 2 * k - d0 - (TD - 1) + oo;
+        double v = 0.0;
+        if (r >= 0 && r < n && o >= 0 && o < w) v = in[r * w + o];
+        tile[rr][oo] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < TC * TD; e += blockDim.x) {
+        const int cc = e / TD;
+        const int dd = e % TD;
+        const int64_t c = c0 + cc;
+        const int d = d0 + dd;
+        if (c >= n || d >= w) continue;
+        const int64_t r = c + d - k;
+        double v = 0.0;
+        if (r >= 0 && r < n) v = tile[cc + dd][(TD - 1) - dd];
+        out[c * w + d] = (T)v;
+    }
+}
+
+}  // namespace
+
+void launch_gen_banded(int64_t n, int k, double dom, uint64_t seed, double* band, hipStream_t st)
+{
+    const int bs = 256;
+    hipLaunchKernelGGL(gen_banded_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, st, n, k, dom, seed,
+                       band);
+    SAP_LAUNCH_CHECK();
+}
+
+void launch_gen_vector(int64_t n, uint64_t seed, double* x, hipStream_t st)
+{
+    const int bs = 256;
+    hipLaunchKernelGGL(gen_vector_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, st, n, seed, x);
+    SAP_LAUNCH_CHECK();
+}
+
+void launch_band_spmv(int64_t n, int k, const double* band, const double* x, double* y, hipStream_t st)
+{
+    const int w = 2 * k + 1;
+    const int bs = 256;
+    if (w >= 192) {
+        const int64_t blocks = (n * 64 + bs - 1) / bs;
+        hipLaunchKernelGGL(band_spmv_kernel<64>, dim3((unsigned)blocks), dim3(bs), 0, st, n, k, band, x, y);
+    } else if (w >= 48) {
+        const int64_t blocks = (n * 16 + bs - 1) / bs;
+        hipLaunchKernelGGL(band_spmv_kernel<16>, dim3((unsigned)blocks), dim3(bs), 0, st, n, k, band, x, y);
+    } else {
+        const int64_t blocks = (n * 4 + bs - 1) / bs;
+        hipLaunchKernelGGL(band_spmv_kernel<4>, dim3((unsigned)blocks), dim3(bs), 0, st, n, k, band, x, y);
+    }
+    SAP_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st)
+{
+    const int w = 2 * k + 1;
+    dim3 grid((unsigned)((n + TC - 1) / TC), (unsigned)((w + TD - 1) / TD));
+    hipLaunchKernelGGL(band_to_colmajor_kernel<T>, grid, dim3(256), 0, st, n, k, band_rm, B);
+    SAP_LAUNCH_CHECK();
+}
+
+template void launch_band_to_colmajor<float>(int64_t, int, const double*, float*, hipStream_t);
+template void launch_band_to_colmajor<double>(int64_t, int, const double*, double*, hipStream_t);
+
+}  // namespace sap_amd

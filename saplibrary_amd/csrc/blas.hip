@@ -1,0 +1,107 @@
+// FP64 vector kernels for the Krylov loops (the cusp::blas calls of
+// sap/bicgstab2.h, sap/minres.h and sap/monitor.h).  Reductions are
+// deterministic: a fixed grid writes per-block partial sums, and a single
+// block combines them in a fixed order.
+#include "sap_internal.hpp"
+
+namespace sap_amd {
+
+namespace {
+
+constexpr int RED_BLOCKS = 512;
+constexpr int RED_THREADS = 256;
+
+__device__ inline double block_reduce_sum(double v)
+{
+    __shared__ double warp_sums[RED_THREADS / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) warp_sums[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += warp_sums[i];
+    }
+    return s;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void dot_partial_kernel(int64_t n, const double* __restrict__ a,
+                                                                  const double* __restrict__ b,
+                                                                  double* __restrict__ partials)
+{
+    double s = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) s = fma(a[i], b[i], s);
+    s = block_reduce_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void reduce_final_kernel(int m, const double* __restrict__ partials,
+                                                                   double* __restrict__ out)
+{
+    double s = 0.0;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) s += partials[i];
+    s = block_reduce_sum(s);
+    if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void axpby_kernel(int64_t n, double alpha, const double* __restrict__ x, double beta,
+                             const double* __restrict__ y, double* __restrict__ z)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        z[i] = alpha * x[i] + beta * y[i];
+}
+
+__global__ void copy_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ y)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[i] = x[i];
+}
+
+__global__ void fill_kernel(int64_t n, double v, double* __restrict__ x)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = v;
+}
+
+inline unsigned grid_for(int64_t n)
+{
+    int64_t g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+int dot_partials_size() { return RED_BLOCKS; }
+
+void launch_dot(int64_t n, const double* a, const double* b, double* d_partials, double* d_out, hipStream_t st)
+{
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, a, b, d_partials);
+    SAP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(RED_THREADS), 0, st, RED_BLOCKS, d_partials, d_out);
+    SAP_LAUNCH_CHECK();
+}
+
+void launch_axpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* z, hipStream_t st)
+{
+    hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, alpha, x, beta, y, z);
+    SAP_LAUNCH_CHECK();
+}
+
+void launch_copy(int64_t n, const double* x, double* y, hipStream_t st)
+{
+    hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
+    SAP_LAUNCH_CHECK();
+}
+
+void launch_fill(int64_t n, double v, double* x, hipStream_t st)
+{
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, v, x);
+    SAP_LAUNCH_CHECK();
+}
+
+}  // namespace sap_amd

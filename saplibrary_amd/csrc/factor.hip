@@ -1,0 +1,543 @@
+// Setup kernels of the SaP preconditioner: banded LU/UL of the partitions,
+// coupling-block extraction, spike tips and the reduced (Schur) blocks.
+//
+// Band layout (column-major, as the reference's m_B, sap/device/
+// data_transfer.cuh:440-447): A(r, c) at B[c*(2k+1) + k + r - c].
+// A UL factorization of a partition is computed as the LU factorization of
+// the same storage read backwards: with X = (s+len)*(2k+1) - 1, the reversed
+// block's element (r, c) lives at X - (c*(2k+1) + k + r - c).  So one kernel
+// serves both directions.
+#include "sap_internal.hpp"
+
+namespace sap_amd {
+
+namespace {
+
+// Pivot boosting (sap/device/factor_band_const.cuh:29-40, thresholds
+// BURST_VALUE = 1e-7 and BURST_NEW_VALUE = 1e-4 of sap/common.h:46-47).
+template <class T>
+__device__ inline T boost_pivot(T v)
+{
+    const T thr = (T)1e-7, nv = (T)1e-4;
+    if (v > thr || v < -thr) return v;
+    return v < (T)0 ? -nv : nv;
+}
+
+// ---------------------------------------------------------------------------
+// Blocked right-looking banded LU of one partition per workgroup.
+// Panel of NB columns staged in LDS together with the NB x k block of U rows
+// to its right; the panel is factored in LDS, written back with the U rows
+// divided by their pivots (so the stored factors are L, D, U' with unit L and
+// U'), and the trailing k x k window is updated in global memory (L2) with
+// the rank-NB product from LDS.
+// ---------------------------------------------------------------------------
+template <class T, int NB>
+__global__ __launch_bounds__(256) void band_factor_kernel(T* __restrict__ B0, T* __restrict__ B1, int k,
+                                                          const FactorJob* __restrict__ jobs)
+{
+    const FactorJob job = jobs[blockIdx.x];
+    T* __restrict__ base = job.buf ? B1 : B0;
+    const int64_t w = 2 * k + 1;
+    const int64_t len = job.len;
+    const int dir = job.dir;
+    const int64_t origin = dir > 0 ? job.start * w : (job.start + len) * w - 1;
+    auto aidx = [&](int64_t r, int64_t c) -> int64_t { return origin + dir * (c * w + k + r - c); };
+
+    extern __shared__ unsigned char smem_raw[];
+    T* P = reinterpret_cast<T*>(smem_raw);   // [(NB + k) rows][PS]
+    constexpr int PS = NB + 1;
+    T* Ub = P + (size_t)(NB + k) * PS;       // [NB rows][k cols]
+    __shared__ T s_piv;
+
+    for (int64_t j0 = 0; j0 < len; j0 += NB) {
+        const int nb = (int)min((int64_t)NB, len - j0);
+        const int H = (int)min(len - j0, (int64_t)nb + k);
+        const int M = (int)min(len - j0 - nb, (int64_t)k);
+
+        for (int e = threadIdx.x; e < H * nb; e += blockDim.x) {
+            const int c = e / H, r = e % H;
+            T v = (T)0;
+            if (r - c <= k && c - r <= k) v = base[aidx(j0 + r, j0 + c)];
+            P[r * PS + c] = v;
+        }
+        for (int e = threadIdx.x; e < nb * M; e += blockDim.x) {
+            const int c = e / nb, r = e % nb;
+            T v = (T)0;
+            if (nb + c - r <= k) v = base[aidx(j0 + r, j0 + nb + c)];
+            Ub[r * k + c] = v;
+        }
+        __syncthreads();
+
+        for (int jj = 0; jj < nb; ++jj) {
+            if (threadIdx.x == 0) {
+                T p = P[jj * PS + jj];
+                if (job.boost && (j0 + jj < len - 1 || job.boost_last)) p = boost_pivot(p);
+                P[jj * PS + jj] = p;
+                s_piv = p;
+            }
+            __syncthreads();
+            const T piv = s_piv;
+            const int rmax = min(H - 1, jj + k);
+            for (int r = jj + 1 + (int)threadIdx.x; r <= rmax; r += blockDim.x) P[r * PS + jj] /= piv;
+            __syncthreads();
+            const int ncol = nb - 1 - jj;
+            const int nrow = rmax - jj;
+            for (int e = threadIdx.x; e < ncol * nrow; e += blockDim.x) {
+                const int c = jj + 1 + e / nrow, r = jj + 1 + e % nrow;
+                P[r * PS + c] -= P[r * PS + jj] * P[jj * PS + c];
+            }
+            const int nrow_u = nb - 1 - jj;
+            for (int e = threadIdx.x; e < nrow_u * M; e += blockDim.x) {
+                const int c = e / nrow_u, r = jj + 1 + e % nrow_u;
+                Ub[r * k + c] -= P[r * PS + jj] * Ub[jj * k + c];
+            }
+            __syncthreads();
+        }
+
+        // write back the factored panel (U entries divided by their pivot)
+        for (int e = threadIdx.x; e < H * nb; e += blockDim.x) {
+            const int c = e / H, r = e % H;
+            if (r - c <= k && c - r <= k) {
+                T v = P[r * PS + c];
+                if (r < c) v /= P[r * PS + r];
+                base[aidx(j0 + r, j0 + c)] = v;
+            }
+        }
+        for (int e = threadIdx.x; e < nb * M; e += blockDim.x) {
+            const int c = e / nb, r = e % nb;
+            if (nb + c - r <= k) base[aidx(j0 + r, j0 + nb + c)] = Ub[r * k + c] / P[r * PS + r];
+        }
+
+        // trailing update of the M x M window (band entries only)
+        const int cq_count = (M + 3) / 4;
+        for (int e = threadIdx.x; e < M * cq_count; e += blockDim.x) {
+            const int i = e % M;
+            const int c = 4 * (e / M);
+            T acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+            const T* prow = P + (size_t)(nb + i) * PS;
+            for (int jj = 0; jj < nb; ++jj) {
+                const T l = prow[jj];
+                const T* u = Ub + jj * k + c;
+                acc0 += l * u[0];
+                if (c + 1 < M) acc1 += l * u[1];
+                if (c + 2 < M) acc2 += l * u[2];
+                if (c + 3 < M) acc3 += l * u[3];
+            }
+            const T accs[4] = {acc0, acc1, acc2, acc3};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int cc = c + q;
+                if (cc < M && i - cc <= k && cc - i <= k) base[aidx(j0 + nb + i, j0 + nb + cc)] -= accs[q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <class T>
+__global__ void zero_pivot_kernel(const T* __restrict__ B, int k, int64_t row0, int64_t rows, int* flag)
+{
+    const int64_t w = 2 * k + 1;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += stride) {
+        const T d = B[(row0 + i) * w + k];
+        if (!(d >= (T)1e-7 || d <= (T)-1e-7)) atomicOr(flag, 1);   // |d| < 1e-7 (SmallerThan, precond.h:526-534)
+    }
+}
+
+// Coupling blocks of boundary i (copydWV, sap/device/data_transfer.cuh:127-199):
+// B_i(a,b) = A(e0-k+a, s1+b) (zero above the diagonal, b > a),
+// C_{i+1}(a,b) = A(s1+a, e0-k+b) (zero below the diagonal, a > b).
+// Stored column-major: offB[b*k + a].
+template <class T>
+__global__ void extract_coupling_kernel(const T* __restrict__ B, int k, PartInfo pi, T* __restrict__ offB,
+                                        T* __restrict__ offC)
+{
+    const int i = blockIdx.x;
+    int64_t s1, l1;
+    pi.bounds(i + 1, s1, l1);
+    const int64_t w = 2 * k + 1;
+    const size_t kk = (size_t)k * k;
+    for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+        const int b = e / k, a = e % k;
+        const int64_t rb = s1 - k + a, cb = s1 + b;        // B_i(a, b)
+        offB[i * kk + e] = (b <= a) ? B[cb * w + k + rb - cb] : (T)0;
+        const int64_t rc = s1 + a, cc = s1 - k + b;        // C_{i+1}(a, b)
+        offC[i * kk + e] = (a <= b) ? B[cc * w + k + rc - cc] : (T)0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-RHS triangular solves on k x k corner blocks, RC right-hand sides per
+// workgroup held in LDS as Y[a][r] (row stride RC+1).
+//   lower_first = 1:  Y <- U'^{-1} D^{-1} L^{-1} Y   (LU factors)
+//   lower_first = 0:  Y <- L'^{-1} D^{-1} U'^{-1} Y  (UL factors)
+// Fl(t,j) = L(t,j) for t > j, Fu(t,j) = U(t,j) for t < j, Fd(t) = pivot.
+// ---------------------------------------------------------------------------
+constexpr int RC = 32;
+
+template <class T, class FL, class FU, class FD>
+__device__ void corner_solve(T* Y, int k, int nrhs, bool lower_first, FL Fl, FU Fu, FD Fd)
+{
+    constexpr int YS = RC + 1;
+    auto lower = [&]() {
+        for (int j = 0; j < k - 1; ++j) {
+            const int nt = k - 1 - j;
+            for (int e = threadIdx.x; e < nt * nrhs; e += blockDim.x) {
+                const int t = j + 1 + e % nt, r = e / nt;
+                Y[t * YS + r] -= Fl(t, j) * Y[j * YS + r];
+            }
+            __syncthreads();
+        }
+    };
+    auto upper = [&]() {
+        for (int j = k - 1; j > 0; --j) {
+            const int nt = j;
+            for (int e = threadIdx.x; e < nt * nrhs; e += blockDim.x) {
+                const int t = e % nt, r = e / nt;
+                Y[t * YS + r] -= Fu(t, j) * Y[j * YS + r];
+            }
+            __syncthreads();
+        }
+    };
+    auto diag = [&]() {
+        for (int e = threadIdx.x; e < k * nrhs; e += blockDim.x) {
+            const int t = e % k, r = e / k;
+            Y[t * YS + r] /= Fd(t);
+        }
+        __syncthreads();
+    };
+    if (lower_first) { lower(); diag(); upper(); }
+    else             { upper(); diag(); lower(); }
+}
+
+// Spike tips (calculateSpikes, sap/precond.h:6230-6287) with L D U' / U' D L'
+// factors.  blockIdx.x = boundary i, blockIdx.y = 0 (V_i) / 1 (W_{i+1}),
+// blockIdx.z = RHS chunk.  V_i = bottom k x k block of A_i^{-1}[0; B_i] from
+// the LU of partition i; W_{i+1} = top k x k block of A_{i+1}^{-1}[C_{i+1}; 0]
+// from the UL of partition i+1.  WV[i][0|1][r*k + a].
+template <class T>
+__global__ __launch_bounds__(256) void spike_tips_kernel(const T* __restrict__ B, const T* __restrict__ Bul, int k,
+                                                         PartInfo pi, const T* __restrict__ offB,
+                                                         const T* __restrict__ offC, T* __restrict__ WV)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* Y = reinterpret_cast<T*>(smem_raw);
+    constexpr int YS = RC + 1;
+    const int i = blockIdx.x;
+    const int which = blockIdx.y;
+    const int r0 = blockIdx.z * RC;
+    const int nrhs = min(RC, k - r0);
+    const int64_t w = 2 * k + 1;
+    const size_t kk = (size_t)k * k;
+    int64_t s0, l0, s1, l1;
+    pi.bounds(i, s0, l0);
+    pi.bounds(i + 1, s1, l1);
+    const T* src = which == 0 ? offB + i * kk : offC + i * kk;
+    for (int e = threadIdx.x; e < k * nrhs; e += blockDim.x) {
+        const int a = e % k, r = e / k;
+        Y[a * YS + r] = src[(size_t)(r0 + r) * k + a];
+    }
+    __syncthreads();
+    if (which == 0) {
+        const int64_t o = s0 + l0 - k;   // first row of the bottom block
+        auto el = [&](int t, int j) -> T { const int64_t rr = o + t, cc = o + j; return B[cc * w + k + rr - cc]; };
+        corner_solve<T>(Y, k, nrhs, true, el, el, [&](int t) -> T { return el(t, t); });
+    } else {
+        const int64_t o = s1;
+        auto el = [&](int t, int j) -> T { const int64_t rr = o + t, cc = o + j; return Bul[cc * w + k + rr - cc]; };
+        corner_solve<T>(Y, k, nrhs, false, el, el, [&](int t) -> T { return el(t, t); });
+    }
+    T* dst = WV + (2 * i + which) * kk;
+    for (int e = threadIdx.x; e < k * nrhs; e += blockDim.x) {
+        const int a = e % k, r = e / k;
+        dst[(size_t)(r0 + r) * k + a] = Y[a * YS + r];
+    }
+}
+
+// W tips for LU_only: the full partition sweep with k right-hand sides
+// (calculateSpikes_const, sap/precond.h:5910-5948), RC columns per
+// workgroup, working vector in global scratch (len x RC, row-major).
+template <class T>
+__global__ __launch_bounds__(256) void spike_w_full_kernel(const T* __restrict__ B, int k, PartInfo pi,
+                                                           const T* __restrict__ offC, T* __restrict__ WV,
+                                                           T* __restrict__ scratch, int64_t scratch_stride)
+{
+    const int i = blockIdx.x;
+    const int r0 = blockIdx.y * RC;
+    const int nrhs = min(RC, k - r0);
+    const int64_t w = 2 * k + 1;
+    const size_t kk = (size_t)k * k;
+    int64_t s1, l1;
+    pi.bounds(i + 1, s1, l1);
+    T* Y = scratch + (size_t)(i * gridDim.y + blockIdx.y) * scratch_stride;
+    auto el = [&](int64_t t, int64_t j) -> T { const int64_t rr = s1 + t, cc = s1 + j; return B[cc * w + k + rr - cc]; };
+    for (int64_t e = threadIdx.x; e < l1 * RC; e += blockDim.x) {
+        const int64_t a = e / RC;
+        const int r = (int)(e % RC);
+        Y[e] = (a < k && r < nrhs) ? offC[i * kk + (size_t)(r0 + r) * k + a] : (T)0;
+    }
+    __syncthreads();
+    for (int64_t j = 0; j < l1 - 1; ++j) {
+        const int nt = (int)min((int64_t)k, l1 - 1 - j);
+        for (int e = threadIdx.x; e < nt * nrhs; e += blockDim.x) {
+            const int64_t t = j + 1 + e % nt;
+            const int r = e / nt;
+            Y[t * RC + r] -= el(t, j) * Y[j * RC + r];
+        }
+        __syncthreads();
+    }
+    for (int64_t e = threadIdx.x; e < l1 * nrhs; e += blockDim.x) {
+        const int64_t t = e / nrhs;
+        const int r = (int)(e % nrhs);
+        Y[t * RC + r] /= el(t, t);
+    }
+    __syncthreads();
+    for (int64_t j = l1 - 1; j > 0; --j) {
+        const int nt = (int)min((int64_t)k, j);
+        for (int e = threadIdx.x; e < nt * nrhs; e += blockDim.x) {
+            const int64_t t = j - 1 - e % nt;
+            const int r = e / nt;
+            Y[t * RC + r] -= el(t, j) * Y[j * RC + r];
+        }
+        __syncthreads();
+    }
+    T* dst = WV + (2 * i + 1) * kk;
+    for (int e = threadIdx.x; e < k * nrhs; e += blockDim.x) {
+        const int a = e % k, r = e / k;
+        dst[(size_t)(r0 + r) * k + a] = Y[(int64_t)a * RC + r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reduced blocks R_i = [[I, V_i], [W_{i+1}, I]] (assembleReducedMat,
+// data_transfer.cuh:14-80) are never formed: their LU needs only the Schur
+// complement S_i = I - W_{i+1} V_i (var::fullLU_sub_spec,
+// factor_band_var.cuh:734-748), factored in place, and the apply uses
+// S_i^{-1}.  All k x k blocks column-major.
+// ---------------------------------------------------------------------------
+constexpr int GT = 32;   // GEMM tile
+
+template <class T>
+__global__ __launch_bounds__(256) void schur_gemm_kernel(const T* __restrict__ WV, int k, T* __restrict__ S)
+{
+    __shared__ T Wt[GT][GT + 1];
+    __shared__ T Vt[GT][GT + 1];
+    const int i = blockIdx.x;
+    const int a0 = blockIdx.y * GT, b0 = blockIdx.z * GT;
+    const size_t kk = (size_t)k * k;
+    const T* V = WV + 2 * i * kk;
+    const T* W = WV + (2 * i + 1) * kk;
+    const int tr = threadIdx.x % GT;          // row within tile
+    const int tc = (threadIdx.x / GT) * 4;    // 4 columns per thread
+    T acc[4] = {0, 0, 0, 0};
+    for (int c0 = 0; c0 < k; c0 += GT) {
+        for (int e = threadIdx.x; e < GT * GT; e += blockDim.x) {
+            const int rr = e % GT, cc = e / GT;
+            const int a = a0 + rr, c = c0 + cc, b = b0 + rr;
+            Wt[rr][cc] = (a < k && c < k) ? W[(size_t)c * k + a] : (T)0;   // W(a, c)
+            Vt[cc][rr] = (c0 + cc < k && b < k) ? V[(size_t)b * k + (c0 + cc)] : (T)0;   // V(c, b) at [c][b]
+        }
+        __syncthreads();
+        for (int cc = 0; cc < GT; ++cc) {
+            const T wv = Wt[tr][cc];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] += wv * Vt[cc][tc + q];
+        }
+        __syncthreads();
+    }
+    T* Si = S + i * kk;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int a = a0 + tr, b = b0 + tc + q;
+        if (a < k && b < k) Si[(size_t)b * k + a] = (a == b ? (T)1 : (T)0) - acc[q];
+    }
+}
+
+// In-place LU of S_i (Doolittle, no pivoting; boosted pivots only with
+// safeFactorization as in var::blockedFullLU_phase1_safe_general), then the
+// U rows divided by their pivots (var::fullLU_post_divide).
+template <class T>
+__global__ __launch_bounds__(256) void schur_lu_kernel(T* __restrict__ S, int k, int safe)
+{
+    T* A = S + (size_t)blockIdx.x * k * k;
+    __shared__ T s_piv;
+    for (int j = 0; j < k; ++j) {
+        if (threadIdx.x == 0) {
+            T p = A[(size_t)j * k + j];
+            if (safe) p = boost_pivot(p);
+            A[(size_t)j * k + j] = p;
+            s_piv = p;
+        }
+        __syncthreads();
+        const T piv = s_piv;
+        for (int t = j + 1 + (int)threadIdx.x; t < k; t += blockDim.x) A[(size_t)j * k + t] /= piv;
+        __syncthreads();
+        const int m = k - 1 - j;
+        for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+            const int t = j + 1 + e % m, c = j + 1 + e / m;
+            A[(size_t)c * k + t] -= A[(size_t)j * k + t] * A[(size_t)c * k + j];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+        const int r = e % k, c = e / k;
+        if (r < c) A[(size_t)c * k + r] /= A[(size_t)r * k + r];
+    }
+}
+
+// S_i^{-1} from its L D U' factors: RC identity columns per workgroup.
+template <class T>
+__global__ __launch_bounds__(256) void schur_inverse_kernel(const T* __restrict__ S, int k, T* __restrict__ Sinv)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* Y = reinterpret_cast<T*>(smem_raw);
+    constexpr int YS = RC + 1;
+    const int i = blockIdx.x;
+    const int r0 = blockIdx.y * RC;
+    const int nrhs = min(RC, k - r0);
+    const T* A = S + (size_t)i * k * k;
+    for (int e = threadIdx.x; e < k * nrhs; e += blockDim.x) {
+        const int a = e % k, r = e / k;
+        Y[a * YS + r] = (a == r0 + r) ? (T)1 : (T)0;
+    }
+    __syncthreads();
+    auto el = [&](int t, int j) -> T { return A[(size_t)j * k + t]; };
+    corner_solve<T>(Y, k, nrhs, true, el, el, [&](int t) -> T { return el(t, t); });
+    T* dst = Sinv + (size_t)i * k * k;
+    for (int e = threadIdx.x; e < k * nrhs; e += blockDim.x) {
+        const int a = e % k, r = e / k;
+        dst[(size_t)(r0 + r) * k + a] = Y[a * YS + r];
+    }
+}
+
+template <class T>
+__global__ void count_zero_kernel(const T* __restrict__ B, int64_t count, int* out)
+{
+    int local = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) local += (B[i] == (T)0);
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
+    if ((threadIdx.x & 63) == 0 && local) atomicAdd(out, local);
+}
+
+template <class T>
+size_t factor_lds_bytes(int k, int nb)
+{
+    return sizeof(T) * ((size_t)(nb + k) * (nb + 1) + (size_t)nb * k);
+}
+
+template <class T, int NB>
+void launch_factor_nb(T* B0, T* B1, int k, const FactorJob* jobs, int njobs, hipStream_t st)
+{
+    const size_t lds = factor_lds_bytes<T>(k, NB);
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_factor_kernel<T, NB>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((band_factor_kernel<T, NB>), dim3(njobs), dim3(256), lds, st, B0, B1, k, jobs);
+    SAP_LAUNCH_CHECK();
+}
+
+constexpr size_t LDS_LIMIT = 160 * 1024 - 1024;   // leave room for static LDS
+
+}  // namespace
+
+template <class T>
+void launch_band_factor(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, hipStream_t st)
+{
+    if (njobs <= 0) return;
+    if (factor_lds_bytes<T>(k, 32) <= LDS_LIMIT)      launch_factor_nb<T, 32>(B0, B1, k, d_jobs, njobs, st);
+    else if (factor_lds_bytes<T>(k, 16) <= LDS_LIMIT) launch_factor_nb<T, 16>(B0, B1, k, d_jobs, njobs, st);
+    else if (factor_lds_bytes<T>(k, 8) <= LDS_LIMIT)  launch_factor_nb<T, 8>(B0, B1, k, d_jobs, njobs, st);
+    else throw Error(SAP_ERR_UNSUPPORTED, "half-bandwidth too large for the banded factorization kernel");
+}
+
+template <class T>
+void launch_zero_pivot_check(const T* B, int k, int64_t row0, int64_t rows, int* d_flag, hipStream_t st)
+{
+    if (rows <= 0) return;
+    int64_t g = (rows + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(zero_pivot_kernel<T>, dim3((unsigned)g), dim3(256), 0, st, B, k, row0, rows, d_flag);
+    SAP_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_extract_coupling(const T* B, int k, PartInfo pi, T* offB, T* offC, hipStream_t st)
+{
+    if (pi.P < 2) return;
+    hipLaunchKernelGGL(extract_coupling_kernel<T>, dim3(pi.P - 1), dim3(256), 0, st, B, k, pi, offB, offC);
+    SAP_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_spike_tips(const T* B, const T* Bul, int k, PartInfo pi, int fact_lu_ul, const T* offB, const T* offC,
+                       T* WV, hipStream_t st)
+{
+    if (pi.P < 2) return;
+    const size_t lds = sizeof(T) * (size_t)k * (RC + 1);
+    if (lds > LDS_LIMIT) throw Error(SAP_ERR_UNSUPPORTED, "half-bandwidth too large for the spike kernel");
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)spike_tips_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    const unsigned chunks = (unsigned)((k + RC - 1) / RC);
+    if (fact_lu_ul) {
+        hipLaunchKernelGGL(spike_tips_kernel<T>, dim3(pi.P - 1, 2, chunks), dim3(256), lds, st, B, Bul, k, pi, offB,
+                           offC, WV);
+        SAP_LAUNCH_CHECK();
+    } else {
+        hipLaunchKernelGGL(spike_tips_kernel<T>, dim3(pi.P - 1, 1, chunks), dim3(256), lds, st, B, Bul, k, pi, offB,
+                           offC, WV);
+        SAP_LAUNCH_CHECK();
+        const int64_t maxlen = pi.n / pi.P + 1;
+        const int64_t stride = maxlen * RC;
+        DevBuf<T> scratch((size_t)stride * (pi.P - 1) * chunks);
+        hipLaunchKernelGGL(spike_w_full_kernel<T>, dim3(pi.P - 1, chunks), dim3(256), 0, st, B, k, pi, offC, WV,
+                           scratch.get(), stride);
+        SAP_LAUNCH_CHECK();
+        SAP_HIP_CHECK(hipStreamSynchronize(st));   // scratch is released on return
+    }
+}
+
+template <class T>
+void launch_reduced_factor(const T* WV, int k, int nb, T* S, T* Sinv, int safe, hipStream_t st)
+{
+    if (nb <= 0) return;
+    const unsigned tiles = (unsigned)((k + GT - 1) / GT);
+    hipLaunchKernelGGL(schur_gemm_kernel<T>, dim3(nb, tiles, tiles), dim3(256), 0, st, WV, k, S);
+    SAP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(schur_lu_kernel<T>, dim3(nb), dim3(256), 0, st, S, k, safe);
+    SAP_LAUNCH_CHECK();
+    const size_t lds = sizeof(T) * (size_t)k * (RC + 1);
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)schur_inverse_kernel<T>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(schur_inverse_kernel<T>, dim3(nb, (k + RC - 1) / RC), dim3(256), lds, st, S, k, Sinv);
+    SAP_LAUNCH_CHECK();
+}
+
+template <class T>
+int64_t count_band_zeros(const T* B, int64_t count, int* d_tmp, hipStream_t st)
+{
+    SAP_HIP_CHECK(hipMemsetAsync(d_tmp, 0, sizeof(int), st));
+    int64_t g = (count + 255) / 256;
+    if (g > 8192) g = 8192;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(count_zero_kernel<T>, dim3((unsigned)g), dim3(256), 0, st, B, count, d_tmp);
+    SAP_LAUNCH_CHECK();
+    int h = 0;
+    SAP_HIP_CHECK(hipMemcpyAsync(&h, d_tmp, sizeof(int), hipMemcpyDeviceToHost, st));
+    SAP_HIP_CHECK(hipStreamSynchronize(st));
+    return h;
+}
+
+#define SAP_INST(T)                                                                                          \
+    template int64_t count_band_zeros<T>(const T*, int64_t, int*, hipStream_t);                               \
+    template void launch_band_factor<T>(T*, T*, int, const FactorJob*, int, hipStream_t);                     \
+    template void launch_zero_pivot_check<T>(const T*, int, int64_t, int64_t, int*, hipStream_t);             \
+    template void launch_extract_coupling<T>(const T*, int, PartInfo, T*, T*, hipStream_t);                   \
+    template void launch_spike_tips<T>(const T*, const T*, int, PartInfo, int, const T*, const T*, T*,        \
+                                       hipStream_t);                                                          \
+    template void launch_reduced_factor<T>(const T*, int, int, T*, T*, int, hipStream_t);
+SAP_INST(float)
+SAP_INST(double)
+#undef SAP_INST
+
+}  // namespace sap_amd

@@ -1,0 +1,227 @@
+// Setup and apply of the SaP preconditioner for the constant-bandwidth
+// banded path (sap/precond.h:910-1362 setup, :1370-1528 apply).
+#include "precond.hpp"
+
+#include <algorithm>
+
+namespace sap_amd {
+
+template <class T>
+double Precond<T>::elapsed_ms(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    SAP_HIP_CHECK(hipEventSynchronize(b));
+    SAP_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    return (double)ms;
+}
+
+template <class T>
+void Precond<T>::factor(const std::vector<FactorJob>& jobs)
+{
+    if (jobs.empty()) return;
+    jobs_.resize(jobs.size());
+    SAP_HIP_CHECK(hipMemcpyAsync(jobs_.get(), jobs.data(), jobs.size() * sizeof(FactorJob), hipMemcpyHostToDevice,
+                                 stream));
+    ProfScope ps(prof, "band_factor");
+    launch_band_factor<T>(B_.get(), Bul_.get(), k, jobs_.get(), (int)jobs.size(), stream);
+}
+
+template <class T>
+bool Precond<T>::zero_pivots(const T* band, int64_t row0, int64_t rows)
+{
+    flag_.resize(1);
+    SAP_HIP_CHECK(hipMemsetAsync(flag_.get(), 0, sizeof(int), stream));
+    launch_zero_pivot_check<T>(band, k, row0, rows, flag_.get(), stream);
+    int h = 0;
+    SAP_HIP_CHECK(hipMemcpyAsync(&h, flag_.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
+    SAP_HIP_CHECK(hipStreamSynchronize(stream));
+    return h != 0;
+}
+
+template <class T>
+int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
+{
+    n = n_;
+    k = k_;
+    time_toBanded = time_offDiags = time_bandLU = time_bandUL = time_assembly = time_fullLU = 0;
+    // Partition clamp of convertToBandedMatrix (sap/precond.h:2116-2117):
+    // each partition needs k+1 <= n_p and 2k <= n_p.
+    const int64_t maxP = std::max<int64_t>(1, n / std::max(k + 1, 2 * k));
+    P = (int)std::min<int64_t>(cfg.numPartitions, maxP);
+    if (P < 1) P = 1;
+    const PartInfo pi{n, P};
+    const int64_t w = 2 * k + 1;
+    flag_.resize(1);
+
+    hipEvent_t e0, e1, e2, e3, e4, e5;
+    SAP_HIP_CHECK(hipEventCreate(&e0));
+    SAP_HIP_CHECK(hipEventCreate(&e1));
+    SAP_HIP_CHECK(hipEventCreate(&e2));
+    SAP_HIP_CHECK(hipEventCreate(&e3));
+    SAP_HIP_CHECK(hipEventCreate(&e4));
+    SAP_HIP_CHECK(hipEventCreate(&e5));
+    struct EvGuard {
+        hipEvent_t* e;
+        ~EvGuard() { for (int i = 0; i < 6; ++i) (void)hipEventDestroy(e[i]); }
+    };
+    hipEvent_t evs[6] = {e0, e1, e2, e3, e4, e5};
+    EvGuard guard{evs};
+
+    // (1) column-major band in the preconditioner precision
+    SAP_HIP_CHECK(hipEventRecord(e0, stream));
+    B_.resize((size_t)w * n);
+    {
+        ProfScope ps(prof, "band_to_colmajor");
+        launch_band_to_colmajor<T>(n, k, d_band, B_.get(), stream);
+    }
+    SAP_HIP_CHECK(hipEventRecord(e1, stream));
+    time_toBanded = elapsed_ms(e0, e1);
+
+    Bul_.release();
+    offB_.release(); offC_.release(); WV_.release(); S_.release(); Sinv_.release(); corr_.release();
+    g_.resize(n);
+    scr0_.resize(n);
+    scr1_.release();
+
+    if (k == 0 || cfg.precondType == SAP_None) return 0;
+
+    const int boost = (k >= 70) || cfg.safeFactorization;   // sap/precond.h:4363-4397
+    const int boost_last = (k >= 70);
+
+    if (cfg.precondType == SAP_Block || P == 1) {
+        std::vector<FactorJob> jobs;
+        for (int p = 0; p < P; ++p) {
+            int64_t s, len;
+            pi.bounds(p, s, len);
+            jobs.push_back({s, len, 1, boost, boost_last, 0});
+        }
+        SAP_HIP_CHECK(hipEventRecord(e2, stream));
+        factor(jobs);
+        SAP_HIP_CHECK(hipEventRecord(e3, stream));
+        time_bandLU = elapsed_ms(e2, e3);
+        if (!cfg.safeFactorization && zero_pivots(B_.get(), 0, n)) return SAP_ERR_ZERO_PIVOT;
+        actual_nnz = (int64_t)w * n - count_band_zeros<T>(B_.get(), (int64_t)w * n, flag_.get(), stream);
+        return 0;
+    }
+
+    // (2) coupling blocks, extracted before factorization
+    const size_t kk = (size_t)k * k;
+    offB_.resize(kk * (P - 1));
+    offC_.resize(kk * (P - 1));
+    WV_.resize(2 * kk * (P - 1));
+    SAP_HIP_CHECK(hipEventRecord(e1, stream));
+    {
+        ProfScope ps(prof, "extract_coupling");
+        launch_extract_coupling<T>(B_.get(), k, pi, offB_.get(), offC_.get(), stream);
+    }
+    SAP_HIP_CHECK(hipEventRecord(e2, stream));
+
+    // (3) factorization: LU of partitions 0..P-2 (LU_UL) or 0..P-1 (LU_only);
+    //     for LU_UL also UL of partitions 1..P-1 on a copy of the band, in the
+    //     same launch.
+    std::vector<FactorJob> jobs;
+    int64_t s_last, l_last;
+    pi.bounds(P - 1, s_last, l_last);
+    if (lu_ul()) {
+        Bul_.resize((size_t)w * n);
+        SAP_HIP_CHECK(hipMemcpyAsync(Bul_.get(), B_.get(), sizeof(T) * w * n, hipMemcpyDeviceToDevice, stream));
+        for (int p = 0; p < P - 1; ++p) {
+            int64_t s, len;
+            pi.bounds(p, s, len);
+            jobs.push_back({s, len, 1, boost, boost_last, 0});
+        }
+        for (int p = 1; p < P; ++p) {
+            int64_t s, len;
+            pi.bounds(p, s, len);
+            jobs.push_back({s, len, -1, boost, boost_last, 1});
+        }
+        scr1_.resize(n);
+    } else {
+        for (int p = 0; p < P; ++p) {
+            int64_t s, len;
+            pi.bounds(p, s, len);
+            jobs.push_back({s, len, 1, boost, boost_last, 0});
+        }
+    }
+    factor(jobs);
+    SAP_HIP_CHECK(hipEventRecord(e3, stream));
+    time_offDiags = elapsed_ms(e1, e2);
+    time_bandLU = elapsed_ms(e2, e3);
+    if (!cfg.safeFactorization) {
+        if (lu_ul()) {
+            int64_t s1, l1;
+            pi.bounds(1, s1, l1);
+            if (zero_pivots(B_.get(), 0, s_last) || zero_pivots(Bul_.get(), s1, n - s1)) return SAP_ERR_ZERO_PIVOT;
+        } else if (zero_pivots(B_.get(), 0, n)) {
+            return SAP_ERR_ZERO_PIVOT;
+        }
+    }
+
+    // (4) spike tips
+    SAP_HIP_CHECK(hipEventRecord(e3, stream));
+    {
+        ProfScope ps(prof, "spike_tips");
+        launch_spike_tips<T>(B_.get(), Bul_.get(), k, pi, lu_ul() ? 1 : 0, offB_.get(), offC_.get(), WV_.get(),
+                             stream);
+    }
+    SAP_HIP_CHECK(hipEventRecord(e4, stream));
+
+    // (5) reduced blocks: S_i = I - W V_i, LU, inverse
+    S_.resize(kk * (P - 1));
+    Sinv_.resize(kk * (P - 1));
+    corr_.resize((size_t)2 * k * (P - 1));
+    {
+        ProfScope ps(prof, "reduced_factor");
+        launch_reduced_factor<T>(WV_.get(), k, P - 1, S_.get(), Sinv_.get(), cfg.safeFactorization, stream);
+    }
+    SAP_HIP_CHECK(hipEventRecord(e5, stream));
+    time_assembly = elapsed_ms(e3, e4);
+    time_fullLU = elapsed_ms(e4, e5);
+    // non-zeros of the factored band (sap/precond.h:1352): the LU part and,
+    // for LU_UL, the UL-factored last partition
+    int64_t zeros = count_band_zeros<T>(B_.get(), (int64_t)w * (lu_ul() ? s_last : n), flag_.get(), stream);
+    if (lu_ul())
+        zeros += count_band_zeros<T>(Bul_.get() + (size_t)w * s_last, (int64_t)w * l_last, flag_.get(), stream);
+    actual_nnz = (int64_t)w * n - zeros;
+    return 0;
+}
+
+template <class T>
+void Precond<T>::apply(const double* v, double* z)
+{
+    const PartInfo pi{n, P};
+    if (cfg.precondType == SAP_None) {
+        launch_copy(n, v, z, stream);
+        return;
+    }
+    if (k == 0) {
+        ProfScope ps(prof, "diag_divide");
+        launch_diag_divide<T>(B_.get(), n, v, z, stream);
+        return;
+    }
+    if (spike_active()) {
+        if (lu_ul()) {
+            ProfScope ps(prof, "sweep_interface");
+            launch_sweep_interface<T>(B_.get(), Bul_.get(), k, pi, v, g_.get(), scr0_.get(), scr1_.get(), stream);
+        } else {
+            ProfScope ps(prof, "sweep_interface");
+            launch_sweep_full<T>(B_.get(), B_.get(), k, pi, 0, v, nullptr, nullptr, g_.get(), scr0_.get(), stream);
+        }
+        {
+            ProfScope ps(prof, "interface_solve");
+            launch_interface_solve<T>(WV_.get(), Sinv_.get(), offB_.get(), offC_.get(), k, pi, g_.get(), corr_.get(),
+                                      stream);
+        }
+        ProfScope ps(prof, "sweep_full");
+        launch_sweep_full<T>(B_.get(), Bul_.get(), k, pi, lu_ul() ? 1 : 0, v, corr_.get(), z, nullptr, scr0_.get(),
+                             stream);
+    } else {
+        ProfScope ps(prof, "sweep_full");
+        launch_sweep_full<T>(B_.get(), B_.get(), k, pi, 0, v, nullptr, z, nullptr, scr0_.get(), stream);
+    }
+}
+
+template class Precond<float>;
+template class Precond<double>;
+
+}  // namespace sap_amd

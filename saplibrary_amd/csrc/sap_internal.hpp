@@ -1,0 +1,161 @@
+// Internal declarations shared by the HIP translation units of libsap_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/sap_c.h"
+
+namespace sap_amd {
+
+// Error carried from a failing call to the C ABI, which turns it into a code.
+struct Error : public std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define SAP_HIP_CHECK(expr)                                                              \
+    do {                                                                                 \
+        hipError_t _e = (expr);                                                          \
+        if (_e != hipSuccess) {                                                          \
+            throw ::sap_amd::Error(_e == hipErrorOutOfMemory ? SAP_ERR_OOM : SAP_ERR_HIP, \
+                                   std::string(#expr) + ": " + hipGetErrorString(_e));    \
+        }                                                                                \
+    } while (0)
+
+#define SAP_LAUNCH_CHECK() SAP_HIP_CHECK(hipGetLastError())
+
+// Owning device allocation.
+template <class T>
+class DevBuf {
+public:
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { resize(n); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    void resize(size_t n)
+    {
+        if (n == n_) return;
+        release();
+        if (n) SAP_HIP_CHECK(hipMalloc(&p_, n * sizeof(T)));
+        n_ = n;
+    }
+    void release()
+    {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T* get() const { return p_; }
+    size_t size() const { return n_; }
+
+private:
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Partition p of P over n rows: the first n mod P partitions get one more row
+// (sap/device/factor_band_const.cuh:1432-1438).
+struct PartInfo {
+    int64_t n;
+    int P;
+    __host__ __device__ inline void bounds(int p, int64_t& s, int64_t& len) const
+    {
+        const int64_t q = n / P, r = n % P;
+        if (p < r) { s = p * (q + 1); len = q + 1; }
+        else       { s = p * q + r;   len = q; }
+    }
+};
+
+// Kernel-family timer (HIP events on the solver stream), used when profiling.
+class Profiler {
+public:
+    struct Entry { std::string name; int launches = 0; double ms = 0; };
+    bool enabled = false;
+    hipStream_t stream = nullptr;
+    void begin(const char* name);
+    void end();
+    void flush();          // resolve pending event pairs into totals (syncs)
+    void reset();
+    std::vector<Entry> entries;
+    ~Profiler();
+
+private:
+    struct Pending { int idx; hipEvent_t a, b; };
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+    int cur_ = -1;
+    hipEvent_t cur_a_ = nullptr;
+    hipEvent_t get_event();
+};
+
+struct ProfScope {
+    Profiler* p;
+    ProfScope(Profiler* p_, const char* name) : p(p_) { if (p && p->enabled) p->begin(name); }
+    ~ProfScope() { if (p && p->enabled) p->end(); }
+};
+
+// ---------------------------------------------------------------- launchers
+// band.hip
+void launch_gen_banded(int64_t n, int k, double dom, uint64_t seed, double* band, hipStream_t st);
+void launch_gen_vector(int64_t n, uint64_t seed, double* x, hipStream_t st);
+void launch_band_spmv(int64_t n, int k, const double* band, const double* x, double* y, hipStream_t st);
+template <class T>
+void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st);
+
+// blas.hip (FP64 vectors, results of reductions left in device memory)
+void launch_dot(int64_t n, const double* a, const double* b, double* d_partials, double* d_out, hipStream_t st);
+void launch_axpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* z, hipStream_t st);
+void launch_copy(int64_t n, const double* x, double* y, hipStream_t st);
+void launch_fill(int64_t n, double v, double* x, hipStream_t st);
+int  dot_partials_size();
+
+// factor.hip
+struct FactorJob {          // one banded LU (dir = +1) or UL (dir = -1) of a partition
+    int64_t start;          // first row of the partition
+    int64_t len;            // rows
+    int     dir;
+    int     boost;          // boost pivots
+    int     boost_last;     // boost also the partition's final pivot
+    int     buf;            // 0: main factor band, 1: UL band
+};
+template <class T>
+void launch_band_factor(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, hipStream_t st);
+template <class T>
+void launch_zero_pivot_check(const T* B, int k, int64_t row0, int64_t rows, int* d_flag, hipStream_t st);
+template <class T>
+void launch_extract_coupling(const T* B, int k, PartInfo pi, T* offB, T* offC, hipStream_t st);
+template <class T>
+void launch_spike_tips(const T* B, const T* Bul, int k, PartInfo pi, int fact_lu_ul,
+                       const T* offB, const T* offC, T* WV, hipStream_t st);
+template <class T>
+void launch_reduced_factor(const T* WV, int k, int nb, T* S, T* Sinv, int safe, hipStream_t st);
+
+// sweep.hip
+template <class T>
+void launch_sweep_interface(const T* B, const T* Bul, int k, PartInfo pi, const double* v, T* g, T* scr0, T* scr1,
+                            hipStream_t st);
+template <class T>
+void launch_interface_solve(const T* WV, const T* Sinv, const T* offB, const T* offC, int k, PartInfo pi,
+                            const T* g, T* corr, hipStream_t st);
+template <class T>
+void launch_sweep_full(const T* B, const T* Bul, int k, PartInfo pi, int last_is_ul, const double* v, const T* corr,
+                       double* outd, T* outt, T* scr, hipStream_t st);
+template <class T>
+void launch_diag_divide(const T* B, int64_t n, const double* v, double* out, hipStream_t st);
+template <class T>
+int64_t count_band_zeros(const T* B, int64_t count, int* d_tmp, hipStream_t st);
+
+}  // namespace sap_amd

@@ -1,0 +1,888 @@
+// Host solver: the Krylov methods and monitors (sap/bicgstab2.h,
+// sap/minres.h, sap/monitor.h), the solver state machine (sap/solver.h) and
+// the extern "C" entry points declared in include/sap_c.h.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+
+#include "precond.hpp"
+
+namespace sap_amd {
+
+// --------------------------------------------------------------- Profiler
+Profiler::~Profiler()
+{
+    for (auto& p : pending_) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : pool_) (void)hipEventDestroy(e);
+}
+hipEvent_t Profiler::get_event()
+{
+    if (!pool_.empty()) { hipEvent_t e = pool_.back(); pool_.pop_back(); return e; }
+    hipEvent_t e;
+    SAP_HIP_CHECK(hipEventCreate(&e));
+    return e;
+}
+void Profiler::begin(const char* name)
+{
+    int idx = -1;
+    for (size_t i = 0; i < entries.size(); ++i)
+        if (entries[i].name == name) { idx = (int)i; break; }
+    if (idx < 0) { entries.push_back({name, 0, 0.0}); idx = (int)entries.size() - 1; }
+    cur_ = idx;
+    cur_a_ = get_event();
+    SAP_HIP_CHECK(hipEventRecord(cur_a_, stream));
+}
+void Profiler::end()
+{
+    if (cur_ < 0) return;
+    hipEvent_t b = get_event();
+    SAP_HIP_CHECK(hipEventRecord(b, stream));
+    pending_.push_back({cur_, cur_a_, b});
+    entries[cur_].launches++;
+    cur_ = -1;
+    if (pending_.size() > 4096) flush();
+}
+void Profiler::flush()
+{
+    for (auto& p : pending_) {
+        float ms = 0.f;
+        SAP_HIP_CHECK(hipEventSynchronize(p.b));
+        SAP_HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+        entries[p.idx].ms += ms;
+        pool_.push_back(p.a);
+        pool_.push_back(p.b);
+    }
+    pending_.clear();
+}
+void Profiler::reset()
+{
+    flush();
+    entries.clear();
+}
+
+namespace {
+
+double wall_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// BiCGStabLMonitor (sap/monitor.h:153-335)
+struct BiCGStabLMonitor {
+    int    maxIterations = 100;
+    float  iterations = 0.f;
+    int    maxmSteps = 8;
+    int    stag = 0, moreSteps = 0, maxStagSteps = 3;
+    double relTol = 1e-6, absTol = 0, rhsNorm = 0, rNorm = std::numeric_limits<double>::max();
+    int    code = 0;
+    std::string message;
+
+    void   reset(int maxIt, int maxm, double rt, double at)
+    {
+        maxIterations = maxIt; maxmSteps = maxm; relTol = rt; absTol = at;
+        iterations = 0.f; stag = 0; moreSteps = 0; code = 0; message.clear();
+        rNorm = std::numeric_limits<double>::max();
+    }
+    void   init(double rhs_norm) { rhsNorm = rhs_norm; iterations = 0; code = 0; message = ""; }
+    double tolerance() const { return absTol + relTol * rhsNorm; }
+    void   stop(int c, const char* m) { code = c; message = m; }
+    void   increment(float x) { iterations += x; }
+    void   incrementStag() { stag++; }
+    void   resetStag() { stag = 0; }
+    bool   finished()
+    {
+        if (code != 0) return true;
+        if (iterations > maxIterations) stop(-1, "Maximum number of iterations was reached");
+        return code != 0;
+    }
+    bool   finished(double r)
+    {
+        rNorm = r;
+        if (code != 0) return true;
+        if (std::isnan(rNorm)) stop(-2, "Residual norm is NaN");
+        else if (rNorm <= tolerance()) stop(1, "Converged");
+        else if (iterations > maxIterations) stop(-1, "Maximum number of iterations was reached");
+        else {
+            if (stag >= maxStagSteps && moreSteps == 0) stag = 0;
+            moreSteps++;
+            if (moreSteps >= maxmSteps) stop(-3, "Stagnation encountered");
+        }
+        return code != 0;
+    }
+    bool   needCheckConvergence(double r)
+    {
+        rNorm = r;
+        if (code != 0) return false;
+        if (std::isnan(rNorm)) stop(-2, "Residual norm is NaN");
+        return code == 0 && (rNorm <= tolerance() || stag >= maxStagSteps || moreSteps);
+    }
+    bool   converged() const { return code > 0; }
+};
+
+// Monitor (sap/monitor.h:25-147)
+struct Monitor {
+    int    maxIterations = 100;
+    float  iterations = 0.f;
+    double relTol = 1e-6, absTol = 0, rhsNorm = 0, rNorm = std::numeric_limits<double>::max();
+    int    code = 0;
+    std::string message;
+    void   reset(int maxIt, double rt, double at)
+    {
+        maxIterations = maxIt; relTol = rt; absTol = at; iterations = 0.f; code = 0; message.clear();
+        rNorm = std::numeric_limits<double>::max();
+    }
+    void   init(double rhs_norm) { rhsNorm = rhs_norm; iterations = 0; code = 0; message = ""; }
+    double tolerance() const { return absTol + relTol * rhsNorm; }
+    void   stop(int c, const char* m) { code = c; message = m; }
+    size_t iteration_count() const { return (size_t)(iterations + 0.5f); }
+    bool   finished(double r)
+    {
+        rNorm = r;
+        if (code != 0) return true;
+        if (std::isnan(rNorm)) stop(-2, "Residual norm is NaN");
+        else if (rNorm <= tolerance()) stop(1, "Converged");
+        else if (iterations > maxIterations) stop(-1, "Maximum number of iterations was reached");
+        return code != 0;
+    }
+    bool   converged() const { return code > 0; }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ Solver
+class Solver {
+public:
+    sap_options opt;
+    int P_req;
+    hipStream_t stream = nullptr;
+    Profiler prof;
+    std::unique_ptr<Precond<float>> pf;
+    std::unique_ptr<Precond<double>> pd;
+    bool setup_done = false;
+    int64_t n = 0;
+    int k = 0;
+    const double* d_band = nullptr;     // non-owning (the MVBanded reference)
+    sap_spmv_fn user_spmv = nullptr;
+    void* user_ctx = nullptr;
+    BiCGStabLMonitor bmon;
+    Monitor mmon;
+    sap_stats stats;
+    std::string last_error;
+    int code = 0;
+    std::string message;
+
+    DevBuf<double> work_, partials_, dscalar_;
+    double* hscalar_ = nullptr;
+    std::vector<double*> vecs_;
+    size_t vec_next_ = 0;
+
+    Solver(int P, const sap_options& o) : opt(o), P_req(P)
+    {
+        std::memset(&stats, 0, sizeof(stats));
+        stats.rhsNorm = stats.residualNorm = stats.relResidualNorm = std::numeric_limits<double>::max();
+        SAP_HIP_CHECK(hipHostMalloc((void**)&hscalar_, sizeof(double) * 8, hipHostMallocDefault));
+        prof.enabled = opt.profile != 0;
+        make_precond();
+    }
+    ~Solver()
+    {
+        if (hscalar_) (void)hipHostFree(hscalar_);
+    }
+
+    void make_precond()
+    {
+        PrecondConfig c;
+        c.numPartitions = P_req;
+        c.factMethod = opt.factMethod;
+        c.precondType = opt.precondType;
+        c.safeFactorization = opt.safeFactorization;
+        if (opt.precValueType == SAP_PREC_DOUBLE) {
+            pd = std::make_unique<Precond<double>>();
+            pd->cfg = c; pd->stream = stream; pd->prof = &prof;
+            pf.reset();
+        } else {
+            pf = std::make_unique<Precond<float>>();
+            pf->cfg = c; pf->stream = stream; pf->prof = &prof;
+            pd.reset();
+        }
+    }
+    void set_stream(hipStream_t s)
+    {
+        stream = s;
+        prof.stream = s;
+        if (pf) pf->stream = s;
+        if (pd) pd->stream = s;
+    }
+    int num_partitions() const { return pf ? pf->P : (pd ? pd->P : 0); }
+
+    // ---------------------------------------------------------- vectors
+    void reserve_vectors(int count)
+    {
+        const size_t need = (size_t)count * n;
+        if (work_.size() < need) work_.resize(need);
+        vecs_.clear();
+        for (int i = 0; i < count; ++i) vecs_.push_back(work_.get() + (size_t)i * n);
+        vec_next_ = 0;
+        partials_.resize(dot_partials_size());
+        dscalar_.resize(8);
+    }
+    double* vec() { return vecs_.at(vec_next_++); }
+
+    double dot(const double* a, const double* b)
+    {
+        launch_dot(n, a, b, partials_.get(), dscalar_.get(), stream);
+        SAP_HIP_CHECK(hipMemcpyAsync(hscalar_, dscalar_.get(), sizeof(double), hipMemcpyDeviceToHost, stream));
+        SAP_HIP_CHECK(hipStreamSynchronize(stream));
+        return hscalar_[0];
+    }
+    double nrm2(const double* a) { return std::sqrt(dot(a, a)); }
+    void axpby(const double* x, const double* y, double* z, double alpha, double beta)
+    {
+        launch_axpby(n, alpha, x, beta, y, z, stream);
+    }
+    void axpy(const double* x, double* y, double alpha) { launch_axpby(n, alpha, x, 1.0, y, y, stream); }
+    void copy(const double* x, double* y) { launch_copy(n, x, y, stream); }
+
+    void spmv(const double* x, double* y)
+    {
+        stats.numSpmv++;
+        ProfScope ps(&prof, "spmv");
+        if (user_spmv) user_spmv(user_ctx, x, y, (int)n);
+        else launch_band_spmv(n, k, d_band, x, y, stream);
+    }
+    void prec(const double* v, double* z)
+    {
+        stats.numApplies++;
+        if (pf) pf->apply(v, z);
+        else pd->apply(v, z);
+    }
+
+    // ------------------------------------------- BiCGStab(L), sap/bicgstab2.h:31-354
+    template <int L>
+    void bicgstabl(double* x, const double* b, BiCGStabLMonitor& monitor, bool x_is_zero)
+    {
+        const double eps = 1e-20;
+        double rho0 = 1, alpha = 0, omega = 1, rho1;
+        double* r0 = vec();
+        double* r = vec();
+        double* u = vec();
+        double* xx = vec();
+        double* Pv = vec();
+        double* rr[L + 1];
+        double* uu[L + 1];
+        for (int i = 0; i <= L; ++i) { rr[i] = vec(); uu[i] = vec(); }
+        double* x_min = vec();
+        double* Pxx = vec();
+        double* APxx = vec();
+        for (int i = 0; i <= L; ++i) { launch_fill(n, 0.0, rr[i], stream); launch_fill(n, 0.0, uu[i], stream); }
+        launch_fill(n, 0.0, u, stream);
+        launch_fill(n, 0.0, x_min, stream);
+        double tao[L + 1][L + 1] = {};
+        double gamma[L + 2] = {}, gamma_prime[L + 2] = {}, gamma_pp[L + 2] = {}, sigma[L + 2] = {};
+
+        // r0 <- b - A x  (A*0 = 0 exactly, so the SpMV is skipped for a zero guess)
+        if (x_is_zero) copy(b, r0);
+        else {
+            spmv(x, r0);
+            axpby(b, r0, r0, 1.0, -1.0);
+        }
+        copy(r0, r);
+        copy(u, uu[0]);
+        copy(x, xx);
+        copy(r, rr[0]);
+        double r_norm_min = nrm2(r);
+        double r_norm = r_norm_min;
+        double r_norm_act = r_norm;
+        bool pxx_valid = false;    // Pxx == M^{-1} xx for the current xx
+
+        auto check_true = [&]() -> bool {
+            prec(xx, Pxx);
+            spmv(Pxx, APxx);
+            axpby(b, APxx, rr[0], 1.0, -1.0);
+            pxx_valid = true;
+            r_norm_act = nrm2(rr[0]);
+            return monitor.finished(r_norm_act);
+        };
+
+        while (true) {
+            rho0 = -omega * rho0;
+            monitor.increment(0.25f);
+            for (int j = 0; j < L; ++j) {
+                rho1 = dot(rr[j], r0);
+                if (rho0 == 0) { monitor.stop(-10, "rho0 is zero"); break; }
+                const double beta = alpha * rho1 / rho0;
+                rho0 = rho1;
+                for (int i = 0; i <= j; ++i) axpby(rr[i], uu[i], uu[i], 1.0, -beta);
+                prec(uu[j], Pv);
+                spmv(Pv, uu[j + 1]);
+                const double gam = dot(uu[j + 1], r0);
+                if (gam == 0) { monitor.stop(-11, "gamma is zero"); break; }
+                alpha = rho0 / gam;
+                for (int i = 0; i <= j; ++i) axpy(uu[i + 1], rr[i], -alpha);
+                r_norm_act = r_norm = nrm2(rr[0]);
+                prec(rr[j], Pv);
+                spmv(Pv, rr[j + 1]);
+                if (std::fabs(alpha) * nrm2(uu[0]) < eps * nrm2(xx)) monitor.incrementStag();
+                else monitor.resetStag();
+                axpy(uu[0], xx, alpha);
+                pxx_valid = false;
+                if (monitor.needCheckConvergence(r_norm)) {
+                    if (check_true()) break;
+                }
+                if (r_norm_act < r_norm_min) { r_norm_min = r_norm_act; copy(xx, x_min); }
+                if (monitor.finished()) break;
+            }
+            if (monitor.finished()) break;
+
+            for (int j = 1; j <= L; ++j) {
+                for (int i = 1; i < j; ++i) {
+                    tao[i][j] = dot(rr[j], rr[i]) / sigma[i];
+                    axpy(rr[i], rr[j], -tao[i][j]);
+                }
+                sigma[j] = dot(rr[j], rr[j]);
+                if (sigma[j] == 0) { monitor.stop(-12, "a sigma value is zero"); break; }
+                gamma_prime[j] = dot(rr[j], rr[0]) / sigma[j];
+            }
+            if (monitor.finished()) break;
+
+            gamma[L] = gamma_prime[L];
+            omega = gamma[L];
+            for (int j = L - 1; j > 0; --j) {
+                gamma[j] = gamma_prime[j];
+                for (int i = j + 1; i <= L; ++i) gamma[j] -= tao[j][i] * gamma[i];
+            }
+            for (int j = 1; j < L; ++j) {
+                gamma_pp[j] = gamma[j + 1];
+                for (int i = j + 1; i < L; ++i) gamma_pp[j] += tao[j][i] * gamma[i + 1];
+            }
+            if (std::fabs(gamma[1]) * nrm2(rr[0]) < eps * nrm2(xx)) monitor.incrementStag();
+            else monitor.resetStag();
+            axpy(rr[0], xx, gamma[1]);
+            axpy(rr[L], rr[0], -gamma_prime[L]);
+            axpy(uu[L], uu[0], -gamma[L]);
+            pxx_valid = false;
+            r_norm_act = r_norm = nrm2(rr[0]);
+            monitor.increment(0.25f);
+            if (monitor.needCheckConvergence(r_norm)) {
+                if (check_true()) break;
+            }
+            if (r_norm_act < r_norm_min) { r_norm_min = r_norm_act; copy(xx, x_min); }
+            if (monitor.finished()) break;
+            monitor.increment(0.25f);
+
+            bool brk = false;
+            for (int j = 1; j < L; ++j) {
+                axpy(uu[j], uu[0], -gamma[j]);
+                if (std::fabs(gamma_pp[j]) * nrm2(rr[j]) < eps * nrm2(xx)) monitor.incrementStag();
+                else monitor.resetStag();
+                axpy(rr[j], xx, gamma_pp[j]);
+                axpy(rr[j], rr[0], -gamma_prime[j]);
+                pxx_valid = false;
+                r_norm_act = r_norm = nrm2(rr[0]);
+                if (monitor.needCheckConvergence(r_norm)) {
+                    if (check_true()) { brk = true; break; }
+                }
+                if (r_norm_act < r_norm_min) { r_norm_min = r_norm_act; copy(xx, x_min); }
+                if (monitor.finished()) { brk = true; break; }
+            }
+            if (brk || monitor.finished()) break;
+            copy(uu[0], u);
+            copy(xx, x);
+            copy(rr[0], r);
+            monitor.increment(0.25f);
+        }
+
+        if (monitor.converged()) {
+            // x <- M^{-1} xx; the preconditioner is deterministic, so a
+            // still-valid M^{-1} xx from the last true-residual check is reused.
+            if (pxx_valid) copy(Pxx, x);
+            else prec(xx, x);
+        } else {
+            double* Pxmin = vec();
+            double* APxmin = vec();
+            double* rc = vec();
+            double* rcm = vec();
+            prec(xx, Pxx);
+            spmv(Pxx, APxx);
+            axpby(b, APxx, rc, 1.0, -1.0);
+            prec(x_min, Pxmin);
+            spmv(Pxmin, APxmin);
+            axpby(b, APxmin, rcm, 1.0, -1.0);
+            const double rcn = nrm2(rc), rcmn = nrm2(rcm);
+            if (rcn < rcmn) { copy(Pxx, x); monitor.rNorm = rcn; }
+            else            { copy(Pxmin, x); monitor.rNorm = rcmn; }
+        }
+    }
+
+    // ------------------------------------------------- MINRES, sap/minres.h:38-181
+    void minres(double* x, const double* b, Monitor& monitor)
+    {
+        const double eps = std::numeric_limits<double>::epsilon();
+        double* y = vec();
+        double* r1 = vec();
+        double* v = vec();
+        double* w = vec();
+        double* w1 = vec();
+        double* w2 = vec();
+        double* r2 = vec();
+        double* tmp = vec();
+        spmv(x, r1);
+        axpby(b, r1, r1, 1.0, -1.0);
+        prec(r1, y);
+        double beta1 = dot(r1, y);
+        if (beta1 < 0) { monitor.stop(-10, "The preconditioner is not positive definite"); beta1 *= -1; }
+        else if (beta1 > 0) beta1 = std::sqrt(beta1);
+        double Anorm = 0, ynorm = 0, phibar = beta1;
+        double oldb = 0, beta = beta1, dbar = 0, epsln = 0, oldeps = 0;
+        double phi = 0, rhs1 = beta1, rhs2 = 0, tnorm2 = 0;
+        double cs = -1, sn = 0, gmax = 0, gmin = std::numeric_limits<double>::max();
+        double alpha = 0, gamma = 0, delta = 0, gbar = 0, z = 0;
+        launch_fill(n, 0.0, w, stream);
+        launch_fill(n, 0.0, w1, stream);
+        launch_fill(n, 0.0, w2, stream);
+        copy(r1, r2);
+        while (!monitor.finished(phibar)) {
+            const double s = 1 / beta;
+            axpby(y, y, v, s, 0.0);
+            spmv(v, y);
+            if (monitor.iteration_count() > 0) axpby(y, r1, y, 1.0, -beta / oldb);
+            alpha = dot(v, y);
+            axpby(y, r2, y, 1.0, -alpha / beta);
+            copy(r2, r1);
+            copy(y, r2);
+            prec(r2, y);
+            oldb = beta;
+            beta = dot(r2, y);
+            if (beta < 0) { monitor.stop(-10, "The preconditioner is not positive definite"); break; }
+            beta = std::sqrt(beta);
+            tnorm2 += alpha * alpha + oldb * oldb + beta * beta;
+            if (monitor.iteration_count() == 0) {
+                if (beta / beta1 <= 10 * eps) monitor.stop(10, "beta2 = 0. If M = I, b and x are eigenvectors");
+            }
+            oldeps = epsln;
+            delta = cs * dbar + sn * alpha;
+            gbar = sn * dbar - cs * alpha;
+            epsln = sn * beta;
+            dbar = -cs * beta;
+            const double root = std::sqrt(gbar * gbar + dbar * dbar);
+            gamma = std::sqrt(gbar * gbar + beta * beta);
+            gamma = std::max(gamma, eps);
+            cs = gbar / gamma;
+            sn = beta / gamma;
+            phi = cs * phibar;
+            phibar = sn * phibar;
+            const double denom = 1 / gamma;
+            copy(w2, w1);
+            copy(w, w2);
+            // w = denom*v - oldeps*denom*w1 - delta*denom*w2  (cusp::blas::axpbypcz)
+            axpby(v, w1, tmp, denom, -oldeps * denom);
+            axpby(tmp, w2, w, 1.0, -delta * denom);
+            axpby(x, w, x, 1.0, phi);
+            gmax = std::max(gmax, gamma);
+            gmin = std::min(gmin, gamma);
+            z = rhs1 / gamma;
+            rhs1 = rhs2 - delta * z;
+            rhs2 = -epsln * z;
+            Anorm = std::sqrt(tnorm2);
+            ynorm = nrm2(x);
+            if (monitor.code != 10) {
+                const double test2 = root / Anorm;
+                const double Acond = gmax / gmin;
+                if (Acond >= 0.1 / eps) monitor.stop(11, "x has converged to an eigenvector");
+                if (Anorm * eps * ynorm >= beta1) monitor.stop(12, "Reasonable accuracy achieved, given eps");
+                if (test2 <= monitor.absTol) monitor.stop(13, "A least-squares solution was found, given tol");
+            }
+            monitor.iterations += 1.f;
+        }
+    }
+
+    bool use_minres() const { return opt.solverType == SAP_MINRES; }
+
+    // One Krylov solve of A x = b (the reference's solve path).
+    int krylov(double* x, const double* b, double rtol, double atol, bool x_is_zero, float* its, double* rnorm,
+               double* rhs_norm)
+    {
+        if (use_minres()) {
+            mmon.reset(opt.maxNumIterations, rtol, atol);
+            mmon.init(nrm2(b));
+            minres(x, b, mmon);
+            *its += mmon.iterations;
+            *rnorm = mmon.rNorm;
+            *rhs_norm = mmon.rhsNorm;
+            message = mmon.message;
+            return mmon.code;
+        }
+        bmon.reset(opt.maxNumIterations, 8, rtol, atol);   // maxmSteps = 8 (sap/solver.h:314-319)
+        bmon.init(nrm2(b));
+        if (opt.solverType == SAP_BiCGStab1) bicgstabl<1>(x, b, bmon, x_is_zero);
+        else bicgstabl<2>(x, b, bmon, x_is_zero);
+        *its += bmon.iterations;
+        *rnorm = bmon.rNorm;
+        *rhs_norm = bmon.rhsNorm;
+        message = bmon.message;
+        return bmon.code;
+    }
+
+    int setup_banded(int64_t n_, int k_, const double* band)
+    {
+        const double t0 = wall_ms();
+        setup_done = false;
+        n = n_;
+        k = k_;
+        d_band = band;
+        int rc = pf ? pf->setup(n, k, band) : pd->setup(n, k, band);
+        SAP_HIP_CHECK(hipStreamSynchronize(stream));
+        if (rc != 0) return rc;
+        stats.timeSetup = wall_ms() - t0;
+        auto fill = [&](auto& p) {
+            stats.time_toBanded = p.time_toBanded;
+            stats.time_offDiags = p.time_offDiags;
+            stats.time_bandLU = p.time_bandLU;
+            stats.time_bandUL = p.time_bandUL;
+            stats.time_assembly = p.time_assembly;
+            stats.time_fullLU = p.time_fullLU;
+            stats.numPartitions = p.P;
+            stats.actual_nnz = (int)std::min<int64_t>(p.actual_nnz, INT_MAX);
+        };
+        if (pf) fill(*pf); else fill(*pd);
+        stats.bandwidth = k;
+        stats.bandwidthReorder = k;
+        stats.bandwidthDB = k;
+        stats.actualDropOff = 0;
+        // Uniform band: ks_row = ks_col = k, so nuKf = 0 (sap/solver.h:360, :391-394)
+        stats.nuKf = 0.0;
+        // flops_LU = sum_i ks_row[i]*ks_col[i] / (time_bandLU * 1e6) (sap/solver.h:361-366, :396-399)
+        stats.flops_LU = stats.time_bandLU > 0 ? (double)k * k * n / (stats.time_bandLU * 1e6) : 0.0;
+        setup_done = true;
+        return 0;
+    }
+
+    int solve(sap_spmv_fn fn, void* ctx, const double* d_b, double* d_x)
+    {
+        if (!setup_done) return SAP_ERR_ILLEGAL_SOLVE;
+        user_spmv = fn;
+        user_ctx = ctx;
+        stats.numApplies = stats.numSpmv = stats.numRefine = 0;
+        // b, x, three refinement vectors, and the Krylov method's vectors
+        // (BiCGStab(2): 20 including the non-converged tail; MINRES: 8)
+        reserve_vectors(32);
+        double* bv = vec();
+        double* xv = vec();
+        copy(d_b, bv);            // SolverVector b_vector = b (sap/solver.h:488-489)
+        copy(d_x, xv);
+        const double t0 = wall_ms();
+        float its = 0.f;
+        double rn = 0, bn = 0;
+        if (opt.refine > 0) {
+            // Outer FP64 refinement: x += d, A d = b - A x solved by the
+            // preconditioned Krylov method to refineTol.  With an FP32
+            // preconditioner the right-preconditioned BiCGStab(l) cannot go
+            // much below 1e-7 in relative residual on its own.
+            double* r = vec();
+            double* Ax = vec();
+            double* dv = vec();
+            const size_t mark = vec_next_;
+            bn = nrm2(bv);
+            const double tol = opt.absTol + opt.relTol * bn;
+            code = -1;
+            for (int outer = 0;; ++outer) {
+                spmv(xv, Ax);
+                axpby(bv, Ax, r, 1.0, -1.0);
+                rn = nrm2(r);
+                if (std::isnan(rn)) { code = -2; break; }
+                if (rn <= tol) { code = 1; break; }
+                if (outer >= opt.refine) { code = -1; break; }
+                const double rt = std::max(opt.refineTol, 0.5 * tol / rn);
+                launch_fill(n, 0.0, dv, stream);
+                vec_next_ = mark;
+                double inner_r, inner_b;
+                const int c = krylov(dv, r, rt, 0.0, true, &its, &inner_r, &inner_b);
+                stats.numRefine++;
+                axpy(dv, xv, 1.0);
+                if (c == -2) { code = -2; break; }
+            }
+            message = code == 1 ? "Converged"
+                      : code == -2 ? "Residual norm is NaN" : "Maximum number of refinement steps was reached";
+            bmon.code = mmon.code = code;
+            bmon.message = mmon.message = message;
+        } else {
+            code = krylov(xv, bv, opt.relTol, opt.absTol, false, &its, &rn, &bn);
+        }
+        copy(xv, d_x);            // thrust::copy(x_vector -> x) (sap/solver.h:534)
+        SAP_HIP_CHECK(hipStreamSynchronize(stream));
+        stats.timeSolve = wall_ms() - t0;
+        stats.rhsNorm = bn;
+        stats.residualNorm = rn;
+        stats.relResidualNorm = rn / bn;
+        stats.numIterations = its;
+        return code > 0 ? 1 : 0;
+    }
+};
+
+}  // namespace sap_amd
+
+// ====================================================================== C ABI
+using sap_amd::Error;
+using sap_amd::Solver;
+
+struct sap_solver {
+    std::unique_ptr<Solver> s;
+    std::string err;
+};
+
+namespace {
+
+template <class F>
+int guarded(sap_solver* h, F&& f)
+{
+    try {
+        return f();
+    } catch (const Error& e) {
+        if (h) h->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc& e) {
+        if (h) h->err = e.what();
+        return SAP_ERR_OOM;
+    } catch (const std::exception& e) {
+        if (h) h->err = e.what();
+        return SAP_ERR_HIP;
+    }
+}
+
+int check_options(const sap_options* o, std::string& why)
+{
+    if (o->gpuCount != 1) { why = "gpuCount > 1 is not supported (one GPU)"; return SAP_ERR_UNSUPPORTED; }
+    if (o->useBCR) { why = "useBCR is not supported"; return SAP_ERR_UNSUPPORTED; }
+    if (o->ilu_level >= 0) { why = "ILU preconditioning is not supported"; return SAP_ERR_UNSUPPORTED; }
+    if (o->saveMem) { why = "saveMem (SPD half-band storage) is not supported"; return SAP_ERR_UNSUPPORTED; }
+    if (o->solverType != SAP_BiCGStab1 && o->solverType != SAP_BiCGStab2 && o->solverType != SAP_MINRES) {
+        why = "only BiCGStab1, BiCGStab2 and MINRES are provided";
+        return SAP_ERR_UNSUPPORTED;
+    }
+    if (o->factMethod != SAP_LU_UL && o->factMethod != SAP_LU_only) { why = "bad factMethod"; return SAP_ERR_BADARG; }
+    if (o->precondType < SAP_Spike || o->precondType > SAP_None) { why = "bad precondType"; return SAP_ERR_BADARG; }
+    if (o->precValueType != SAP_PREC_FLOAT && o->precValueType != SAP_PREC_DOUBLE) {
+        why = "bad precValueType";
+        return SAP_ERR_BADARG;
+    }
+    return SAP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void sap_default_options(sap_options* o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    // sap::Options::Options() (sap/solver.h:232-256)
+    o->solverType = SAP_BiCGStab2;
+    o->maxNumIterations = 100;
+    o->gpuCount = 1;
+    o->relTol = 1e-6;
+    o->absTol = 0;
+    o->testDB = 0;
+    o->isSPD = 0;
+    o->saveMem = 0;
+    o->performReorder = 1;
+    o->performDB = 1;
+    o->dbFirstStageOnly = 0;
+    o->applyScaling = 1;
+    o->maxBandwidth = INT_MAX;
+    o->dropOffFraction = 0;
+    o->factMethod = SAP_LU_only;
+    o->precondType = SAP_Spike;
+    o->safeFactorization = 0;
+    o->variableBandwidth = 1;
+    o->trackReordering = 0;
+    o->useBCR = 0;
+    o->ilu_level = -1;
+    o->precValueType = SAP_PREC_FLOAT;
+    o->refine = 0;
+    o->refineTol = 1e-6;
+    o->profile = 0;
+}
+
+sap_solver* sap_create(int numPartitions, const sap_options* opts)
+{
+    sap_options o;
+    if (opts) o = *opts;
+    else sap_default_options(&o);
+    std::string why;
+    if (numPartitions < 1 || check_options(&o, why) != SAP_OK) return nullptr;
+    auto* h = new (std::nothrow) sap_solver();
+    if (!h) return nullptr;
+    const int rc = guarded(h, [&] {
+        h->s = std::make_unique<Solver>(numPartitions, o);
+        return 0;
+    });
+    if (rc != 0) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void sap_destroy(sap_solver* h) { delete h; }
+
+int sap_set_stream(sap_solver* h, void* stream)
+{
+    if (!h) return SAP_ERR_BADARG;
+    h->s->set_stream((hipStream_t)stream);
+    return SAP_OK;
+}
+
+int sap_setup_banded(sap_solver* h, int n, int k, const double* d_band)
+{
+    if (!h) return SAP_ERR_BADARG;
+    if (n <= 0 || k < 0 || !d_band) { h->err = "bad matrix arguments"; return SAP_ERR_BADARG; }
+    return guarded(h, [&] { return h->s->setup_banded(n, k, d_band); });
+}
+
+int sap_solve(sap_solver* h, sap_spmv_fn spmv, void* ctx, const double* d_b, double* d_x, int nrhs)
+{
+    if (!h) return SAP_ERR_BADARG;
+    if (!d_b || !d_x || nrhs < 1) { h->err = "bad solve arguments"; return SAP_ERR_BADARG; }
+    return guarded(h, [&] {
+        int all = 1;
+        for (int c = 0; c < nrhs; ++c) {
+            const int r = h->s->solve(spmv, ctx, d_b + (size_t)c * h->s->n, d_x + (size_t)c * h->s->n);
+            if (r < 0) return r;
+            all &= r;
+        }
+        return all;
+    });
+}
+
+int sap_get_stats(const sap_solver* h, sap_stats* out)
+{
+    if (!h || !out) return SAP_ERR_BADARG;
+    *out = h->s->stats;
+    return SAP_OK;
+}
+
+int sap_monitor_code(const sap_solver* h) { return h ? h->s->code : 0; }
+const char* sap_monitor_message(const sap_solver* h) { return h ? h->s->message.c_str() : ""; }
+const char* sap_last_error(const sap_solver* h) { return h ? h->err.c_str() : ""; }
+
+int sap_precond_apply(sap_solver* h, const double* d_v, double* d_z)
+{
+    if (!h) return SAP_ERR_BADARG;
+    if (!h->s->setup_done) return SAP_ERR_ILLEGAL_SOLVE;
+    return guarded(h, [&] {
+        h->s->prec(d_v, d_z);
+        SAP_HIP_CHECK(hipStreamSynchronize(h->s->stream));
+        return 0;
+    });
+}
+
+int sap_band_spmv(int n, int k, const double* d_band, const double* d_x, double* d_y, void* stream)
+{
+    if (n <= 0 || k < 0 || !d_band || !d_x || !d_y) return SAP_ERR_BADARG;
+    return guarded(nullptr, [&] {
+        sap_amd::launch_band_spmv(n, k, d_band, d_x, d_y, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+int sap_get_profile(const sap_solver* h, sap_kernel_profile* out, int max_entries)
+{
+    if (!h) return SAP_ERR_BADARG;
+    auto& p = const_cast<sap_amd::Profiler&>(h->s->prof);
+    return guarded(const_cast<sap_solver*>(h), [&] {
+        p.flush();
+        const int m = std::min<int>(max_entries, (int)p.entries.size());
+        for (int i = 0; i < m && out; ++i) {
+            out[i].name = p.entries[i].name.c_str();
+            out[i].launches = p.entries[i].launches;
+            out[i].total_ms = p.entries[i].ms;
+        }
+        return (int)p.entries.size();
+    });
+}
+
+void sap_reset_profile(sap_solver* h)
+{
+    if (h) guarded(h, [&] { h->s->prof.reset(); return 0; });
+}
+
+int sap_num_partitions(const sap_solver* h) { return h ? h->s->num_partitions() : 0; }
+
+int sap_get_factors(const sap_solver* h, double* h_out, int64_t count)
+{
+    if (!h || !h_out) return SAP_ERR_BADARG;
+    Solver& S = *h->s;
+    if (!S.setup_done) return SAP_ERR_ILLEGAL_SOLVE;
+    return guarded(const_cast<sap_solver*>(h), [&] {
+        auto grab = [&](auto& p) {
+            using T = typename std::remove_reference<decltype(*p.factors().get())>::type;
+            const int64_t w = 2 * (int64_t)S.k + 1;
+            if (count < w * S.n) throw Error(SAP_ERR_BADARG, "output too small");
+            std::vector<T> tmp((size_t)w * S.n);
+            SAP_HIP_CHECK(hipMemcpy(tmp.data(), p.factors().get(), sizeof(T) * tmp.size(), hipMemcpyDeviceToHost));
+            if (p.spike_active() && p.lu_ul()) {
+                // the last partition's factors are its UL factors
+                sap_amd::PartInfo pi{S.n, p.P};
+                int64_t s, len;
+                pi.bounds(p.P - 1, s, len);
+                SAP_HIP_CHECK(hipMemcpy(tmp.data() + w * s, p.factors_ul().get() + w * s, sizeof(T) * w * len,
+                                        hipMemcpyDeviceToHost));
+            }
+            for (size_t i = 0; i < tmp.size(); ++i) h_out[i] = (double)tmp[i];
+        };
+        if (S.pf) grab(*S.pf); else grab(*S.pd);
+        return 0;
+    });
+}
+
+int sap_get_spikes(const sap_solver* h, double* h_wv, double* h_schur, int64_t count)
+{
+    if (!h) return SAP_ERR_BADARG;
+    Solver& S = *h->s;
+    if (!S.setup_done) return SAP_ERR_ILLEGAL_SOLVE;
+    return guarded(const_cast<sap_solver*>(h), [&] {
+        auto grab = [&](auto& p) {
+            using T = typename std::remove_reference<decltype(*p.spikes().get())>::type;
+            const size_t nwv = p.spikes().size(), ns = p.schur().size();
+            if ((size_t)count < nwv) throw Error(SAP_ERR_BADARG, "output too small");
+            std::vector<T> tmp(std::max(nwv, ns));
+            if (h_wv && nwv) {
+                SAP_HIP_CHECK(hipMemcpy(tmp.data(), p.spikes().get(), sizeof(T) * nwv, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < nwv; ++i) h_wv[i] = (double)tmp[i];
+            }
+            if (h_schur && ns) {
+                SAP_HIP_CHECK(hipMemcpy(tmp.data(), p.schur().get(), sizeof(T) * ns, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < ns; ++i) h_schur[i] = (double)tmp[i];
+            }
+        };
+        if (S.pf) grab(*S.pf); else grab(*S.pd);
+        return 0;
+    });
+}
+
+int sap_gen_banded(int64_t n, int k, double dominance, uint64_t seed, double* d_band, void* stream)
+{
+    if (n <= 0 || k < 0 || !d_band) return SAP_ERR_BADARG;
+    return guarded(nullptr, [&] {
+        sap_amd::launch_gen_banded(n, k, dominance, seed, d_band, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+int sap_gen_vector(int64_t n, uint64_t seed, double* d_x, void* stream)
+{
+    if (n <= 0 || !d_x) return SAP_ERR_BADARG;
+    return guarded(nullptr, [&] {
+        sap_amd::launch_gen_vector(n, seed, d_x, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+}  // extern "C"

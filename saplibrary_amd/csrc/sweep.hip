@@ -1,0 +1,335 @@
+// Preconditioner apply: z = M^{-1} v (Precond::solve / getSRev,
+// sap/precond.h:1392-1528).
+//
+// SaP-C runs three launches:
+//   pass 1  interface values of g = A_D^{-1} v.  The reduced system needs
+//           only the bottom k rows of A_i^{-1} v_i and the top k rows of
+//           A_{i+1}^{-1} v_{i+1}: the first come from a full L sweep of the
+//           LU factors plus a k-row U' sweep, the second from a full U' sweep
+//           of the UL factors plus a k-row L' sweep.  The two are
+//           independent, so every partition boundary gets two workgroups.
+//   iface   reduced solve per boundary with S_i^{-1} and purification
+//           (partFullFwdSweep/partFullBckSweep + purifyRHS,
+//           sap/precond.h:5418-5530), producing corrections of the rhs rows
+//           next to each boundary.
+//   pass 2  full L, D^{-1}, U' sweep of (v - corrections) for every
+//           partition (partBandedFwdSweep/partBandedBckSweep); the last
+//           partition uses its UL factors when factMethod = LU_UL.
+// SaP-D (or P = 1) runs pass 2 alone.
+//
+// Triangular sweeps: one workgroup per (partition, direction) walks the rows
+// in blocks of 64.  Wavefront 0 solves the 64 x 64 unit-lower diagonal block
+// with a lane-broadcast chain; then every thread updates one of the next k
+// rows with the block's 64 multipliers.  The pending values of the next
+// 2*64 + k rows live in an LDS ring.
+#include "sap_internal.hpp"
+
+namespace sap_amd {
+
+namespace {
+
+constexpr int NBS = 64;
+
+template <class T>
+struct BandView {
+    const T* base;
+    int64_t origin;
+    int64_t w;
+    int k;
+    int dir;
+    __device__ inline T at(int64_t r, int64_t c) const { return base[origin + dir * (c * w + k + r - c)]; }
+};
+
+template <class T>
+__device__ inline BandView<T> make_view(const T* B, int64_t s, int64_t len, int k, int dir)
+{
+    BandView<T> v;
+    v.base = B;
+    v.w = 2 * k + 1;
+    v.k = k;
+    v.dir = dir;
+    v.origin = dir > 0 ? s * v.w : (s + len) * v.w - 1;
+    return v;
+}
+
+__device__ inline float bcast(float v, int j)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ inline double bcast(double v, int j)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), j);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// y = L^{-1} b over local rows [0, len) of a view (L unit lower, band k).
+// in(r) gives b_r; out(r, y_r) receives each final value.
+template <class T, class In, class Out>
+__device__ void lower_sweep(const BandView<T>& V, int64_t len, In in, Out out, T* pend, int ring_mask, T* xblk)
+{
+    const int k = V.k;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int64_t init = min(len, (int64_t)NBS + k);
+    for (int64_t r = tid; r < init; r += blockDim.x) pend[r & ring_mask] = in(r);
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < len; c0 += NBS) {
+        const int nb = (int)min((int64_t)NBS, len - c0);
+        if (wave == 0) {
+            T Ld[NBS];
+#pragma unroll
+            for (int j = 0; j < NBS; ++j)
+                Ld[j] = (j < lane && lane < nb && lane - j <= k) ? V.at(c0 + lane, c0 + j) : (T)0;
+            T x = lane < nb ? pend[(c0 + lane) & ring_mask] : (T)0;
+#pragma unroll
+            for (int j = 0; j < NBS - 1; ++j) {
+                const T xj = bcast(x, j);
+                x -= Ld[j] * xj;
+            }
+            if (lane < nb) {
+                xblk[lane] = x;
+                out(c0 + lane, x);
+            }
+        }
+        __syncthreads();
+        for (int t = tid; t < k; t += blockDim.x) {
+            const int64_t r = c0 + nb + t;
+            if (r >= len) break;
+            const int jlo = max(0, nb + t - k);
+            T acc = (T)0;
+            for (int j = jlo; j < nb; ++j) acc += V.at(r, c0 + j) * xblk[j];
+            pend[r & ring_mask] -= acc;
+        }
+        {
+            const int64_t lo = c0 + nb + k;
+            const int64_t hi = min(len, c0 + nb + k + NBS);
+            for (int64_t r = lo + tid; r < hi; r += blockDim.x) pend[r & ring_mask] = in(r);
+        }
+        __syncthreads();
+    }
+}
+
+inline int ring_size(int k)
+{
+    int r = 1;
+    while (r < 2 * NBS + k) r <<= 1;
+    return r;
+}
+
+template <class T>
+__device__ inline T* lds_ring()
+{
+    extern __shared__ unsigned char smem_raw[];
+    return reinterpret_cast<T*>(smem_raw);
+}
+
+// ---------------------------------------------------------------------------
+// pass 1 (LU_UL): blockIdx.x = 2*i + side, i = boundary.
+//   side 0: partition i, LU factors (B), bottom k rows of A_i^{-1} v_i
+//   side 1: partition i+1, UL factors (Bul), top k rows of A_{i+1}^{-1} v_{i+1}
+// g receives those 2k values per boundary at their row positions;
+// scr is a length-n scratch vector per side.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void sweep_interface_kernel(const T* __restrict__ B, const T* __restrict__ Bul,
+                                                              int k, PartInfo pi, const double* __restrict__ v,
+                                                              T* __restrict__ g, T* __restrict__ scr0,
+                                                              T* __restrict__ scr1, int ring_mask)
+{
+    T* pend = lds_ring<T>();
+    __shared__ T xblk[NBS];
+    const int i = blockIdx.x >> 1;
+    const int side = blockIdx.x & 1;
+    const int p = i + side;
+    int64_t s, len;
+    pi.bounds(p, s, len);
+    const T* F = side == 0 ? B : Bul;
+    T* scr = side == 0 ? scr0 : scr1;
+    const int d1 = side == 0 ? 1 : -1;       // stage-1 direction
+    const BandView<T> V1 = make_view(F, s, len, k, d1);
+    const BandView<T> V2 = make_view(F, s, len, k, -d1);
+    // local row r of a view with direction d -> global row
+    auto grow = [&](int d, int64_t r) -> int64_t { return d > 0 ? s + r : s + len - 1 - r; };
+
+    lower_sweep<T>(
+        V1, len, [&](int64_t r) -> T { return (T)v[grow(d1, r)]; },
+        [&](int64_t r, T y) { scr[grow(d1, r)] = y / V1.at(r, r); }, pend, ring_mask, xblk);
+    __syncthreads();
+    const int64_t kl = min((int64_t)k, len);
+    lower_sweep<T>(
+        V2, kl, [&](int64_t r) -> T { return scr[grow(-d1, r)]; },
+        [&](int64_t r, T y) { g[grow(-d1, r)] = y; }, pend, ring_mask, xblk);
+}
+
+// ---------------------------------------------------------------------------
+// pass 2: blockIdx.x = partition.  out = A_p^{-1} (v - corr) with the LU
+// factors (stage 1 forward L with D^{-1}, stage 2 backward U'), or for the
+// UL-factored last partition the mirror order.  corr (2k per boundary) may
+// be null.  Output FP64 (outd) or preconditioner precision (outt).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void sweep_full_kernel(const T* __restrict__ B, const T* __restrict__ Bul, int k,
+                                                         PartInfo pi, int last_is_ul, const double* __restrict__ v,
+                                                         const T* __restrict__ corr, double* __restrict__ outd,
+                                                         T* __restrict__ outt, T* __restrict__ scr, int ring_mask)
+{
+    T* pend = lds_ring<T>();
+    __shared__ T xblk[NBS];
+    const int p = blockIdx.x;
+    int64_t s, len;
+    pi.bounds(p, s, len);
+    const bool ul = last_is_ul && p == pi.P - 1;
+    const T* F = ul ? Bul : B;
+    const int d1 = ul ? -1 : 1;
+    const BandView<T> V1 = make_view(F, s, len, k, d1);
+    const BandView<T> V2 = make_view(F, s, len, k, -d1);
+    auto grow = [&](int d, int64_t r) -> int64_t { return d > 0 ? s + r : s + len - 1 - r; };
+    const bool has_top = corr && p > 0;
+    const bool has_bot = corr && p < pi.P - 1;
+    auto rhs = [&](int64_t r) -> T {
+        const int64_t gr = grow(d1, r);
+        T val = (T)v[gr];
+        const int64_t lr = gr - s;
+        if (has_top && lr < k) val -= corr[(size_t)(p - 1) * 2 * k + k + lr];
+        if (has_bot && lr >= len - k) val -= corr[(size_t)p * 2 * k + (lr - (len - k))];
+        return val;
+    };
+    lower_sweep<T>(
+        V1, len, rhs, [&](int64_t r, T y) { scr[grow(d1, r)] = y / V1.at(r, r); }, pend, ring_mask, xblk);
+    __syncthreads();
+    if (outd) {
+        lower_sweep<T>(
+            V2, len, [&](int64_t r) -> T { return scr[grow(-d1, r)]; },
+            [&](int64_t r, T y) { outd[grow(-d1, r)] = (double)y; }, pend, ring_mask, xblk);
+    } else {
+        lower_sweep<T>(
+            V2, len, [&](int64_t r) -> T { return scr[grow(-d1, r)]; },
+            [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, pend, ring_mask, xblk);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reduced solve and purification per boundary i (one workgroup):
+//   zt' = g_t - W g_b ;  x_t = S^{-1} zt' ;  x_b = g_b - V x_t
+//   corr_b = B_i x_t (rows e0-k..e0-1),  corr_t = C_{i+1} x_b (rows s1..s1+k-1)
+// All k x k matrices column-major.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ void gemv_cm(const T* __restrict__ M, int k, const T* x, T* y, T beta_src_sign, const T* src)
+{
+    // y[a] = src[a] + sign * sum_b M(a, b) x[b]   (src may be null -> 0)
+    for (int a = threadIdx.x; a < k; a += blockDim.x) {
+        T acc = (T)0;
+        for (int b = 0; b < k; ++b) acc += M[(size_t)b * k + a] * x[b];
+        y[a] = (src ? src[a] : (T)0) + beta_src_sign * acc;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void interface_kernel(const T* __restrict__ WV, const T* __restrict__ Sinv,
+                                                        const T* __restrict__ offB, const T* __restrict__ offC, int k,
+                                                        PartInfo pi, const T* __restrict__ g, T* __restrict__ corr)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* gb = reinterpret_cast<T*>(smem_raw);
+    T* gt = gb + k;
+    T* zt = gt + k;
+    T* xt = zt + k;
+    T* xb = xt + k;
+    const int i = blockIdx.x;
+    int64_t s1, l1;
+    pi.bounds(i + 1, s1, l1);
+    const size_t kk = (size_t)k * k;
+    for (int a = threadIdx.x; a < k; a += blockDim.x) {
+        gb[a] = g[s1 - k + a];
+        gt[a] = g[s1 + a];
+    }
+    __syncthreads();
+    gemv_cm<T>(WV + (2 * i + 1) * kk, k, gb, zt, (T)-1, gt);   // zt = gt - W gb
+    __syncthreads();
+    gemv_cm<T>(Sinv + i * kk, k, zt, xt, (T)1, nullptr);        // xt = S^{-1} zt
+    __syncthreads();
+    gemv_cm<T>(WV + 2 * i * kk, k, xt, xb, (T)-1, gb);          // xb = gb - V xt
+    __syncthreads();
+    T* c = corr + (size_t)i * 2 * k;
+    gemv_cm<T>(offB + i * kk, k, xt, c, (T)1, nullptr);         // B_i xt
+    gemv_cm<T>(offC + i * kk, k, xb, c + k, (T)1, nullptr);     // C_{i+1} xb
+}
+
+template <class T>
+__global__ void diag_divide_kernel(const T* __restrict__ B, int64_t n, const double* __restrict__ v,
+                                   double* __restrict__ out)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        out[i] = (double)((T)v[i] / B[i]);
+}
+
+template <class T>
+size_t sweep_lds(int k)
+{
+    return sizeof(T) * (size_t)ring_size(k);
+}
+
+}  // namespace
+
+template <class T>
+void launch_sweep_interface(const T* B, const T* Bul, int k, PartInfo pi, const double* v, T* g, T* scr0, T* scr1,
+                            hipStream_t st)
+{
+    if (pi.P < 2) return;
+    const size_t lds = sweep_lds<T>(k);
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)sweep_interface_kernel<T>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(sweep_interface_kernel<T>, dim3(2 * (pi.P - 1)), dim3(256), lds, st, B, Bul, k, pi, v, g, scr0,
+                       scr1, ring_size(k) - 1);
+    SAP_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_interface_solve(const T* WV, const T* Sinv, const T* offB, const T* offC, int k, PartInfo pi, const T* g,
+                            T* corr, hipStream_t st)
+{
+    if (pi.P < 2) return;
+    const size_t lds = sizeof(T) * 5 * (size_t)k;
+    hipLaunchKernelGGL(interface_kernel<T>, dim3(pi.P - 1), dim3(256), lds, st, WV, Sinv, offB, offC, k, pi, g, corr);
+    SAP_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_sweep_full(const T* B, const T* Bul, int k, PartInfo pi, int last_is_ul, const double* v, const T* corr,
+                       double* outd, T* outt, T* scr, hipStream_t st)
+{
+    const size_t lds = sweep_lds<T>(k);
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)sweep_full_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds));
+    hipLaunchKernelGGL(sweep_full_kernel<T>, dim3(pi.P), dim3(256), lds, st, B, Bul, k, pi, last_is_ul, v, corr, outd,
+                       outt, scr, ring_size(k) - 1);
+    SAP_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_diag_divide(const T* B, int64_t n, const double* v, double* out, hipStream_t st)
+{
+    int64_t g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(diag_divide_kernel<T>, dim3((unsigned)g), dim3(256), 0, st, B, n, v, out);
+    SAP_LAUNCH_CHECK();
+}
+
+#define SAP_INST(T)                                                                                               \
+    template void launch_sweep_interface<T>(const T*, const T*, int, PartInfo, const double*, T*, T*, T*,          \
+                                            hipStream_t);                                                          \
+    template void launch_interface_solve<T>(const T*, const T*, const T*, const T*, int, PartInfo, const T*, T*,   \
+                                            hipStream_t);                                                          \
+    template void launch_sweep_full<T>(const T*, const T*, int, PartInfo, int, const double*, const T*, double*,   \
+                                       T*, T*, hipStream_t);                                                       \
+    template void launch_diag_divide<T>(const T*, int64_t, const double*, double*, hipStream_t);
+SAP_INST(float)
+SAP_INST(double)
+#undef SAP_INST
+
+}  // namespace sap_amd
