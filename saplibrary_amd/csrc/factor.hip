@@ -134,6 +134,233 @@ __global__ __launch_bounds__(256) void band_factor_kernel(T* __restrict__ B0, T*
     }
 }
 
+// ---------------------------------------------------------------------------
+// MFMA variant of the blocked banded LU (32-column panels), used for
+// k <= 224 (one panel row per thread).  Per panel:
+//   1. every thread holds one or two rows of the (32 + k) x 32 panel in
+//      registers and the panel is factored column by column (boosting at
+//      pivot time); the pivot row is broadcast through LDS, one barrier per
+//      column;
+//   2. each thread also holds one column of the 32 x k block of U rows to
+//      the right of the panel and applies the panel's row operations to it,
+//      one column step behind (multipliers shared through LDS);
+//   3. the k x k trailing window in global memory receives the rank-32
+//      update -L_below * U_right on the matrix cores
+//      (v_mfma_f32_32x32x2f32, or v_mfma_f64_16x16x4f64 in FP64).
+// Factors are written as L (unit), D, U' (divided), like the LDS variant.
+// ---------------------------------------------------------------------------
+template <class T>
+struct Mfma;
+
+template <>
+struct Mfma<float> {
+    static constexpr int TILE = 32, KSTEP = 2, NACC = 16;
+    typedef float acc_t __attribute__((ext_vector_type(16)));
+    __device__ static inline acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+    }
+    // accumulator register r of lane l holds D[row(l, r)][col(l)]
+    __device__ static inline int row(int l, int r) { return (r & 3) + 4 * (l >> 5) + 8 * (r >> 2); }
+    __device__ static inline int col(int l) { return l & 31; }
+    // A (TILE x KSTEP): lane l holds A[l % 32][l / 32]; B (KSTEP x TILE): B[l / 32][l % 32]
+    __device__ static inline int ab_mn(int l) { return l & 31; }
+    __device__ static inline int ab_k(int l) { return l >> 5; }
+};
+
+template <>
+struct Mfma<double> {
+    static constexpr int TILE = 16, KSTEP = 4, NACC = 4;
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    __device__ static inline acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // FP64 accumulators interleave rows: register r of lane l is row l/16 + 4r
+    __device__ static inline int row(int l, int r) { return (l >> 4) + 4 * r; }
+    __device__ static inline int col(int l) { return l & 15; }
+    __device__ static inline int ab_mn(int l) { return l & 15; }
+    __device__ static inline int ab_k(int l) { return l >> 4; }
+};
+
+constexpr int MNB = 32;   // panel width of the MFMA factorization
+
+template <class T>
+struct MfmaLds {
+    int kp, ls_stride, us_stride;
+    size_t ls_off, us_off, ld_off, prow_off, piv_off, bytes;
+    __host__ __device__ MfmaLds(int k)
+    {
+        constexpr int TILE = sizeof(T) == 4 ? 32 : 16;
+        kp = ((k + TILE - 1) / TILE) * TILE;
+        ls_stride = MNB + 1;
+        us_stride = kp + 4;
+        ls_off = 0;
+        us_off = ls_off + (size_t)kp * ls_stride;
+        ld_off = us_off + (size_t)MNB * us_stride;
+        prow_off = ld_off + (size_t)MNB * (MNB + 1);
+        piv_off = prow_off + 2 * MNB;
+        bytes = (piv_off + MNB) * sizeof(T);
+    }
+};
+
+template <class T, int RPT>
+__global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B0, T* __restrict__ B1, int k,
+                                                               const FactorJob* __restrict__ jobs)
+{
+    using MF = Mfma<T>;
+    constexpr int TILE = MF::TILE;
+    const FactorJob job = jobs[blockIdx.x];
+    T* __restrict__ base = job.buf ? B1 : B0;
+    const int64_t w = 2 * k + 1;
+    const int64_t len = job.len;
+    const int dir = job.dir;
+    const int64_t origin = dir > 0 ? job.start * w : (job.start + len) * w - 1;
+    auto aidx = [&](int64_t r, int64_t c) -> int64_t { return origin + dir * (c * w + k + r - c); };
+    const int64_t cstep = (int64_t)dir * (w - 1);   // next column, same row
+
+    extern __shared__ unsigned char smem_raw[];
+    T* sm = reinterpret_cast<T*>(smem_raw);
+    const MfmaLds<T> lay(k);
+    T* Ls = sm + lay.ls_off;       // -L_below [kp][MNB+1]
+    T* Us = sm + lay.us_off;       // U_right [MNB][kp+4]
+    T* lcol = sm + lay.ld_off;     // multipliers of one panel column for rows < MNB, [2][MNB]
+    T* prow = sm + lay.prow_off;   // pivot row, double-buffered [2][MNB]
+    T* pivs = sm + lay.piv_off;    // pivots of the panel [MNB]
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+
+    for (int64_t j0 = 0; j0 < len; j0 += MNB) {
+        const int nb = (int)min((int64_t)MNB, len - j0);
+        const int H = (int)min(len - j0, (int64_t)nb + k);
+        const int Mw = (int)min(len - j0 - nb, (int64_t)k);
+
+        // (1) thread tid + 256 q owns panel row r = tid + 256 q (32 registers)
+        //     and U-block column c = tid + 256 q (rows j0..j0+31, 32 registers)
+        T row[RPT][MNB];
+        T ucol[RPT][MNB];
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + 256 * q;
+            const T* rp = base + aidx(j0 + r, j0);
+#pragma unroll
+            for (int c = 0; c < MNB; ++c)
+                row[q][c] = (r < H && c < nb && r - c <= k && c - r <= k) ? rp[c * cstep] : (T)0;
+            const int c = r;
+            const T* cp = base + aidx(j0, j0 + nb + c);
+#pragma unroll
+            for (int m = 0; m < MNB; ++m) ucol[q][m] = (c < Mw && m < nb && nb + c - m <= k) ? cp[dir * m] : (T)0;
+        }
+
+        // (2) panel LU, one barrier per column.  The U block receives the row
+        //     operations of column jj one step later (multipliers via LDS).
+#pragma unroll
+        for (int jj = 0; jj < MNB; ++jj) {
+            if (jj < nb) {
+                T* pr = prow + (jj & 1) * MNB;
+                if (tid == jj) {
+                    T p = row[0][jj];
+                    if (job.boost && (j0 + jj < len - 1 || job.boost_last)) p = boost_pivot(p);
+                    row[0][jj] = p;
+                    pivs[jj] = p;
+#pragma unroll
+                    for (int c = jj; c < MNB; ++c) pr[c] = row[0][c];
+                }
+                __syncthreads();
+                const T piv = pr[jj];
+                T* lc = lcol + (jj & 1) * MNB;
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const int r = tid + 256 * q;
+                    T l = (T)0;
+                    if (r > jj && r < H && r - jj <= k) {
+                        l = row[q][jj] / piv;
+                        row[q][jj] = l;
+#pragma unroll
+                        for (int c = jj + 1; c < MNB; ++c) row[q][c] -= l * pr[c];
+                    }
+                    if (r < MNB) lc[r] = l;   // zero outside the band
+                }
+                if (jj >= 1) {
+                    const T* lp = lcol + ((jj - 1) & 1) * MNB;
+#pragma unroll
+                    for (int q = 0; q < RPT; ++q) {
+                        const T u = ucol[q][jj - 1];
+#pragma unroll
+                        for (int r = jj; r < MNB; ++r) ucol[q][r] -= lp[r] * u;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // (3) write back: panel (U entries divided by the pivot), U block
+        //     (divided); stage -L_below and the undivided U block in LDS
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + 256 * q;
+            if (r < H) {
+                const T pr_r = r < nb ? pivs[r] : (T)1;
+                T* rp = base + aidx(j0 + r, j0);
+#pragma unroll
+                for (int c = 0; c < MNB; ++c) {
+                    if (c < nb && r - c <= k && c - r <= k) {
+                        T v = row[q][c];
+                        if (r < c) v /= pr_r;
+                        rp[c * cstep] = v;
+                    }
+                }
+            }
+            if (r >= nb && r - nb < lay.kp) {
+#pragma unroll
+                for (int c = 0; c < MNB; ++c) Ls[(r - nb) * lay.ls_stride + c] = (r < H) ? -row[q][c] : (T)0;
+            }
+            const int c = r;
+            if (c < lay.kp) {
+                T* cp = base + aidx(j0, j0 + nb + c);
+#pragma unroll
+                for (int m = 0; m < MNB; ++m) {
+                    Us[m * lay.us_stride + c] = ucol[q][m];
+                    if (c < Mw && m < nb && nb + c - m <= k) cp[dir * m] = ucol[q][m] / pivs[m];
+                }
+            }
+        }
+        // Ls rows not covered above (panel rows beyond the threads) are padding
+        for (int r = 256 * RPT - nb + tid; r < lay.kp; r += 256) {
+#pragma unroll
+            for (int c = 0; c < MNB; ++c) Ls[r * lay.ls_stride + c] = (T)0;
+        }
+        __syncthreads();
+
+        // (4) trailing window -= L_below * U_right on the matrix cores
+        const int nt = (Mw + TILE - 1) / TILE;
+        for (int e = wave; e < nt * nt; e += 4) {
+            const int ti = e / nt, tj = e % nt;
+            typename MF::acc_t acc;
+            const int cc = tj * TILE + MF::col(lane);
+            // rows j0+nb+ii of column j0+nb+cc: step dir from the column's row j0+nb
+            T* tp = base + aidx(j0 + nb, j0 + nb + cc);
+#pragma unroll
+            for (int r = 0; r < MF::NACC; ++r) {
+                const int ii = ti * TILE + MF::row(lane, r);
+                acc[r] = (ii < Mw && cc < Mw && ii - cc <= k && cc - ii <= k) ? tp[dir * ii] : (T)0;
+            }
+            const T* la = Ls + (ti * TILE + MF::ab_mn(lane)) * lay.ls_stride + MF::ab_k(lane);
+            const T* ub = Us + MF::ab_k(lane) * lay.us_stride + tj * TILE + MF::ab_mn(lane);
+#pragma unroll
+            for (int kk = 0; kk < MNB; kk += MF::KSTEP)
+                acc = MF::mma(la[kk], ub[kk * lay.us_stride], acc);
+#pragma unroll
+            for (int r = 0; r < MF::NACC; ++r) {
+                const int ii = ti * TILE + MF::row(lane, r);
+                if (ii < Mw && cc < Mw && ii - cc <= k && cc - ii <= k) tp[dir * ii] = acc[r];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <class T>
 __global__ void zero_pivot_kernel(const T* __restrict__ B, int k, int64_t row0, int64_t rows, int* flag)
 {
@@ -441,10 +668,22 @@ constexpr size_t LDS_LIMIT = 160 * 1024 - 1024;   // leave room for static LDS
 
 }  // namespace
 
+template <class T, int RPT>
+void launch_factor_mfma(T* B0, T* B1, int k, const FactorJob* jobs, int njobs, hipStream_t st)
+{
+    const size_t lds = MfmaLds<T>(k).bytes;
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_factor_mfma_kernel<T, RPT>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((band_factor_mfma_kernel<T, RPT>), dim3(njobs), dim3(256), lds, st, B0, B1, k, jobs);
+    SAP_LAUNCH_CHECK();
+}
+
 template <class T>
 void launch_band_factor(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, hipStream_t st)
 {
     if (njobs <= 0) return;
+    const bool fits = MfmaLds<T>(k).bytes <= LDS_LIMIT;
+    if (fits && k + MNB <= 256) { launch_factor_mfma<T, 1>(B0, B1, k, d_jobs, njobs, st); return; }
     if (factor_lds_bytes<T>(k, 32) <= LDS_LIMIT)      launch_factor_nb<T, 32>(B0, B1, k, d_jobs, njobs, st);
     else if (factor_lds_bytes<T>(k, 16) <= LDS_LIMIT) launch_factor_nb<T, 16>(B0, B1, k, d_jobs, njobs, st);
     else if (factor_lds_bytes<T>(k, 8) <= LDS_LIMIT)  launch_factor_nb<T, 8>(B0, B1, k, d_jobs, njobs, st);

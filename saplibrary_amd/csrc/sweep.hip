@@ -112,6 +112,141 @@ __device__ void lower_sweep(const BandView<T>& V, int64_t len, In in, Out out, T
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pipelined variant for k <= 192 + BS (BS = 32 rows per block for FP32, 16
+// for FP64, so two blocks of multipliers fit in registers).  Per block b:
+//   wave 0    solves the BS x BS diagonal block with the lane-broadcast chain
+//             and immediately applies the block to the next BS rows (the
+//             sub-diagonal BS x BS block), so the chain never waits for the
+//             other waves;
+//   waves 1-3 apply the previous block b-1 to its remaining k - BS rows
+//             (rows of block b+1 go to a separate accumulator, later ones to
+//             the LDS ring) and stream the next input rows into the ring.
+// Every wave loads its multipliers for the next block into registers one
+// block ahead, so the loads overlap the chain.  One barrier per block.
+// ---------------------------------------------------------------------------
+template <class T>
+struct SweepRegs {
+    static constexpr int BS = sizeof(T) == 4 ? 32 : 16;
+    T a[BS];     // wave 0: diagonal-block multipliers of its row; waves 1-3: its far row
+    T b[BS];     // wave 0: sub-diagonal block row
+    T d;         // wave 0: pivot of its row (for the D^{-1} scaling)
+};
+
+template <class T>
+constexpr int sweep_bs() { return SweepRegs<T>::BS; }
+
+template <class T, class In, class Out>
+__device__ __forceinline__ void lower_sweep_fast(const BandView<T>& V, int64_t len, In in, Out out, bool scale,
+                                                 T* pend, int ring_mask, T* xblk2, T* acc2)
+{
+    constexpr int BS = sweep_bs<T>();
+    const int k = V.k;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int t = tid - 64;                   // far-row index of waves 1-3
+    const int nfar = k - BS;                  // far rows per block
+
+    // rows below c0 + H can be touched while block b is processed; the input
+    // rows [c0 + H, c0 + H + BS) are streamed in during step b.
+    const int64_t H = BS + max(BS, k);
+    const int64_t init = min(len, H);
+    for (int64_t r = tid; r < init; r += blockDim.x) pend[r & ring_mask] = in(r);
+    if (tid < BS) { acc2[tid] = (T)0; acc2[BS + tid] = (T)0; }
+
+    // Multiplier (r, c0 + j) of the view sits at colp + j * cstep + (r - c0) * dir
+    // with colp = V.base + V.origin + dir * (c0 * w + k) and cstep = dir * (w - 1):
+    // one 64-bit lane pointer per row, uniform steps per column.
+    const int64_t cstep = (int64_t)V.dir * (V.w - 1);
+    auto load = [&](SweepRegs<T>& R, int64_t c0) {
+        const int nb = (int)min((int64_t)BS, len - c0);
+        const T* colp = V.base + V.origin + (int64_t)V.dir * (c0 * V.w + k);
+        if (wave == 0) {
+            const int i = lane;
+            const int64_t r = c0 + i;
+            const bool rv = i < BS && r < len;
+            const T* p = colp + (int64_t)V.dir * i;
+#pragma unroll
+            for (int j = 0; j < BS; ++j) R.a[j] = (rv && j < i && i - j <= k) ? p[j * cstep] : (T)0;
+            const bool sv = i < BS && r + BS < len;
+            const T* q = p + (int64_t)V.dir * BS;
+#pragma unroll
+            for (int j = 0; j < BS; ++j) R.b[j] = (sv && j < nb && BS + i - j <= k) ? q[j * cstep] : (T)0;
+            R.d = (rv && scale) ? p[i * cstep] : (T)1;
+        } else {
+            const int64_t r = c0 + 2 * BS + t;
+            const bool fv = t < nfar && r < len;
+            const T* p = colp + (int64_t)V.dir * (2 * BS + t);
+#pragma unroll
+            for (int j = 0; j < BS; ++j) R.a[j] = (fv && j < nb && 2 * BS + t - j <= k) ? p[j * cstep] : (T)0;
+        }
+    };
+
+    // one pipeline step: block b (wave 0) and block b-1 (waves 1-3)
+    auto step = [&](int64_t b, SweepRegs<T>& cur, SweepRegs<T>& nxt) {
+        const int64_t c0 = b * BS;
+        const int nb = (int)min((int64_t)BS, len - c0);
+        const int par = (int)(b & 1);
+        if (wave == 0) {
+            if (c0 + BS < len) load(nxt, c0 + BS);
+            T x = (lane < nb) ? pend[(c0 + lane) & ring_mask] - acc2[par * BS + lane] : (T)0;
+#pragma unroll
+            for (int j = 0; j < BS - 1; ++j) {
+                const T xj = bcast(x, j);
+                x -= cur.a[j] * xj;
+            }
+            if (lane < nb) {
+                xblk2[par * BS + lane] = x;
+                out(c0 + lane, scale ? x / cur.d : x);
+            }
+            T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+            for (int j = 0; j < BS; j += 4) {
+                s0 += cur.b[j] * bcast(x, j);
+                s1 += cur.b[j + 1] * bcast(x, j + 1);
+                s2 += cur.b[j + 2] * bcast(x, j + 2);
+                s3 += cur.b[j + 3] * bcast(x, j + 3);
+            }
+            const int64_t rn = c0 + BS + lane;
+            if (lane < BS && rn < len) pend[rn & ring_mask] -= (s0 + s1) + (s2 + s3);
+        } else {
+            // far rows of block b-1 (multipliers loaded one step earlier)
+            if (b >= 1) {
+                const int64_t cp = c0 - BS;
+                const T* xp = xblk2 + (1 - par) * BS;
+                T s0 = 0, s1 = 0;
+#pragma unroll
+                for (int j = 0; j < BS; j += 2) {
+                    s0 += cur.a[j] * xp[j];
+                    s1 += cur.a[j + 1] * xp[j + 1];
+                }
+                const T acc = s0 + s1;
+                const int64_t r = cp + 2 * BS + t;
+                if (t < BS) acc2[(1 - par) * BS + t] = (t < nfar && r < len) ? acc : (T)0;
+                else if (t < nfar && r < len) pend[r & ring_mask] -= acc;
+            } else if (t < BS) {
+                acc2[BS + t] = (T)0;
+            }
+            if (c0 < len) load(nxt, c0);
+            // stream the input rows first touched in a later step
+            const int64_t rr = c0 + H + t;
+            if (t < BS && rr < len) pend[rr & ring_mask] = in(rr);
+        }
+        __syncthreads();
+    };
+
+    SweepRegs<T> R0, R1;
+    if (wave == 0) load(R0, 0);
+    __syncthreads();
+    const int64_t nblk = (len + BS - 1) / BS;
+    // waves 1-3 lag one block behind wave 0: one extra step drains them
+    for (int64_t b = 0; b <= nblk; b += 2) {
+        step(b, R0, R1);
+        if (b + 1 <= nblk) step(b + 1, R1, R0);
+    }
+}
+
 inline int ring_size(int k)
 {
     int r = 1;
@@ -133,7 +268,7 @@ __device__ inline T* lds_ring()
 // g receives those 2k values per boundary at their row positions;
 // scr is a length-n scratch vector per side.
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, bool FAST>
 __global__ __launch_bounds__(256) void sweep_interface_kernel(const T* __restrict__ B, const T* __restrict__ Bul,
                                                               int k, PartInfo pi, const double* __restrict__ v,
                                                               T* __restrict__ g, T* __restrict__ scr0,
@@ -141,6 +276,8 @@ __global__ __launch_bounds__(256) void sweep_interface_kernel(const T* __restric
 {
     T* pend = lds_ring<T>();
     __shared__ T xblk[NBS];
+    __shared__ T xblk2[2 * NBS];
+    __shared__ T acc2[2 * NBS];
     const int i = blockIdx.x >> 1;
     const int side = blockIdx.x & 1;
     const int p = i + side;
@@ -154,14 +291,21 @@ __global__ __launch_bounds__(256) void sweep_interface_kernel(const T* __restric
     // local row r of a view with direction d -> global row
     auto grow = [&](int d, int64_t r) -> int64_t { return d > 0 ? s + r : s + len - 1 - r; };
 
-    lower_sweep<T>(
-        V1, len, [&](int64_t r) -> T { return (T)v[grow(d1, r)]; },
-        [&](int64_t r, T y) { scr[grow(d1, r)] = y / V1.at(r, r); }, pend, ring_mask, xblk);
-    __syncthreads();
+    auto in1 = [&](int64_t r) -> T { return (T)v[grow(d1, r)]; };
+    auto in2 = [&](int64_t r) -> T { return scr[grow(-d1, r)]; };
+    auto out2 = [&](int64_t r, T y) { g[grow(-d1, r)] = y; };
     const int64_t kl = min((int64_t)k, len);
-    lower_sweep<T>(
-        V2, kl, [&](int64_t r) -> T { return scr[grow(-d1, r)]; },
-        [&](int64_t r, T y) { g[grow(-d1, r)] = y; }, pend, ring_mask, xblk);
+    if constexpr (FAST) {
+        lower_sweep_fast<T>(V1, len, in1, [&](int64_t r, T y) { scr[grow(d1, r)] = y; }, true, pend, ring_mask,
+                            xblk2, acc2);
+        __syncthreads();
+        lower_sweep_fast<T>(V2, kl, in2, out2, false, pend, ring_mask, xblk2, acc2);
+    } else {
+        lower_sweep<T>(V1, len, in1, [&](int64_t r, T y) { scr[grow(d1, r)] = y / V1.at(r, r); }, pend, ring_mask,
+                       xblk);
+        __syncthreads();
+        lower_sweep<T>(V2, kl, in2, out2, pend, ring_mask, xblk);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -170,7 +314,7 @@ __global__ __launch_bounds__(256) void sweep_interface_kernel(const T* __restric
 // UL-factored last partition the mirror order.  corr (2k per boundary) may
 // be null.  Output FP64 (outd) or preconditioner precision (outt).
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, bool FAST>
 __global__ __launch_bounds__(256) void sweep_full_kernel(const T* __restrict__ B, const T* __restrict__ Bul, int k,
                                                          PartInfo pi, int last_is_ul, const double* __restrict__ v,
                                                          const T* __restrict__ corr, double* __restrict__ outd,
@@ -178,6 +322,8 @@ __global__ __launch_bounds__(256) void sweep_full_kernel(const T* __restrict__ B
 {
     T* pend = lds_ring<T>();
     __shared__ T xblk[NBS];
+    __shared__ T xblk2[2 * NBS];
+    __shared__ T acc2[2 * NBS];
     const int p = blockIdx.x;
     int64_t s, len;
     pi.bounds(p, s, len);
@@ -197,17 +343,26 @@ __global__ __launch_bounds__(256) void sweep_full_kernel(const T* __restrict__ B
         if (has_bot && lr >= len - k) val -= corr[(size_t)p * 2 * k + (lr - (len - k))];
         return val;
     };
-    lower_sweep<T>(
-        V1, len, rhs, [&](int64_t r, T y) { scr[grow(d1, r)] = y / V1.at(r, r); }, pend, ring_mask, xblk);
-    __syncthreads();
-    if (outd) {
-        lower_sweep<T>(
-            V2, len, [&](int64_t r) -> T { return scr[grow(-d1, r)]; },
-            [&](int64_t r, T y) { outd[grow(-d1, r)] = (double)y; }, pend, ring_mask, xblk);
+    auto in2 = [&](int64_t r) -> T { return scr[grow(-d1, r)]; };
+    if constexpr (FAST) {
+        lower_sweep_fast<T>(V1, len, rhs, [&](int64_t r, T y) { scr[grow(d1, r)] = y; }, true, pend, ring_mask, xblk2,
+                            acc2);
+        __syncthreads();
+        if (outd)
+            lower_sweep_fast<T>(V2, len, in2, [&](int64_t r, T y) { outd[grow(-d1, r)] = (double)y; }, false, pend,
+                                ring_mask, xblk2, acc2);
+        else
+            lower_sweep_fast<T>(V2, len, in2, [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, false, pend, ring_mask,
+                                xblk2, acc2);
     } else {
-        lower_sweep<T>(
-            V2, len, [&](int64_t r) -> T { return scr[grow(-d1, r)]; },
-            [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, pend, ring_mask, xblk);
+        lower_sweep<T>(V1, len, rhs, [&](int64_t r, T y) { scr[grow(d1, r)] = y / V1.at(r, r); }, pend, ring_mask,
+                       xblk);
+        __syncthreads();
+        if (outd)
+            lower_sweep<T>(V2, len, in2, [&](int64_t r, T y) { outd[grow(-d1, r)] = (double)y; }, pend, ring_mask,
+                           xblk);
+        else
+            lower_sweep<T>(V2, len, in2, [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, pend, ring_mask, xblk);
     }
 }
 
@@ -277,16 +432,30 @@ size_t sweep_lds(int k)
 }  // namespace
 
 template <class T>
+bool sweep_fast(int k)
+{
+    return k <= 192 + sweep_bs<T>();
+}
+
+template <class T, bool FAST>
+void launch_sweep_interface_t(const T* B, const T* Bul, int k, PartInfo pi, const double* v, T* g, T* scr0, T* scr1,
+                              hipStream_t st)
+{
+    const size_t lds = sweep_lds<T>(k);
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)sweep_interface_kernel<T, FAST>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((sweep_interface_kernel<T, FAST>), dim3(2 * (pi.P - 1)), dim3(256), lds, st, B, Bul, k, pi, v,
+                       g, scr0, scr1, ring_size(k) - 1);
+    SAP_LAUNCH_CHECK();
+}
+
+template <class T>
 void launch_sweep_interface(const T* B, const T* Bul, int k, PartInfo pi, const double* v, T* g, T* scr0, T* scr1,
                             hipStream_t st)
 {
     if (pi.P < 2) return;
-    const size_t lds = sweep_lds<T>(k);
-    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)sweep_interface_kernel<T>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(sweep_interface_kernel<T>, dim3(2 * (pi.P - 1)), dim3(256), lds, st, B, Bul, k, pi, v, g, scr0,
-                       scr1, ring_size(k) - 1);
-    SAP_LAUNCH_CHECK();
+    if (sweep_fast<T>(k)) launch_sweep_interface_t<T, true>(B, Bul, k, pi, v, g, scr0, scr1, st);
+    else                  launch_sweep_interface_t<T, false>(B, Bul, k, pi, v, g, scr0, scr1, st);
 }
 
 template <class T>
@@ -299,16 +468,24 @@ void launch_interface_solve(const T* WV, const T* Sinv, const T* offB, const T* 
     SAP_LAUNCH_CHECK();
 }
 
+template <class T, bool FAST>
+void launch_sweep_full_t(const T* B, const T* Bul, int k, PartInfo pi, int last_is_ul, const double* v, const T* corr,
+                         double* outd, T* outt, T* scr, hipStream_t st)
+{
+    const size_t lds = sweep_lds<T>(k);
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)sweep_full_kernel<T, FAST>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((sweep_full_kernel<T, FAST>), dim3(pi.P), dim3(256), lds, st, B, Bul, k, pi, last_is_ul, v,
+                       corr, outd, outt, scr, ring_size(k) - 1);
+    SAP_LAUNCH_CHECK();
+}
+
 template <class T>
 void launch_sweep_full(const T* B, const T* Bul, int k, PartInfo pi, int last_is_ul, const double* v, const T* corr,
                        double* outd, T* outt, T* scr, hipStream_t st)
 {
-    const size_t lds = sweep_lds<T>(k);
-    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)sweep_full_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds));
-    hipLaunchKernelGGL(sweep_full_kernel<T>, dim3(pi.P), dim3(256), lds, st, B, Bul, k, pi, last_is_ul, v, corr, outd,
-                       outt, scr, ring_size(k) - 1);
-    SAP_LAUNCH_CHECK();
+    if (sweep_fast<T>(k)) launch_sweep_full_t<T, true>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, st);
+    else                  launch_sweep_full_t<T, false>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, st);
 }
 
 template <class T>

@@ -106,7 +106,10 @@ def test_factors_match_oracle(sap, oracle, n, k, P, fact, precond, prec_double):
 
 # ------------------------------------------------------------------- apply
 APPLY_CASES = FACTOR_CASES + [(20000, 100, 16, "LU_UL", "Spike"), (5000, 0, 4, "LU_UL", "Spike"),
-                              (3000, 10, 3, "LU_UL", "None")]
+                              (3000, 10, 3, "LU_UL", "None"),
+                              # k above the pipelined-sweep limit (224 FP32, 208 FP64) and between them
+                              (5000, 300, 4, "LU_UL", "Spike"), (8000, 210, 8, "LU_UL", "Spike"),
+                              (7000, 230, 3, "LU_only", "Spike")]
 
 
 @pytest.mark.parametrize("n,k,P,fact,precond", APPLY_CASES)
