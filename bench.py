@@ -70,6 +70,31 @@ def algorithmic(name, n, k, P, sp, spike):
     return None, None, None
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c1_sapc_fp32_pmc_traffic.md")
+FAMILY_KERNEL = {"band_factor": "band_factor_mfma_kernel", "sweep_full": "sweep_full_kernel",
+                 "sweep_interface": "sweep_interface_kernel", "spmv": "band_spmv_kernel",
+                 "band_to_colmajor": "band_to_colmajor_kernel"}
+
+
+def measured_traffic(family):
+    """HBM bytes per launch (read + write) of a kernel family, from the
+    committed PMC summary (FETCH_SIZE/WRITE_SIZE runs, copy-calibrated)."""
+    if family not in FAMILY_KERNEL or not os.path.exists(TRAFFIC_FILE):
+        return None
+    in_table = False
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("| kernel | HBM read"):
+            in_table = True
+            continue
+        if in_table and line.startswith("|") and FAMILY_KERNEL[family] in line:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            try:
+                return float(cells[1]) + float(cells[2])
+            except ValueError:
+                return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +104,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device-to-device (a known-size kernel for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build runs on one GPU "
@@ -87,6 +114,14 @@ def main():
 
     import torch
     import saplibrary_amd as sap
+
+    if args.copy_calibration:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] calibration copy: 1073741824 bytes read, 1073741824 bytes written", flush=True)
+        del src, dst
 
     n, k, dom, P, sa, sx = WORKLOADS[args.workload]
     spike = args.workload == "c1"
@@ -153,8 +188,10 @@ def main():
         else:
             achieved = algo / avg_s / 1e12
             peak = FP32_MFMA_PEAK_TFLOPS
+        traffic = measured_traffic(name)
         roof = {"kernel": name, "bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": unit,
-                "frac": round(achieved / peak, 4), "traffic": None,
+                "frac": round(achieved / peak, 4), "traffic": traffic,
+                "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None,
                 "avg_launch_ms": round(total_ms / launches, 4), "launches_per_step": launches / steps_profiled,
                 "algorithmic_per_launch": algo}
     kernels = {kname: {"launches_per_step": round(l / steps_profiled, 2),

@@ -71,3 +71,28 @@ def test_create_rejects_unsupported_options(sap):
     for kw in ({"useBCR": 1}, {"ilu_level": 0}, {"gpuCount": 2}, {"solverType": sap.GMRES_C}):
         with pytest.raises(sap.SapError):
             sap.Solver(4, sap.Options(**kw))
+
+
+def test_cpp_facade_compiles(tmp_path):
+    # include/sap/solver.hpp: the reference-style C++ interface over the ABI
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    src = tmp_path / "facade.cpp"
+    src.write_text(
+        '#include "sap/solver.hpp"\n'
+        "int main() {\n"
+        "  typedef sap::DeviceArray<double> Vec;\n"
+        "  sap::Options opts; opts.factMethod = sap::LU_UL; opts.relTol = 1e-10;\n"
+        "  Vec band(30), b(10), x(10);\n"
+        "  sap::BandedMatrix<Vec> A(10, 1, band);\n"
+        "  sap::Solver<Vec, float> solver(2, opts);\n"
+        "  sap::MVBanded<Vec> spmv(A);\n"
+        "  solver.setup(A);\n"
+        "  return solver.solve(spmv, b, x) ? 0 : 1;\n"
+        "}\n")
+    r = subprocess.run([hipcc, "-std=c++17", "-I", os.path.join(ROOT, "include"), "-c", str(src), "-o",
+                        str(tmp_path / "facade.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
