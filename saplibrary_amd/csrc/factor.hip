@@ -13,16 +13,6 @@ namespace sap_amd {
 
 namespace {
 
-// Pivot boosting (sap/device/factor_band_const.cuh:29-40, thresholds
-// BURST_VALUE = 1e-7 and BURST_NEW_VALUE = 1e-4 of sap/common.h:46-47).
-template <class T>
-__device__ inline T boost_pivot(T v)
-{
-    const T thr = (T)1e-7, nv = (T)1e-4;
-    if (v > thr || v < -thr) return v;
-    return v < (T)0 ? -nv : nv;
-}
-
 // ---------------------------------------------------------------------------
 // Blocked right-looking banded LU of one partition per workgroup.
 // Panel of NB columns staged in LDS together with the NB x k block of U rows
@@ -682,6 +672,7 @@ template <class T>
 void launch_band_factor(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, hipStream_t st)
 {
     if (njobs <= 0) return;
+    if (launch_band_factor_resident<T>(B0, B1, k, d_jobs, njobs, st)) return;
     const bool fits = MfmaLds<T>(k).bytes <= LDS_LIMIT;
     if (fits && k + MNB <= 256) { launch_factor_mfma<T, 1>(B0, B1, k, d_jobs, njobs, st); return; }
     if (factor_lds_bytes<T>(k, 32) <= LDS_LIMIT)      launch_factor_nb<T, 32>(B0, B1, k, d_jobs, njobs, st);

@@ -69,7 +69,9 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
 
     // (1) column-major band in the preconditioner precision
     SAP_HIP_CHECK(hipEventRecord(e0, stream));
-    B_.resize((size_t)w * n);
+    // 8 elements of slack: the sweeps copy 16-byte chunks covering each
+    // column segment, which may reach a few elements past the last column
+    B_.resize((size_t)w * n + 8);
     {
         ProfScope ps(prof, "band_to_colmajor");
         launch_band_to_colmajor<T>(n, k, d_band, B_.get(), stream);
@@ -77,7 +79,7 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
     SAP_HIP_CHECK(hipEventRecord(e1, stream));
     time_toBanded = elapsed_ms(e0, e1);
 
-    Bul_.release();
+    if (!lu_ul()) Bul_.release();   // kept across setups otherwise (same size)
     offB_.release(); offC_.release(); WV_.release(); S_.release(); Sinv_.release(); corr_.release();
     g_.resize(n);
     scr0_.resize(n);
@@ -123,7 +125,7 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
     int64_t s_last, l_last;
     pi.bounds(P - 1, s_last, l_last);
     if (lu_ul()) {
-        Bul_.resize((size_t)w * n);
+        Bul_.resize((size_t)w * n + 8);
         SAP_HIP_CHECK(hipMemcpyAsync(Bul_.get(), B_.get(), sizeof(T) * w * n, hipMemcpyDeviceToDevice, stream));
         for (int p = 0; p < P - 1; ++p) {
             int64_t s, len;

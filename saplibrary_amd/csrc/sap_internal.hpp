@@ -107,6 +107,16 @@ struct ProfScope {
     ~ProfScope() { if (p && p->enabled) p->end(); }
 };
 
+// Pivot boosting (sap/device/factor_band_const.cuh:29-40, thresholds
+// BURST_VALUE = 1e-7 and BURST_NEW_VALUE = 1e-4 of sap/common.h:46-47).
+template <class T>
+__device__ inline T boost_pivot(T v)
+{
+    const T thr = (T)1e-7, nv = (T)1e-4;
+    if (v > thr || v < -thr) return v;
+    return v < (T)0 ? -nv : nv;
+}
+
 // ---------------------------------------------------------------- launchers
 // band.hip
 void launch_gen_banded(int64_t n, int k, double dom, uint64_t seed, double* band, hipStream_t st);
@@ -133,6 +143,10 @@ struct FactorJob {          // one banded LU (dir = +1) or UL (dir = -1) of a pa
 };
 template <class T>
 void launch_band_factor(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, hipStream_t st);
+// factor_resident.hip: register-resident MFMA factorization; false if k is
+// too wide for it (the caller then uses the LDS-blocked kernels)
+template <class T>
+bool launch_band_factor_resident(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, hipStream_t st);
 template <class T>
 void launch_zero_pivot_check(const T* B, int k, int64_t row0, int64_t rows, int* d_flag, hipStream_t st);
 template <class T>

@@ -24,6 +24,8 @@
 // 2*64 + k rows live in an LDS ring.
 #include "sap_internal.hpp"
 
+#include <algorithm>
+
 namespace sap_amd {
 
 namespace {
@@ -247,6 +249,203 @@ __device__ __forceinline__ void lower_sweep_fast(const BandView<T>& V, int64_t l
     }
 }
 
+// ---------------------------------------------------------------------------
+// LDS-prefetched variant (k <= 192 + BS, 512 threads).  The entries a block
+// of BS columns needs (column c: its diagonal and the k entries below it,
+// contiguous in the band storage) are copied HBM -> LDS by four producer
+// wavefronts with global_load_lds_dwordx4, as 16-byte chunks covering each
+// column segment, up to NBUF - 2 blocks ahead of the solve; the solve then
+// runs at the pace of its dependency chain instead of paying one HBM latency
+// per block.  In step b:
+//   wave 0     diagonal-block chain and sub-diagonal block of block b (as in
+//              lower_sweep_fast), multipliers from buffer b;
+//   waves 1-3  far rows of block b-1 from buffer b-1, and the input stream;
+//   waves 4-7  producers, loading block b + NBUF - 2 into the buffer of
+//              block b-2.
+// ---------------------------------------------------------------------------
+constexpr int SW_PROD = 4;           // producer wavefronts
+constexpr int SW_THREADS = 512;
+
+// s_waitcnt takes an immediate: dispatch on a wave-uniform count (<= 63).
+// gfx9 encoding: vmcnt bits [3:0] and [15:14]; expcnt [6:4] and lgkmcnt
+// [11:8] left at their maxima (no wait).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt_imm()
+{
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt_le(int n)
+{
+    if constexpr (N == 0) {
+        wait_vmcnt_imm<0>();
+    } else {
+        if (n >= N) wait_vmcnt_imm<N>();
+        else wait_vmcnt_le<N - 1>(n);
+    }
+}
+
+__device__ __forceinline__ void wait_vmcnt(int n) { wait_vmcnt_le<63>(n); }
+
+// Workgroup barrier for data exchanged through LDS only: unlike
+// __syncthreads() it does not wait for this wavefront's outstanding global
+// loads and stores, so the producers' LDS-DMA prefetches stay in flight.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Chunk geometry of one view for the LDS-prefetched sweep.
+template <class T>
+struct SweepChunks {
+    static constexpr int E = 16 / sizeof(T);   // elements per 16-byte chunk
+    int nch;      // chunks per column (covers any alignment of a (k+1)-segment)
+    int segl;     // elements per column in LDS = nch * E
+    int ncb;      // chunks per block, padded to a whole number of producer rounds
+    int ni;       // load instructions per producer wavefront per block
+    __host__ __device__ SweepChunks(int k, int bs)
+    {
+        nch = (k + E - 1) / E + 1;
+        segl = nch * E;
+        const int round = 64 * SW_PROD;
+        ncb = ((bs * nch + round - 1) / round) * round;
+        ni = ncb / round;
+    }
+    __host__ __device__ size_t buf_elems() const { return (size_t)ncb * E; }
+};
+
+template <class T, class In, class Out>
+__device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t len, In in, Out out, bool scale,
+                                                T* pend, int ring_mask, T* xblk2, T* acc2, T* bufs, int nbuf)
+{
+    constexpr int BS = sweep_bs<T>();
+    constexpr int E = SweepChunks<T>::E;
+    const SweepChunks<T> G(V.k, BS);
+    const int k = V.k;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int t = tid - 64;                   // far-row index of waves 1-3
+    const int nfar = k - BS;                  // far rows per block
+    const int64_t nblk = (len + BS - 1) / BS;
+    const size_t bufe = G.buf_elems();
+
+    const int64_t H = BS + max(BS, k);
+    const int64_t init = min(len, H);
+    for (int64_t r = tid; r < init; r += blockDim.x) pend[r & ring_mask] = in(r);
+    if (tid < BS) { acc2[tid] = (T)0; acc2[BS + tid] = (T)0; }
+
+    // Segment of view column c: elements q = 0..k at V.origin + dir * (c*w + k + q);
+    // lo = its lowest address, lo & ~(E-1) the first chunk.  In LDS, element q of
+    // block column j is at j*segl + off + dir*q, off = (lo & (E-1)) + (dir < 0 ? k : 0).
+    auto seg_lo = [&](int64_t c) -> int64_t {
+        const int64_t a = V.origin + (int64_t)V.dir * (c * V.w + k);
+        return V.dir > 0 ? a : a - k;
+    };
+    auto seg_off = [&](int64_t c) -> int { return (int)(seg_lo(c) & (E - 1)) + (V.dir < 0 ? k : 0); };
+
+    // producer: chunks of block b into buffer b % nbuf (columns past the end
+    // and padding lanes re-read column 0's first chunk: in bounds, never used)
+    const int pw = wave - 4;
+    auto produce = [&](int64_t b) {
+        T* dst = bufs + (size_t)(b % nbuf) * bufe;
+        const int64_t c0 = b * BS;
+        const int64_t lo_first = seg_lo(c0) & ~(int64_t)(E - 1);
+        for (int i = 0; i < G.ni; ++i) {
+            const int rnd = i * SW_PROD + pw;               // 64-chunk round of this block
+            const int g = rnd * 64 + lane;
+            const int j = g / G.nch, ch = g - j * G.nch;
+            int64_t src = lo_first;
+            if (g < BS * G.nch && c0 + j < len) src = (seg_lo(c0 + j) & ~(int64_t)(E - 1)) + (int64_t)ch * E;
+            __builtin_amdgcn_global_load_lds(V.base + src, dst + (size_t)rnd * 64 * E, 16, 0, 0);
+        }
+    };
+    auto produced_after = [&](int64_t b) -> int {   // this wave's loads issued for blocks > b
+        const int64_t last = min(nblk - 1, b + nbuf - 3);
+        const int64_t cnt = max((int64_t)0, last - b) * G.ni;
+        return (int)min(cnt, (int64_t)63);
+    };
+
+    if (wave >= 4) {
+        for (int64_t b = 0; b < min(nblk, (int64_t)nbuf - 2); ++b) produce(b);
+        wait_vmcnt(produced_after(0));
+    }
+    lds_barrier();
+
+    auto step = [&](int64_t b) {
+        const int64_t c0 = b * BS;
+        const int nb = (int)min((int64_t)BS, len - c0);
+        const int par = (int)(b & 1);
+        if (wave == 0) {
+            if (c0 < len) {
+                const T* buf = bufs + (size_t)(b % nbuf) * bufe;
+                const int i = lane;
+                const bool rv = i < nb;
+                const bool sv = i < BS && c0 + BS + i < len;
+                T a[BS], sb[BS];
+#pragma unroll
+                for (int j = 0; j < BS; ++j) {
+                    const int off = j * G.segl + seg_off(c0 + j);
+                    a[j] = (rv && j < i && i - j <= k) ? buf[off + V.dir * (i - j)] : (T)0;
+                    sb[j] = (sv && j < nb && BS + i - j <= k) ? buf[off + V.dir * (BS + i - j)] : (T)0;
+                }
+                const T d = (rv && scale) ? buf[(i < BS ? i : 0) * G.segl + seg_off(c0 + (i < BS ? i : 0))] : (T)1;
+                T x = rv ? pend[(c0 + lane) & ring_mask] - acc2[par * BS + lane] : (T)0;
+#pragma unroll
+                for (int j = 0; j < BS - 1; ++j) {
+                    const T xj = bcast(x, j);
+                    x -= a[j] * xj;
+                }
+                if (rv) {
+                    xblk2[par * BS + lane] = x;
+                    out(c0 + lane, scale ? x / d : x);
+                }
+                T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+                for (int j = 0; j < BS; j += 4) {
+                    s0 += sb[j] * bcast(x, j);
+                    s1 += sb[j + 1] * bcast(x, j + 1);
+                    s2 += sb[j + 2] * bcast(x, j + 2);
+                    s3 += sb[j + 3] * bcast(x, j + 3);
+                }
+                if (sv) pend[(c0 + BS + lane) & ring_mask] -= (s0 + s1) + (s2 + s3);
+            }
+        } else if (wave < 4) {
+            if (b >= 1) {
+                const int64_t cp = c0 - BS;
+                const T* buf = bufs + (size_t)((b - 1) % nbuf) * bufe;
+                const T* xp = xblk2 + (1 - par) * BS;
+                const int64_t r = cp + 2 * BS + t;
+                const bool fv = t < nfar && r < len;
+                T s0 = 0, s1 = 0;
+                if (fv) {   // row r < len: block b-1 is a full block
+#pragma unroll
+                    for (int j = 0; j < BS; j += 2) {
+                        const int q0 = 2 * BS + t - j, q1 = q0 - 1;
+                        if (q0 <= k) s0 += buf[j * G.segl + seg_off(cp + j) + V.dir * q0] * xp[j];
+                        if (q1 <= k) s1 += buf[(j + 1) * G.segl + seg_off(cp + j + 1) + V.dir * q1] * xp[j + 1];
+                    }
+                }
+                const T acc = s0 + s1;
+                if (t < BS) acc2[(1 - par) * BS + t] = fv ? acc : (T)0;
+                else if (fv) pend[r & ring_mask] -= acc;
+            } else if (t < BS) {
+                acc2[BS + t] = (T)0;
+            }
+            const int64_t rr = c0 + H + t;
+            if (t < BS && rr < len) pend[rr & ring_mask] = in(rr);
+        } else {
+            if (b + nbuf - 2 < nblk) produce(b + nbuf - 2);
+            wait_vmcnt(produced_after(b + 1));
+        }
+        lds_barrier();
+    };
+
+    // waves 1-3 lag one block behind wave 0: one extra step drains them
+    for (int64_t b = 0; b <= nblk; ++b) step(b);
+}
+
 inline int ring_size(int k)
 {
     int r = 1;
@@ -268,11 +467,12 @@ __device__ inline T* lds_ring()
 // g receives those 2k values per boundary at their row positions;
 // scr is a length-n scratch vector per side.
 // ---------------------------------------------------------------------------
-template <class T, bool FAST>
-__global__ __launch_bounds__(256) void sweep_interface_kernel(const T* __restrict__ B, const T* __restrict__ Bul,
-                                                              int k, PartInfo pi, const double* __restrict__ v,
-                                                              T* __restrict__ g, T* __restrict__ scr0,
-                                                              T* __restrict__ scr1, int ring_mask)
+// MODE: 0 simple blocked sweep, 1 register-pipelined, 2 LDS-prefetched
+// (512 threads; nbuf block buffers after the ring in dynamic LDS).
+template <class T, int MODE>
+__global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_interface_kernel(
+    const T* __restrict__ B, const T* __restrict__ Bul, int k, PartInfo pi, const double* __restrict__ v,
+    T* __restrict__ g, T* __restrict__ scr0, T* __restrict__ scr1, int ring_mask, int nbuf)
 {
     T* pend = lds_ring<T>();
     __shared__ T xblk[NBS];
@@ -295,7 +495,13 @@ __global__ __launch_bounds__(256) void sweep_interface_kernel(const T* __restric
     auto in2 = [&](int64_t r) -> T { return scr[grow(-d1, r)]; };
     auto out2 = [&](int64_t r, T y) { g[grow(-d1, r)] = y; };
     const int64_t kl = min((int64_t)k, len);
-    if constexpr (FAST) {
+    if constexpr (MODE == 2) {
+        T* bufs = pend + ring_mask + 1;
+        lower_sweep_lds<T>(V1, len, in1, [&](int64_t r, T y) { scr[grow(d1, r)] = y; }, true, pend, ring_mask,
+                           xblk2, acc2, bufs, nbuf);
+        __syncthreads();
+        lower_sweep_lds<T>(V2, kl, in2, out2, false, pend, ring_mask, xblk2, acc2, bufs, nbuf);
+    } else if constexpr (MODE == 1) {
         lower_sweep_fast<T>(V1, len, in1, [&](int64_t r, T y) { scr[grow(d1, r)] = y; }, true, pend, ring_mask,
                             xblk2, acc2);
         __syncthreads();
@@ -314,11 +520,11 @@ __global__ __launch_bounds__(256) void sweep_interface_kernel(const T* __restric
 // UL-factored last partition the mirror order.  corr (2k per boundary) may
 // be null.  Output FP64 (outd) or preconditioner precision (outt).
 // ---------------------------------------------------------------------------
-template <class T, bool FAST>
-__global__ __launch_bounds__(256) void sweep_full_kernel(const T* __restrict__ B, const T* __restrict__ Bul, int k,
-                                                         PartInfo pi, int last_is_ul, const double* __restrict__ v,
-                                                         const T* __restrict__ corr, double* __restrict__ outd,
-                                                         T* __restrict__ outt, T* __restrict__ scr, int ring_mask)
+template <class T, int MODE>
+__global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_full_kernel(
+    const T* __restrict__ B, const T* __restrict__ Bul, int k, PartInfo pi, int last_is_ul,
+    const double* __restrict__ v, const T* __restrict__ corr, double* __restrict__ outd, T* __restrict__ outt,
+    T* __restrict__ scr, int ring_mask, int nbuf)
 {
     T* pend = lds_ring<T>();
     __shared__ T xblk[NBS];
@@ -344,7 +550,18 @@ __global__ __launch_bounds__(256) void sweep_full_kernel(const T* __restrict__ B
         return val;
     };
     auto in2 = [&](int64_t r) -> T { return scr[grow(-d1, r)]; };
-    if constexpr (FAST) {
+    if constexpr (MODE == 2) {
+        T* bufs = pend + ring_mask + 1;
+        lower_sweep_lds<T>(V1, len, rhs, [&](int64_t r, T y) { scr[grow(d1, r)] = y; }, true, pend, ring_mask, xblk2,
+                           acc2, bufs, nbuf);
+        __syncthreads();
+        if (outd)
+            lower_sweep_lds<T>(V2, len, in2, [&](int64_t r, T y) { outd[grow(-d1, r)] = (double)y; }, false, pend,
+                               ring_mask, xblk2, acc2, bufs, nbuf);
+        else
+            lower_sweep_lds<T>(V2, len, in2, [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, false, pend, ring_mask,
+                               xblk2, acc2, bufs, nbuf);
+    } else if constexpr (MODE == 1) {
         lower_sweep_fast<T>(V1, len, rhs, [&](int64_t r, T y) { scr[grow(d1, r)] = y; }, true, pend, ring_mask, xblk2,
                             acc2);
         __syncthreads();
@@ -437,15 +654,33 @@ bool sweep_fast(int k)
     return k <= 192 + sweep_bs<T>();
 }
 
-template <class T, bool FAST>
-void launch_sweep_interface_t(const T* B, const T* Bul, int k, PartInfo pi, const double* v, T* g, T* scr0, T* scr1,
-                              hipStream_t st)
+// Sweep configuration: mode and (mode 2) number of LDS block buffers, as
+// many as fit next to the ring (at most 8) within the 63 outstanding loads a
+// producer wavefront can track.
+struct SweepCfg { int mode, nbuf; size_t lds; int threads; };
+
+template <class T>
+SweepCfg sweep_cfg(int k)
 {
-    const size_t lds = sweep_lds<T>(k);
-    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)sweep_interface_kernel<T, FAST>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((sweep_interface_kernel<T, FAST>), dim3(2 * (pi.P - 1)), dim3(256), lds, st, B, Bul, k, pi, v,
-                       g, scr0, scr1, ring_size(k) - 1);
+    constexpr size_t LDS_MAX = 160 * 1024 - 4096;   // static LDS of the kernels + margin
+    const size_t ring = sweep_lds<T>(k);
+    if (!sweep_fast<T>(k)) return {0, 0, ring, 256};
+    const SweepChunks<T> G(k, sweep_bs<T>());
+    const size_t bb = G.buf_elems() * sizeof(T);
+    int nbuf = (int)std::min<size_t>(8, (LDS_MAX - ring) / bb);
+    nbuf = std::min(nbuf, 63 / G.ni + 3);
+    if (nbuf < 4) return {1, 0, ring, 256};
+    return {2, nbuf, ring + nbuf * bb, SW_THREADS};
+}
+
+template <class T, int MODE>
+void launch_sweep_interface_t(const T* B, const T* Bul, int k, PartInfo pi, const double* v, T* g, T* scr0, T* scr1,
+                              const SweepCfg& c, hipStream_t st)
+{
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)sweep_interface_kernel<T, MODE>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds));
+    hipLaunchKernelGGL((sweep_interface_kernel<T, MODE>), dim3(2 * (pi.P - 1)), dim3(c.threads), c.lds, st, B, Bul,
+                       k, pi, v, g, scr0, scr1, ring_size(k) - 1, c.nbuf);
     SAP_LAUNCH_CHECK();
 }
 
@@ -454,8 +689,10 @@ void launch_sweep_interface(const T* B, const T* Bul, int k, PartInfo pi, const 
                             hipStream_t st)
 {
     if (pi.P < 2) return;
-    if (sweep_fast<T>(k)) launch_sweep_interface_t<T, true>(B, Bul, k, pi, v, g, scr0, scr1, st);
-    else                  launch_sweep_interface_t<T, false>(B, Bul, k, pi, v, g, scr0, scr1, st);
+    const SweepCfg c = sweep_cfg<T>(k);
+    if (c.mode == 2)      launch_sweep_interface_t<T, 2>(B, Bul, k, pi, v, g, scr0, scr1, c, st);
+    else if (c.mode == 1) launch_sweep_interface_t<T, 1>(B, Bul, k, pi, v, g, scr0, scr1, c, st);
+    else                  launch_sweep_interface_t<T, 0>(B, Bul, k, pi, v, g, scr0, scr1, c, st);
 }
 
 template <class T>
@@ -468,15 +705,14 @@ void launch_interface_solve(const T* WV, const T* Sinv, const T* offB, const T* 
     SAP_LAUNCH_CHECK();
 }
 
-template <class T, bool FAST>
+template <class T, int MODE>
 void launch_sweep_full_t(const T* B, const T* Bul, int k, PartInfo pi, int last_is_ul, const double* v, const T* corr,
-                         double* outd, T* outt, T* scr, hipStream_t st)
+                         double* outd, T* outt, T* scr, const SweepCfg& c, hipStream_t st)
 {
-    const size_t lds = sweep_lds<T>(k);
-    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)sweep_full_kernel<T, FAST>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((sweep_full_kernel<T, FAST>), dim3(pi.P), dim3(256), lds, st, B, Bul, k, pi, last_is_ul, v,
-                       corr, outd, outt, scr, ring_size(k) - 1);
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)sweep_full_kernel<T, MODE>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds));
+    hipLaunchKernelGGL((sweep_full_kernel<T, MODE>), dim3(pi.P), dim3(c.threads), c.lds, st, B, Bul, k, pi,
+                       last_is_ul, v, corr, outd, outt, scr, ring_size(k) - 1, c.nbuf);
     SAP_LAUNCH_CHECK();
 }
 
@@ -484,8 +720,10 @@ template <class T>
 void launch_sweep_full(const T* B, const T* Bul, int k, PartInfo pi, int last_is_ul, const double* v, const T* corr,
                        double* outd, T* outt, T* scr, hipStream_t st)
 {
-    if (sweep_fast<T>(k)) launch_sweep_full_t<T, true>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, st);
-    else                  launch_sweep_full_t<T, false>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, st);
+    const SweepCfg c = sweep_cfg<T>(k);
+    if (c.mode == 2)      launch_sweep_full_t<T, 2>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, c, st);
+    else if (c.mode == 1) launch_sweep_full_t<T, 1>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, c, st);
+    else                  launch_sweep_full_t<T, 0>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, c, st);
 }
 
 template <class T>
