@@ -1,5 +1,5 @@
 // Register-resident MFMA banded LU/UL of the partitions (the factorization
-// used for half-bandwidths up to 224 in FP32 and 128 in FP64).
+// used for half-bandwidths up to 224 in FP32 and 112 in FP64).
 //
 // Band layout (column-major, as the reference's m_B, sap/device/
 // data_transfer.cuh:440-447): A(r, c) at B[c*(2k+1) + k + r - c]; a UL
@@ -23,17 +23,22 @@ namespace {
 // every band entry is read from HBM once and every factor entry written once.
 // Per panel:
 //   A.  slots holding the panel columns and the U rows are copied to LDS and
-//       start loading the entering tiles (used one panel later);
+//       start loading the entering tiles (first used one panel later); the
+//       previous panel's factors are stored from the other LDS buffer;
 //   B1. wavefront 0 factors the 32 x 32 diagonal block in registers, one row
 //       per lane, pivot rows broadcast with readlane, boosting at pivot time;
 //   B2. 16(KT-2) threads solve L21 = A21 U11^-1 row by row, another 16(KT-2)
 //       threads U12 = L11^-1 A12 column by column (unit L11);
 //   C.  trailing tiles += (-L21) U12 on the matrix cores
 //       (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64, the 32 panel columns
-//       split over the four lane groups), U12 / pivot written out.
+//       split over the four lane groups).
 // Outside the band every stored value is an exact zero, so the padded tiles
-// never create fill-in.  Factors are stored as L (unit), D, U' like the other
-// factorization kernels.
+// never create fill-in.  The wavefronts exchange data only through LDS (each
+// tile load is consumed by the wavefront that issued it; factor stores are
+// not read back), so the barriers do not wait for global memory, and all
+// global accesses of a panel are issued in phase A, where they have B1 and
+// B2 to complete.  Factors are stored as L (unit), D, U' like the other
+// factorization kernels; the multipliers use the pivot's reciprocal.
 // ---------------------------------------------------------------------------
 template <class T>
 struct Mfma16;
@@ -76,12 +81,19 @@ template <class T, int KT>
 struct ResLds {
     static constexpr int PS = 36;                         // row stride: 16-byte aligned rows, few bank conflicts
     static constexpr int NTR = 16 * (KT - 2);             // trailing rows / columns
-    static constexpr int pn_off = 0;                      // Pn [16 KT][PS]: panel rows, then -L21
+    // two panel buffers, used by alternate panels (a panel's factors are
+    // stored while the next panel is being factored):
+    static constexpr int pn_off = 0;                      // Pn [16 KT][PS]: panel rows; after B2 rows >= 32 hold -L21
     static constexpr int ub_off = pn_off + 16 * KT * PS;  // UbT [NTR][PS]: U block, one column per row
-    static constexpr int lt_off = ub_off + NTR * PS;      // L11 transposed [32][33]
-    static constexpr int pv_off = lt_off + 32 * 33;       // pivots [32]
-    static constexpr size_t bytes = sizeof(T) * (size_t)(pv_off + 32);
+    static constexpr int pv_off = ub_off + NTR * PS;      // pivots [32]
+    static constexpr int rp_off = pv_off + 32;            // their reciprocals [32]
+    static constexpr int buf_elems = rp_off + 32;
+    static constexpr int lt_off = 2 * buf_elems;          // L11 transposed [32][33]
+    static constexpr size_t bytes = sizeof(T) * (size_t)(lt_off + 32 * 33);
 };
+
+__device__ inline float pivot_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ inline double pivot_rcp(double v) { return 1.0 / v; }
 
 template <class T, int KT>
 __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict__ B0, T* __restrict__ B1, int k,
@@ -90,7 +102,7 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
     using MF = Mfma16<T>;
     using acc_t = typename MF::acc_t;
     using LY = ResLds<T, KT>;
-    constexpr int PS = LY::PS, NTR = LY::NTR, NS = KT * KT, SPW = (NS + 7) / 8;
+    constexpr int PS = LY::PS, NTR = LY::NTR, NS = KT * KT, SPW = (NS + 7) / 8, HR = 32 + NTR;
     const FactorJob job = jobs[blockIdx.x];
     T* __restrict__ base = job.buf ? B1 : B0;
     const int64_t w = 2 * k + 1;
@@ -114,6 +126,30 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
             t[q] = inb(dr, dc, rem) ? P[dir * (dc * wm1 + dr)] : (T)0;
         }
     };
+    // factors of panel pp from panel buffer bq: its 32 columns (rows 0 .. HR-1:
+    // L, D, U' of the diagonal block and L21) and its 32 U' rows, coalesced
+    auto store_panel = [&](int64_t pp, const T* bq, int tid) {
+        const int64_t pj0 = 32 * pp;
+        T* __restrict__ Pq = base + origin + dir * (pj0 * w + k);
+        const int prem = (int)min(len - pj0, (int64_t)(1 << 30));
+        const int pnb = min(32, prem);
+        const T* qPn = bq + LY::pn_off;
+        const T* qUb = bq + LY::ub_off;
+        const T* qpv = bq + LY::pv_off;
+        for (int e = tid; e < 32 * HR; e += 512) {
+            const int c = e / HR, r = e - c * HR;
+            if (c < pnb && r < prem && r - c <= k && c - r <= k) {
+                T v = qPn[r * PS + c];
+                if (r >= 32) v = -v;
+                else if (r < c) v = v / qpv[r];
+                Pq[dir * (c * wm1 + r)] = v;
+            }
+        }
+        for (int e = tid; e < 32 * NTR; e += 512) {
+            const int cc = e >> 5, m = e & 31;
+            if (m < pnb && 32 + cc < prem && 32 + cc - m <= k) Pq[dir * ((32 + cc) * wm1 + m)] = qUb[cc * PS + m] / qpv[m];
+        }
+    };
 
     acc_t acc[SPW];
     {
@@ -133,20 +169,26 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         // per-lane predicates of the unrolled loops out of the registers)
         int tid = threadIdx.x, zoff = 0;
         asm volatile("" : "+v"(tid), "+v"(zoff));
+        const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+        const int cur = (int)(p & 1);
         // LDS addresses from an opaque zero fold into the ds_* immediate
         // offsets instead of living in scalar registers across the loop
-        T* Pn = sm + zoff + LY::pn_off;
-        T* UbT = sm + zoff + LY::ub_off;
+        T* bcur = sm + zoff + cur * LY::buf_elems;
+        T* Pn = bcur + LY::pn_off;
+        T* UbT = bcur + LY::ub_off;
+        T* pivs = bcur + LY::pv_off;
+        T* rpiv = bcur + LY::rp_off;
         T* L11T = sm + zoff + LY::lt_off;
-        T* pivs = sm + zoff + LY::pv_off;
-        const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
         const int64_t j0 = 32 * p;
         T* __restrict__ P = base + origin + dir * (j0 * w + k);
         const int rem = (int)min(len - j0, (int64_t)(1 << 30));
         const int nb = min(32, rem);
         const int sh = (int)((2 * p) % KT);
 
-        // A. leaving tiles to LDS; their slots load the entering tiles
+        // A. leaving tiles to LDS; their slots load the entering tiles.  Then
+        //    the previous panel's factors go out: every global access of the
+        //    panel is issued here, so loads and stores drain during B1 and B2
+        //    before the matrix-core phase needs the registers.
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
             const int s = wave + 8 * j;
@@ -166,7 +208,8 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
                 }
             }
         }
-        __syncthreads();
+        if (p > 0) store_panel(p - 1, sm + zoff + (1 - cur) * LY::buf_elems, tid);
+        lds_barrier();
 
         // B1. LU of the diagonal block, row r in lane r (lanes 32..63 mirror)
         if (wave == 0) {
@@ -180,30 +223,34 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
                 if (jj < nb) {
                     T piv = lane_bcast(row[jj], jj);
                     if (job.boost && (jj + 1 < rem || job.boost_last)) piv = boost_pivot(piv);
-                    const bool below = r > jj;
-                    const T l = row[jj] / piv;
-                    pr = r == jj ? piv : pr;
-                    row[jj] = below ? l : (r == jj ? piv : row[jj]);
+                    const T rp = pivot_rcp(piv);
+                    T u[32];
 #pragma unroll
-                    for (int c = jj + 1; c < 32; ++c) {
-                        const T u = lane_bcast(row[c], jj);   // pivot row, read before any lane changes it
-                        row[c] = below ? row[c] - l * u : row[c];
+                    for (int c = jj + 1; c < 32; ++c) u[c] = lane_bcast(row[c], jj);   // pivot row, before any update
+                    if (r == jj) {
+                        row[jj] = piv;
+                        pr = piv;
+                    } else if (r > jj) {
+                        const T l = row[jj] * rp;
+                        row[jj] = l;
+#pragma unroll
+                        for (int c = jj + 1; c < 32; ++c) row[c] -= l * u[c];
                     }
                 }
             }
             if (lane < 32) {
                 pivs[r] = pr;
+                rpiv[r] = pivot_rcp(pr);
 #pragma unroll
                 for (int c = 0; c < 32; ++c) {
                     Pn[r * PS + c] = row[c];
                     if (c < r) L11T[c * 33 + r] = row[c];
-                    if (inb(r, c, nb)) P[dir * (c * wm1 + r)] = c > r ? row[c] / pr : row[c];
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
 
-        // B2. L21 rows and U12 columns
+        // B2. L21 rows (stored negated: the trailing update adds) and U12 columns
         if (tid < NTR) {
             const int t = tid;
             T a[32];
@@ -212,18 +259,14 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
 #pragma unroll
             for (int jj = 0; jj < 32; ++jj) {
                 if (jj < nb) {
-                    const T l = a[jj] / pivs[jj];
+                    const T l = a[jj] * rpiv[jj];
                     a[jj] = l;
 #pragma unroll
                     for (int c = jj + 1; c < 32; ++c) a[c] -= l * Pn[jj * PS + c];
                 }
             }
-            const bool rowin = 32 + t < rem;
 #pragma unroll
-            for (int c = 0; c < 32; ++c) {
-                Pn[(32 + t) * PS + c] = -a[c];
-                if (rowin && c < nb && 32 + t - c <= k) P[dir * (c * wm1 + 32 + t)] = a[c];
-            }
+            for (int c = 0; c < 32; ++c) Pn[(32 + t) * PS + c] = -a[c];
         } else if (tid >= 256 && tid < 256 + NTR) {
             const int cc = tid - 256;
             T u[32];
@@ -240,9 +283,10 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
 #pragma unroll
             for (int m = 0; m < 32; ++m) UbT[cc * PS + m] = u[m];
         }
-        __syncthreads();
+        lds_barrier();
 
-        // C. trailing update on the matrix cores; U' rows out
+        // C. trailing update on the matrix cores.  The next panel writes the
+        //    other buffer, so no barrier is needed after it.
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
             const int s = wave + 8 * j;
@@ -259,12 +303,8 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
                 }
             }
         }
-        for (int e = tid; e < 32 * NTR; e += 512) {
-            const int cc = e >> 5, m = e & 31;
-            if (m < nb && 32 + cc < rem && 32 + cc - m <= k) P[dir * ((32 + cc) * wm1 + m)] = UbT[cc * PS + m] / pivs[m];
-        }
-        __syncthreads();
     }
+    if (npan > 0) store_panel(npan - 1, sm + ((npan - 1) & 1) * LY::buf_elems, threadIdx.x);
 }
 
 template <class T, int KT>
@@ -278,9 +318,9 @@ void launch_factor_resident(T* B0, T* B1, int k, const FactorJob* jobs, int njob
 }
 
 // Largest window (in 16-row tiles) kept in registers: FP32 up to k = 224,
-// FP64 (twice the registers per tile) up to k = 128.
+// FP64 (twice the registers and LDS per tile) up to k = 112.
 template <class T>
-constexpr int resident_max_kt() { return sizeof(T) == 4 ? 16 : 10; }
+constexpr int resident_max_kt() { return sizeof(T) == 4 ? 16 : 9; }
 
 template <class T, int KT>
 bool dispatch_resident(int kt, T* B0, T* B1, int k, const FactorJob* jobs, int njobs, hipStream_t st)

@@ -117,6 +117,15 @@ __device__ inline T boost_pivot(T v)
     return v < (T)0 ? -nv : nv;
 }
 
+// Workgroup barrier for data exchanged through LDS only.  __syncthreads()
+// also waits for every outstanding global load and store of the wavefront
+// (vmcnt(0)); kernels whose wavefronts share data only through LDS use this
+// instead, so prefetches and result stores stay in flight across it.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // ---------------------------------------------------------------- launchers
 // band.hip
 void launch_gen_banded(int64_t n, int k, double dom, uint64_t seed, double* band, hipStream_t st);

@@ -254,17 +254,23 @@ __device__ __forceinline__ void lower_sweep_fast(const BandView<T>& V, int64_t l
 // of BS columns needs (column c: its diagonal and the k entries below it,
 // contiguous in the band storage) are copied HBM -> LDS by four producer
 // wavefronts with global_load_lds_dwordx4, as 16-byte chunks covering each
-// column segment, up to NBUF - 2 blocks ahead of the solve; the solve then
-// runs at the pace of its dependency chain instead of paying one HBM latency
-// per block.  In step b:
+// column segment, up to NBUF - 2 blocks ahead of the solve.  The right-hand
+// side rows stream into an LDS staging ring the same way, SW_DIN steps
+// ahead.  No step waits for HBM; the sweep runs at the pace of its
+// dependency chain.  In step b:
 //   wave 0     diagonal-block chain and sub-diagonal block of block b (as in
 //              lower_sweep_fast), multipliers from buffer b;
-//   waves 1-3  far rows of block b-1 from buffer b-1, and the input stream;
+//   waves 1-3  far rows of block b-1 from buffer b-1; wave 1 also moves the
+//              staged right-hand side rows into the pending ring;
 //   waves 4-7  producers, loading block b + NBUF - 2 into the buffer of
 //              block b-2.
+// Each producer (and wave 1 for the staging ring) waits with an explicit
+// vmcnt count for its own copies before the barrier that publishes them.
 // ---------------------------------------------------------------------------
 constexpr int SW_PROD = 4;           // producer wavefronts
 constexpr int SW_THREADS = 512;
+constexpr int SW_DIN = 8;            // right-hand side steps in flight
+constexpr int SW_NIN = SW_DIN + 1;   // staging slots of 64 dwords
 
 // s_waitcnt takes an immediate: dispatch on a wave-uniform count (<= 63).
 // gfx9 encoding: vmcnt bits [3:0] and [15:14]; expcnt [6:4] and lgkmcnt
@@ -288,14 +294,6 @@ __device__ __forceinline__ void wait_vmcnt_le(int n)
 
 __device__ __forceinline__ void wait_vmcnt(int n) { wait_vmcnt_le<63>(n); }
 
-// Workgroup barrier for data exchanged through LDS only: unlike
-// __syncthreads() it does not wait for this wavefront's outstanding global
-// loads and stores, so the producers' LDS-DMA prefetches stay in flight.
-__device__ __forceinline__ void lds_barrier()
-{
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
-
 // Chunk geometry of one view for the LDS-prefetched sweep.
 template <class T>
 struct SweepChunks {
@@ -315,82 +313,111 @@ struct SweepChunks {
     __host__ __device__ size_t buf_elems() const { return (size_t)ncb * E; }
 };
 
-template <class T, class In, class Out>
-__device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t len, In in, Out out, bool scale,
-                                                T* pend, int ring_mask, T* xblk2, T* acc2, T* bufs, int nbuf)
+// Right-hand side of a sweep: view row r reads src[s0 + sdir * r] (RawT is
+// double for the input vector, T for the intermediate), and fin(r, raw)
+// turns it into the preconditioner type.
+template <class T, class RawT, class Fin, class Out>
+__device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t len, const RawT* __restrict__ src,
+                                                int64_t s0, int sdir, Fin fin, Out out, bool scale, T* pend,
+                                                int ring_mask, T* xblk2, T* acc2, T* dyn, int nbuf)
 {
     constexpr int BS = sweep_bs<T>();
     constexpr int E = SweepChunks<T>::E;
+    constexpr int WPR = sizeof(RawT) / 4;     // dwords per right-hand side row
     const SweepChunks<T> G(V.k, BS);
     const int k = V.k;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int t = tid - 64;                   // far-row index of waves 1-3
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nfar = k - BS;                  // far rows per block
     const int64_t nblk = (len + BS - 1) / BS;
     const size_t bufe = G.buf_elems();
+    unsigned* instage = reinterpret_cast<unsigned*>(dyn);
+    T* bufs = dyn + SW_NIN * 64 * 4 / sizeof(T);
 
     const int64_t H = BS + max(BS, k);
     const int64_t init = min(len, H);
-    for (int64_t r = tid; r < init; r += blockDim.x) pend[r & ring_mask] = in(r);
-    if (tid < BS) { acc2[tid] = (T)0; acc2[BS + tid] = (T)0; }
+    for (int64_t r = threadIdx.x; r < init; r += blockDim.x) pend[r & ring_mask] = fin(r, src[s0 + sdir * r]);
+    if (threadIdx.x < BS) { acc2[threadIdx.x] = (T)0; acc2[BS + threadIdx.x] = (T)0; }
 
     // Segment of view column c: elements q = 0..k at V.origin + dir * (c*w + k + q);
     // lo = its lowest address, lo & ~(E-1) the first chunk.  In LDS, element q of
-    // block column j is at j*segl + off + dir*q, off = (lo & (E-1)) + (dir < 0 ? k : 0).
+    // block column j is at j*segl + off_j + dir*q with off_j = (lo_j & (E-1)) +
+    // (dir < 0 ? k : 0); lo_j = lo_0 + j * dir * w.
     auto seg_lo = [&](int64_t c) -> int64_t {
         const int64_t a = V.origin + (int64_t)V.dir * (c * V.w + k);
         return V.dir > 0 ? a : a - k;
     };
-    auto seg_off = [&](int64_t c) -> int { return (int)(seg_lo(c) & (E - 1)) + (V.dir < 0 ? k : 0); };
+    const int dwm = (int)(V.dir * V.w);
+    const int kadj = V.dir < 0 ? k : 0;
 
     // producer: chunks of block b into buffer b % nbuf (columns past the end
-    // and padding lanes re-read column 0's first chunk: in bounds, never used)
-    const int pw = wave - 4;
-    auto produce = [&](int64_t b) {
+    // and padding lanes re-read the block's first chunk: in bounds, never used)
+    auto produce = [&](int64_t b, int lane) {
         T* dst = bufs + (size_t)(b % nbuf) * bufe;
         const int64_t c0 = b * BS;
         const int64_t lo_first = seg_lo(c0) & ~(int64_t)(E - 1);
         for (int i = 0; i < G.ni; ++i) {
-            const int rnd = i * SW_PROD + pw;               // 64-chunk round of this block
+            const int rnd = i * SW_PROD + (wave - 4);       // 64-chunk round of this block
             const int g = rnd * 64 + lane;
             const int j = g / G.nch, ch = g - j * G.nch;
-            int64_t src = lo_first;
-            if (g < BS * G.nch && c0 + j < len) src = (seg_lo(c0 + j) & ~(int64_t)(E - 1)) + (int64_t)ch * E;
-            __builtin_amdgcn_global_load_lds(V.base + src, dst + (size_t)rnd * 64 * E, 16, 0, 0);
+            int64_t off = lo_first;
+            if (g < BS * G.nch && c0 + j < len) off = (seg_lo(c0 + j) & ~(int64_t)(E - 1)) + (int64_t)ch * E;
+            __builtin_amdgcn_global_load_lds(V.base + off, dst + (size_t)rnd * 64 * E, 16, 0, 0);
         }
     };
-    auto produced_after = [&](int64_t b) -> int {   // this wave's loads issued for blocks > b
+    auto produced_after = [&](int64_t b) -> int {   // this wave's block loads issued for blocks > b
         const int64_t last = min(nblk - 1, b + nbuf - 3);
-        const int64_t cnt = max((int64_t)0, last - b) * G.ni;
-        return (int)min(cnt, (int64_t)63);
+        return (int)min(max((int64_t)0, last - b) * G.ni, (int64_t)63);
+    };
+    // right-hand side stream (wave 1): step s carries rows [s*BS + H, s*BS + H + BS)
+    const int64_t nstream = len > H ? (len - H + BS - 1) / BS : 0;
+    auto stream = [&](int64_t st, int lane) {
+        const int tr = (lane / WPR) % BS, word = lane % WPR;
+        int64_t r = st * BS + H + tr;
+        if (r >= len) r = len - 1;
+        const unsigned* sp = reinterpret_cast<const unsigned*>(src + (s0 + sdir * r)) + word;
+        __builtin_amdgcn_global_load_lds(sp, instage + (st % SW_NIN) * 64, 4, 0, 0);
     };
 
     if (wave >= 4) {
-        for (int64_t b = 0; b < min(nblk, (int64_t)nbuf - 2); ++b) produce(b);
+        for (int64_t b = 0; b < min(nblk, (int64_t)nbuf - 2); ++b) produce(b, threadIdx.x & 63);
         wait_vmcnt(produced_after(0));
+    } else if (wave == 1) {
+        for (int64_t st = 0; st < min(nstream, (int64_t)SW_DIN); ++st) stream(st, threadIdx.x & 63);
     }
     lds_barrier();
 
-    auto step = [&](int64_t b) {
+    for (int64_t b = 0; b <= nblk; ++b) {
+        // lane-derived values re-materialised per step (no hoisted predicates)
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int lane = tid & 63;
         const int64_t c0 = b * BS;
         const int nb = (int)min((int64_t)BS, len - c0);
         const int par = (int)(b & 1);
         if (wave == 0) {
             if (c0 < len) {
+                // element (row c0 + i, column c0 + j) of the block sits at
+                // colb(j) + dir * i with colb(j) = j * segl + off_j - dir * j
+                // (wave-uniform); rows outside the band read a neighbouring
+                // LDS word (inside the buffers or their guard) and are masked
                 const T* buf = bufs + (size_t)(b % nbuf) * bufe;
+                const int lo0 = (int)seg_lo(c0);
                 const int i = lane;
+                const int di = V.dir * i;
                 const bool rv = i < nb;
                 const bool sv = i < BS && c0 + BS + i < len;
                 T a[BS], sb[BS];
 #pragma unroll
                 for (int j = 0; j < BS; ++j) {
-                    const int off = j * G.segl + seg_off(c0 + j);
-                    a[j] = (rv && j < i && i - j <= k) ? buf[off + V.dir * (i - j)] : (T)0;
-                    sb[j] = (sv && j < nb && BS + i - j <= k) ? buf[off + V.dir * (BS + i - j)] : (T)0;
+                    const int colb = j * G.segl + ((lo0 + j * dwm) & (E - 1)) + kadj - V.dir * j;
+                    const T va = buf[colb + di];
+                    const T vb = buf[colb + V.dir * BS + di];
+                    a[j] = (rv && j < i && i - j <= k) ? va : (T)0;
+                    sb[j] = (sv && j < nb && BS + i - j <= k) ? vb : (T)0;
                 }
-                const T d = (rv && scale) ? buf[(i < BS ? i : 0) * G.segl + seg_off(c0 + (i < BS ? i : 0))] : (T)1;
+                const int ii = i < BS ? i : 0;
+                const T dv = buf[ii * G.segl + ((lo0 + ii * dwm) & (E - 1)) + kadj];
+                const T d = (rv && scale) ? dv : (T)1;
                 T x = rv ? pend[(c0 + lane) & ring_mask] - acc2[par * BS + lane] : (T)0;
 #pragma unroll
                 for (int j = 0; j < BS - 1; ++j) {
@@ -401,49 +428,58 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
                     xblk2[par * BS + lane] = x;
                     out(c0 + lane, scale ? x / d : x);
                 }
-                T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+                T s0_ = 0, s1_ = 0, s2_ = 0, s3_ = 0;
 #pragma unroll
                 for (int j = 0; j < BS; j += 4) {
-                    s0 += sb[j] * bcast(x, j);
-                    s1 += sb[j + 1] * bcast(x, j + 1);
-                    s2 += sb[j + 2] * bcast(x, j + 2);
-                    s3 += sb[j + 3] * bcast(x, j + 3);
+                    s0_ += sb[j] * bcast(x, j);
+                    s1_ += sb[j + 1] * bcast(x, j + 1);
+                    s2_ += sb[j + 2] * bcast(x, j + 2);
+                    s3_ += sb[j + 3] * bcast(x, j + 3);
                 }
-                if (sv) pend[(c0 + BS + lane) & ring_mask] -= (s0 + s1) + (s2 + s3);
+                if (sv) pend[(c0 + BS + lane) & ring_mask] -= (s0_ + s1_) + (s2_ + s3_);
             }
         } else if (wave < 4) {
+            const int t = tid - 64;   // far-row index
             if (b >= 1) {
                 const int64_t cp = c0 - BS;
                 const T* buf = bufs + (size_t)((b - 1) % nbuf) * bufe;
+                const int lo0 = (int)seg_lo(cp);
                 const T* xp = xblk2 + (1 - par) * BS;
                 const int64_t r = cp + 2 * BS + t;
-                const bool fv = t < nfar && r < len;
-                T s0 = 0, s1 = 0;
-                if (fv) {   // row r < len: block b-1 is a full block
+                const bool fv = t < nfar && r < len;   // r < len: block b-1 is a full block
+                const int dt = V.dir * (2 * BS + t);
+                T s0_ = 0, s1_ = 0;
 #pragma unroll
-                    for (int j = 0; j < BS; j += 2) {
-                        const int q0 = 2 * BS + t - j, q1 = q0 - 1;
-                        if (q0 <= k) s0 += buf[j * G.segl + seg_off(cp + j) + V.dir * q0] * xp[j];
-                        if (q1 <= k) s1 += buf[(j + 1) * G.segl + seg_off(cp + j + 1) + V.dir * q1] * xp[j + 1];
-                    }
+                for (int j = 0; j < BS; j += 2) {
+                    const int colb0 = j * G.segl + ((lo0 + j * dwm) & (E - 1)) + kadj - V.dir * j;
+                    const int colb1 = (j + 1) * G.segl + ((lo0 + (j + 1) * dwm) & (E - 1)) + kadj - V.dir * (j + 1);
+                    const int q0 = 2 * BS + t - j;
+                    const T v0 = buf[colb0 + dt];
+                    const T v1 = buf[colb1 + dt];
+                    s0_ += (q0 <= k ? v0 : (T)0) * xp[j];
+                    s1_ += (q0 - 1 <= k ? v1 : (T)0) * xp[j + 1];
                 }
-                const T acc = s0 + s1;
+                const T acc = s0_ + s1_;
                 if (t < BS) acc2[(1 - par) * BS + t] = fv ? acc : (T)0;
                 else if (fv) pend[r & ring_mask] -= acc;
             } else if (t < BS) {
                 acc2[BS + t] = (T)0;
             }
-            const int64_t rr = c0 + H + t;
-            if (t < BS && rr < len) pend[rr & ring_mask] = in(rr);
+            if (wave == 1 && b < nstream) {
+                if (b + SW_DIN < nstream) stream(b + SW_DIN, lane);
+                wait_vmcnt((int)(min(b + SW_DIN, nstream - 1) - b));
+                const int64_t rr = c0 + H + t;
+                if (t < BS && rr < len) {
+                    const RawT raw = reinterpret_cast<const RawT*>(instage + (b % SW_NIN) * 64)[t];
+                    pend[rr & ring_mask] = fin(rr, raw);
+                }
+            }
         } else {
-            if (b + nbuf - 2 < nblk) produce(b + nbuf - 2);
+            if (b + nbuf - 2 < nblk) produce(b + nbuf - 2, lane);
             wait_vmcnt(produced_after(b + 1));
         }
         lds_barrier();
-    };
-
-    // waves 1-3 lag one block behind wave 0: one extra step drains them
-    for (int64_t b = 0; b <= nblk; ++b) step(b);
+    }
 }
 
 inline int ring_size(int k)
@@ -496,11 +532,14 @@ __global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_interface_
     auto out2 = [&](int64_t r, T y) { g[grow(-d1, r)] = y; };
     const int64_t kl = min((int64_t)k, len);
     if constexpr (MODE == 2) {
-        T* bufs = pend + ring_mask + 1;
-        lower_sweep_lds<T>(V1, len, in1, [&](int64_t r, T y) { scr[grow(d1, r)] = y; }, true, pend, ring_mask,
-                           xblk2, acc2, bufs, nbuf);
+        T* dyn = pend + ring_mask + 1;
+        auto cvt = [](int64_t, double raw) -> T { return (T)raw; };
+        auto same = [](int64_t, T raw) -> T { return raw; };
+        lower_sweep_lds<T>(V1, len, v, grow(d1, 0), d1, cvt, [&](int64_t r, T y) { scr[grow(d1, r)] = y; }, true,
+                           pend, ring_mask, xblk2, acc2, dyn, nbuf);
         __syncthreads();
-        lower_sweep_lds<T>(V2, kl, in2, out2, false, pend, ring_mask, xblk2, acc2, bufs, nbuf);
+        lower_sweep_lds<T>(V2, kl, (const T*)scr, grow(-d1, 0), -d1, same, out2, false, pend, ring_mask, xblk2, acc2,
+                           dyn, nbuf);
     } else if constexpr (MODE == 1) {
         lower_sweep_fast<T>(V1, len, in1, [&](int64_t r, T y) { scr[grow(d1, r)] = y; }, true, pend, ring_mask,
                             xblk2, acc2);
@@ -551,16 +590,26 @@ __global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_full_kerne
     };
     auto in2 = [&](int64_t r) -> T { return scr[grow(-d1, r)]; };
     if constexpr (MODE == 2) {
-        T* bufs = pend + ring_mask + 1;
-        lower_sweep_lds<T>(V1, len, rhs, [&](int64_t r, T y) { scr[grow(d1, r)] = y; }, true, pend, ring_mask, xblk2,
-                           acc2, bufs, nbuf);
+        T* dyn = pend + ring_mask + 1;
+        auto rhs_fin = [&](int64_t r, double raw) -> T {
+            T val = (T)raw;
+            const int64_t lr = grow(d1, r) - s;
+            if (has_top && lr < k) val -= corr[(size_t)(p - 1) * 2 * k + k + lr];
+            if (has_bot && lr >= len - k) val -= corr[(size_t)p * 2 * k + (lr - (len - k))];
+            return val;
+        };
+        auto same = [](int64_t, T raw) -> T { return raw; };
+        lower_sweep_lds<T>(V1, len, v, grow(d1, 0), d1, rhs_fin, [&](int64_t r, T y) { scr[grow(d1, r)] = y; }, true,
+                           pend, ring_mask, xblk2, acc2, dyn, nbuf);
         __syncthreads();
         if (outd)
-            lower_sweep_lds<T>(V2, len, in2, [&](int64_t r, T y) { outd[grow(-d1, r)] = (double)y; }, false, pend,
-                               ring_mask, xblk2, acc2, bufs, nbuf);
+            lower_sweep_lds<T>(V2, len, (const T*)scr, grow(-d1, 0), -d1, same,
+                               [&](int64_t r, T y) { outd[grow(-d1, r)] = (double)y; }, false, pend, ring_mask, xblk2,
+                               acc2, dyn, nbuf);
         else
-            lower_sweep_lds<T>(V2, len, in2, [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, false, pend, ring_mask,
-                               xblk2, acc2, bufs, nbuf);
+            lower_sweep_lds<T>(V2, len, (const T*)scr, grow(-d1, 0), -d1, same,
+                               [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, false, pend, ring_mask, xblk2, acc2,
+                               dyn, nbuf);
     } else if constexpr (MODE == 1) {
         lower_sweep_fast<T>(V1, len, rhs, [&](int64_t r, T y) { scr[grow(d1, r)] = y; }, true, pend, ring_mask, xblk2,
                             acc2);
@@ -667,10 +716,12 @@ SweepCfg sweep_cfg(int k)
     if (!sweep_fast<T>(k)) return {0, 0, ring, 256};
     const SweepChunks<T> G(k, sweep_bs<T>());
     const size_t bb = G.buf_elems() * sizeof(T);
-    int nbuf = (int)std::min<size_t>(8, (LDS_MAX - ring) / bb);
+    const size_t stage = SW_NIN * 64 * 4;
+    const size_t guard = 256 * sizeof(T);   // masked reads past the last buffer stay inside the allocation
+    int nbuf = (int)std::min<size_t>(8, (LDS_MAX - ring - stage - guard) / bb);
     nbuf = std::min(nbuf, 63 / G.ni + 3);
     if (nbuf < 4) return {1, 0, ring, 256};
-    return {2, nbuf, ring + nbuf * bb, SW_THREADS};
+    return {2, nbuf, ring + stage + nbuf * bb + guard, SW_THREADS};
 }
 
 template <class T, int MODE>
