@@ -68,9 +68,12 @@ __global__ void gen_vector_kernel(int64_t n, uint64_t seed, double* __restrict__
 // ---------------------------------------------------------------------------
 // y = A x on the row-major FP64 band (the MVBanded functor, sap/spmv.h:27-74).
 // G lanes cooperate on one row: lane l sums diagonals l, l+G, ... in order,
-// then a fixed shuffle tree combines the G partial sums.
+// then a fixed shuffle tree combines the G partial sums.  Each lane issues
+// the loads of NI of its diagonals before the multiply-adds (NI * G entries
+// of the row per round); the band is read with non-temporal loads since a
+// product streams it exactly once, leaving the caches to x.
 // ---------------------------------------------------------------------------
-template <int G>
+template <int G, int NI>
 __global__ __launch_bounds__(256) void band_spmv_kernel(int64_t n, int k, const double* __restrict__ band,
                                                         const double* __restrict__ x, double* __restrict__ y)
 {
@@ -84,7 +87,18 @@ __global__ __launch_bounds__(256) void band_spmv_kernel(int64_t n, int k, const 
         const int64_t dhi64 = n - 1 - row + k;
         const int dhi = (int)(dhi64 < w - 1 ? dhi64 : w - 1);
         const double* xr = x + row - k;
-        for (int d = dlo + lane; d <= dhi; d += G) s = fma(r[d], xr[d], s);
+        for (int d0 = dlo + lane; d0 <= dhi; d0 += NI * G) {
+            double av[NI], xv[NI];
+#pragma unroll
+            for (int m = 0; m < NI; ++m) {
+                const int d = d0 + m * G;
+                av[m] = d <= dhi ? __builtin_nontemporal_load(r + d) : 0.0;
+                xv[m] = d <= dhi ? xr[d] : 0.0;
+            }
+#pragma unroll
+            for (int m = 0; m < NI; ++m)
+                if (d0 + m * G <= dhi) s = fma(av[m], xv[m], s);
+        }
     }
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) s += __shfl_down(s, off, G);
@@ -158,13 +172,13 @@ void launch_band_spmv(int64_t n, int k, const double* band, const double* x, dou
     const int bs = 256;
     if (w >= 192) {
         const int64_t blocks = (n * 64 + bs - 1) / bs;
-        hipLaunchKernelGGL(band_spmv_kernel<64>, dim3((unsigned)blocks), dim3(bs), 0, st, n, k, band, x, y);
+        hipLaunchKernelGGL((band_spmv_kernel<64, 8>), dim3((unsigned)blocks), dim3(bs), 0, st, n, k, band, x, y);
     } else if (w >= 48) {
         const int64_t blocks = (n * 16 + bs - 1) / bs;
-        hipLaunchKernelGGL(band_spmv_kernel<16>, dim3((unsigned)blocks), dim3(bs), 0, st, n, k, band, x, y);
+        hipLaunchKernelGGL((band_spmv_kernel<16, 12>), dim3((unsigned)blocks), dim3(bs), 0, st, n, k, band, x, y);
     } else {
         const int64_t blocks = (n * 4 + bs - 1) / bs;
-        hipLaunchKernelGGL(band_spmv_kernel<4>, dim3((unsigned)blocks), dim3(bs), 0, st, n, k, band, x, y);
+        hipLaunchKernelGGL((band_spmv_kernel<4, 12>), dim3((unsigned)blocks), dim3(bs), 0, st, n, k, band, x, y);
     }
     SAP_LAUNCH_CHECK();
 }

@@ -6,6 +6,8 @@
 // factorization is the LU factorization of the same storage read backwards.
 #include "sap_internal.hpp"
 
+#include <climits>
+
 namespace sap_amd {
 
 namespace {
@@ -92,6 +94,36 @@ struct ResLds {
     static constexpr size_t bytes = sizeof(T) * (size_t)(lt_off + 32 * 33);
 };
 
+template <class T>
+__device__ inline T buffer_load(__amdgpu_buffer_rsrc_t r, int off);
+template <>
+__device__ inline float buffer_load<float>(__amdgpu_buffer_rsrc_t r, int off)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <>
+__device__ inline double buffer_load<double>(__amdgpu_buffer_rsrc_t r, int off)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    return __longlong_as_double((long long)(((unsigned long long)v[1] << 32) | v[0]));
+}
+
+template <class T>
+__device__ inline void buffer_store(T v, __amdgpu_buffer_rsrc_t r, int off);
+template <>
+__device__ inline void buffer_store<float>(float v, __amdgpu_buffer_rsrc_t r, int off)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 0);
+}
+template <>
+__device__ inline void buffer_store<double>(double v, __amdgpu_buffer_rsrc_t r, int off)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    const u2 x = {(unsigned)b, (unsigned)(b >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b64(x, r, off, 0, 0);
+}
+
 __device__ inline float pivot_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
 __device__ inline double pivot_rcp(double v) { return 1.0 / v; }
 
@@ -118,48 +150,76 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
     // Entry (j0 + dr, j0 + dc) of the partition is P[dir * (dc * 2k + dr)]
     // with P = base + origin + dir * (j0 * (2k + 1) + k); rem = len - j0.
     auto inb = [&](int dr, int dc, int rem) { return dr < rem && dc < rem && dr - dc <= k && dc - dr <= k; };
-    auto load_tile = [&](acc_t& t, const T* P, int rem, int dr0, int dc0, int lr, int lane) {
+    // Tile loads are buffer loads over the partition's storage: every element
+    // is one unconditional load, and entries outside the band or the
+    // partition use an offset past the end of the buffer, which reads zero.
+    // (A register only ever written by loads needs no wait for an earlier
+    // load into it.)  Element (j0 + dr, j0 + dc) is at element offset
+    // ob + dir * (dc * 2k + dr) with ob = rel0 + dir * (j0 * (2k + 1) + k).
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(base + job.start * w), 0, (int)(len * w * (int64_t)sizeof(T)), 0x00020000);
+    const int rel0 = dir > 0 ? 0 : (int)(len * w - 1);
+    auto load_tile = [&](acc_t& t, int ob, int rem, int dr0, int dc0, int lr, int lane) {
         const int dc = dc0 + lr;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int dr = dr0 + MF::row(lane, q);
-            t[q] = inb(dr, dc, rem) ? P[dir * (dc * wm1 + dr)] : (T)0;
+            const int off = inb(dr, dc, rem) ? (ob + dir * (dc * wm1 + dr)) * (int)sizeof(T) : -1;
+            t[q] = buffer_load<T>(rsrc, off);
         }
     };
     // factors of panel pp from panel buffer bq: its 32 columns (rows 0 .. HR-1:
     // L, D, U' of the diagonal block and L21) and its 32 U' rows, coalesced
+    // (values are gathered from LDS first and stored afterwards, so the LDS
+    // reads overlap instead of each store waiting for its own read).  The
+    // stores are unconditional buffer stores: entries outside the band get
+    // an offset past the end of the buffer and are dropped, so every panel
+    // issues the same number of stores (pp < 0: none land).
     auto store_panel = [&](int64_t pp, const T* bq, int tid) {
-        const int64_t pj0 = 32 * pp;
-        T* __restrict__ Pq = base + origin + dir * (pj0 * w + k);
-        const int prem = (int)min(len - pj0, (int64_t)(1 << 30));
+        constexpr int N1 = (32 * HR + 511) / 512, N2 = (32 * NTR + 511) / 512;
+        const int64_t pj0 = 32 * max(pp, (int64_t)0);
+        const int obq = rel0 + dir * (int)(pj0 * w + k);
+        const int prem = pp < 0 ? 0 : (int)min(len - pj0, (int64_t)(1 << 30));
         const int pnb = min(32, prem);
         const T* qPn = bq + LY::pn_off;
         const T* qUb = bq + LY::ub_off;
         const T* qpv = bq + LY::pv_off;
-        for (int e = tid; e < 32 * HR; e += 512) {
+        T v1[N1], v2[N2];
+        int o1[N1], o2[N2];
+#pragma unroll
+        for (int i = 0; i < N1; ++i) {
+            const int e = tid + 512 * i;
             const int c = e / HR, r = e - c * HR;
-            if (c < pnb && r < prem && r - c <= k && c - r <= k) {
-                T v = qPn[r * PS + c];
-                if (r >= 32) v = -v;
-                else if (r < c) v = v / qpv[r];
-                Pq[dir * (c * wm1 + r)] = v;
-            }
+            const bool ok = e < 32 * HR && c < pnb && r < prem && r - c <= k && c - r <= k;
+            T v = qPn[(ok ? r * PS + c : 0)];
+            const T pv = qpv[(r < 32 ? r : 0)];
+            v = r >= 32 ? -v : (r < c ? v / pv : v);
+            v1[i] = v;
+            o1[i] = ok ? (obq + dir * (c * wm1 + r)) * (int)sizeof(T) : -1;
         }
-        for (int e = tid; e < 32 * NTR; e += 512) {
+#pragma unroll
+        for (int i = 0; i < N2; ++i) {
+            const int e = tid + 512 * i;
             const int cc = e >> 5, m = e & 31;
-            if (m < pnb && 32 + cc < prem && 32 + cc - m <= k) Pq[dir * ((32 + cc) * wm1 + m)] = qUb[cc * PS + m] / qpv[m];
+            const bool ok = e < 32 * NTR && m < pnb && 32 + cc < prem && 32 + cc - m <= k;
+            v2[i] = qUb[(ok ? cc * PS + m : 0)] / qpv[m];
+            o2[i] = ok ? (obq + dir * ((32 + cc) * wm1 + m)) * (int)sizeof(T) : -1;
         }
+#pragma unroll
+        for (int i = 0; i < N1; ++i) buffer_store<T>(v1[i], rsrc, o1[i]);
+#pragma unroll
+        for (int i = 0; i < N2; ++i) buffer_store<T>(v2[i], rsrc, o2[i]);
     };
 
     acc_t acc[SPW];
     {
         const int lane = threadIdx.x & 63;
-        const T* P = base + origin + dir * (int64_t)k;
+        const int ob = rel0 + dir * k;
         const int rem = (int)min(len, (int64_t)(1 << 30));
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
             const int s = wave + 8 * j;
-            if (s < NS) load_tile(acc[j], P, rem, 16 * (s / KT), 16 * (s % KT), lane & 15, lane);
+            if (s < NS) load_tile(acc[j], ob, rem, 16 * (s / KT), 16 * (s % KT), lane & 15, lane);
         }
     }
 
@@ -184,11 +244,14 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         const int rem = (int)min(len - j0, (int64_t)(1 << 30));
         const int nb = min(32, rem);
         const int sh = (int)((2 * p) % KT);
+        const int obp = rel0 + dir * (int)(j0 * w + k);
 
-        // A. leaving tiles to LDS; their slots load the entering tiles.  Then
-        //    the previous panel's factors go out: every global access of the
-        //    panel is issued here, so loads and stores drain during B1 and B2
-        //    before the matrix-core phase needs the registers.
+        // A. leaving tiles to LDS, then their slots load the entering tiles,
+        //    then the previous panel's factors go out: every global access of
+        //    the panel is issued here, so loads and stores drain during B1 and
+        //    B2 before the matrix-core phase needs the registers.  (All copies
+        //    to LDS come first: reading a slot that may still be loading
+        //    waits for every load issued before it.)
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
             const int s = wave + 8 * j;
@@ -201,17 +264,28 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
 #pragma unroll
                     for (int q = 0; q < 4; ++q) UbT[(16 * (gam - 2) + lr) * PS + 16 * rho + MF::row(lane, q)] = acc[j][q];
                 }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < SPW; ++j) {
+            const int s = wave + 8 * j;
+            if (s < NS) {
+                const int rho = (s / KT - sh + KT) % KT, gam = (s % KT - sh + KT) % KT;
                 if (gam < 2 || rho < 2) {
                     const int rn = rho < 2 ? rho + KT - 2 : rho - 2;
                     const int cn = gam < 2 ? gam + KT - 2 : gam - 2;
-                    load_tile(acc[j], P, rem, 32 + 16 * rn, 32 + 16 * cn, lr, lane);
+                    load_tile(acc[j], obp, rem, 32 + 16 * rn, 32 + 16 * cn, lr, lane);
                 }
             }
         }
-        if (p > 0) store_panel(p - 1, sm + zoff + (1 - cur) * LY::buf_elems, tid);
+        store_panel(p - 1, sm + zoff + (1 - cur) * LY::buf_elems, tid);
         lds_barrier();
 
-        // B1. LU of the diagonal block, row r in lane r (lanes 32..63 mirror)
+        // B1. LU of the diagonal block, row r in lane r (lanes 32..63 mirror).
+        //     Straight-line code (no branch per column), so the scheduler can
+        //     run ahead: a lane that must not change gets multiplier 0, and
+        //     columns past the partition's end (last panel) get reciprocal 0,
+        //     which makes their steps exact no-ops.
         if (wave == 0) {
             const int r = lane & 31;
             T row[32];
@@ -220,27 +294,18 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
             for (int c = 0; c < 32; ++c) row[c] = Pn[r * PS + c];
 #pragma unroll
             for (int jj = 0; jj < 32; ++jj) {
-                if (jj < nb) {
-                    T piv = lane_bcast(row[jj], jj);
-                    if (job.boost && (jj + 1 < rem || job.boost_last)) piv = boost_pivot(piv);
-                    const T rp = pivot_rcp(piv);
-                    T u[32];
+                T piv = lane_bcast(row[jj], jj);
+                if (job.boost && (jj + 1 < rem || job.boost_last)) piv = boost_pivot(piv);
+                const T rp = jj < nb ? pivot_rcp(piv) : (T)0;
+                const T l = r > jj ? row[jj] * rp : (T)0;
+                pr = r == jj ? piv : pr;
 #pragma unroll
-                    for (int c = jj + 1; c < 32; ++c) u[c] = lane_bcast(row[c], jj);   // pivot row, before any update
-                    if (r == jj) {
-                        row[jj] = piv;
-                        pr = piv;
-                    } else if (r > jj) {
-                        const T l = row[jj] * rp;
-                        row[jj] = l;
-#pragma unroll
-                        for (int c = jj + 1; c < 32; ++c) row[c] -= l * u[c];
-                    }
-                }
+                for (int c = jj + 1; c < 32; ++c) row[c] -= l * lane_bcast(row[c], jj);   // pivot row, before any update
+                row[jj] = r > jj ? l : (r == jj ? piv : row[jj]);
             }
             if (lane < 32) {
                 pivs[r] = pr;
-                rpiv[r] = pivot_rcp(pr);
+                rpiv[r] = r < nb ? pivot_rcp(pr) : (T)0;
 #pragma unroll
                 for (int c = 0; c < 32; ++c) {
                     Pn[r * PS + c] = row[c];
@@ -257,13 +322,11 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
 #pragma unroll
             for (int c = 0; c < 32; ++c) a[c] = Pn[(32 + t) * PS + c];
 #pragma unroll
-            for (int jj = 0; jj < 32; ++jj) {
-                if (jj < nb) {
-                    const T l = a[jj] * rpiv[jj];
-                    a[jj] = l;
+            for (int jj = 0; jj < 32; ++jj) {   // rpiv = 0 past the partition's end
+                const T l = a[jj] * rpiv[jj];
+                a[jj] = l;
 #pragma unroll
-                    for (int c = jj + 1; c < 32; ++c) a[c] -= l * Pn[jj * PS + c];
-                }
+                for (int c = jj + 1; c < 32; ++c) a[c] -= l * Pn[jj * PS + c];
             }
 #pragma unroll
             for (int c = 0; c < 32; ++c) Pn[(32 + t) * PS + c] = -a[c];
@@ -273,12 +336,10 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
 #pragma unroll
             for (int m = 0; m < 32; ++m) u[m] = UbT[cc * PS + m];
 #pragma unroll
-            for (int jj = 0; jj < 31; ++jj) {
-                if (jj < nb) {
-                    const T uj = u[jj];
+            for (int jj = 0; jj < 31; ++jj) {   // u[jj] = 0 past the partition's end
+                const T uj = u[jj];
 #pragma unroll
-                    for (int m = jj + 1; m < 32; ++m) u[m] -= L11T[jj * 33 + m] * uj;
-                }
+                for (int m = jj + 1; m < 32; ++m) u[m] -= L11T[jj * 33 + m] * uj;
             }
 #pragma unroll
             for (int m = 0; m < 32; ++m) UbT[cc * PS + m] = u[m];
@@ -286,7 +347,7 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         lds_barrier();
 
         // C. trailing update on the matrix cores.  The next panel writes the
-        //    other buffer, so no barrier is needed after it.
+        //    other buffer, so no barrier is needed after this phase.
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
             const int s = wave + 8 * j;
@@ -336,12 +397,15 @@ bool dispatch_resident(int kt, T* B0, T* B1, int k, const FactorJob* jobs, int n
 }  // namespace
 
 template <class T>
-bool launch_band_factor_resident(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, hipStream_t st)
+bool launch_band_factor_resident(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, int64_t max_len,
+                                 hipStream_t st)
 {
+    // buffer offsets are 32-bit: a partition's storage must stay below 2 GiB
+    if (max_len * (2 * (int64_t)k + 1) * (int64_t)sizeof(T) >= ((int64_t)1 << 31)) return false;
     return dispatch_resident<T, 3>((k + 32 + 15) / 16, B0, B1, k, d_jobs, njobs, st);
 }
 
-template bool launch_band_factor_resident<float>(float*, float*, int, const FactorJob*, int, hipStream_t);
-template bool launch_band_factor_resident<double>(double*, double*, int, const FactorJob*, int, hipStream_t);
+template bool launch_band_factor_resident<float>(float*, float*, int, const FactorJob*, int, int64_t, hipStream_t);
+template bool launch_band_factor_resident<double>(double*, double*, int, const FactorJob*, int, int64_t, hipStream_t);
 
 }  // namespace sap_amd

@@ -23,7 +23,9 @@ void Precond<T>::factor(const std::vector<FactorJob>& jobs)
     SAP_HIP_CHECK(hipMemcpyAsync(jobs_.get(), jobs.data(), jobs.size() * sizeof(FactorJob), hipMemcpyHostToDevice,
                                  stream));
     ProfScope ps(prof, "band_factor");
-    launch_band_factor<T>(B_.get(), Bul_.get(), k, jobs_.get(), (int)jobs.size(), stream);
+    int64_t max_len = 0;
+    for (const FactorJob& j : jobs) max_len = std::max(max_len, (int64_t)j.len);
+    launch_band_factor<T>(B_.get(), Bul_.get(), k, jobs_.get(), (int)jobs.size(), max_len, stream);
 }
 
 template <class T>
@@ -72,6 +74,11 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
     // 8 elements of slack: the sweeps copy 16-byte chunks covering each
     // column segment, which may reach a few elements past the last column
     B_.resize((size_t)w * n + 8);
+    // LU_UL factors the UL of partitions 1..P-1 on a second copy of the band
+    // (kept across setups of the same size)
+    const bool need_ul = lu_ul() && cfg.precondType == SAP_Spike && P > 1 && k > 0;
+    if (need_ul) Bul_.resize((size_t)w * n + 8);
+    else Bul_.release();
     {
         ProfScope ps(prof, "band_to_colmajor");
         launch_band_to_colmajor<T>(n, k, d_band, B_.get(), stream);
@@ -79,7 +86,6 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
     SAP_HIP_CHECK(hipEventRecord(e1, stream));
     time_toBanded = elapsed_ms(e0, e1);
 
-    if (!lu_ul()) Bul_.release();   // kept across setups otherwise (same size)
     offB_.release(); offC_.release(); WV_.release(); S_.release(); Sinv_.release(); corr_.release();
     g_.resize(n);
     scr0_.resize(n);
@@ -125,7 +131,6 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
     int64_t s_last, l_last;
     pi.bounds(P - 1, s_last, l_last);
     if (lu_ul()) {
-        Bul_.resize((size_t)w * n + 8);
         SAP_HIP_CHECK(hipMemcpyAsync(Bul_.get(), B_.get(), sizeof(T) * w * n, hipMemcpyDeviceToDevice, stream));
         for (int p = 0; p < P - 1; ++p) {
             int64_t s, len;

@@ -150,12 +150,14 @@ struct FactorJob {          // one banded LU (dir = +1) or UL (dir = -1) of a pa
     int     boost_last;     // boost also the partition's final pivot
     int     buf;            // 0: main factor band, 1: UL band
 };
-template <class T>
-void launch_band_factor(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, hipStream_t st);
+template <class T>   // max_len: longest partition among the jobs
+void launch_band_factor(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, int64_t max_len, hipStream_t st);
 // factor_resident.hip: register-resident MFMA factorization; false if k is
-// too wide for it (the caller then uses the LDS-blocked kernels)
+// too wide for it or a partition too long (the caller then uses the
+// LDS-blocked kernels)
 template <class T>
-bool launch_band_factor_resident(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, hipStream_t st);
+bool launch_band_factor_resident(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, int64_t max_len,
+                                 hipStream_t st);
 template <class T>
 void launch_zero_pivot_check(const T* B, int k, int64_t row0, int64_t rows, int* d_flag, hipStream_t st);
 template <class T>
