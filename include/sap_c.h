@@ -11,6 +11,11 @@
  *   ~Solver()                              -> sap_destroy      (sap/solver.h:159-168)
  *   Solver::setup(BandedMatrix)            -> sap_setup_banded (sap/solver.h:339-404,
  *                                              sap/precond.h:2103-2233)
+ *   Solver::setup(cusp::csr_matrix)        -> sap_setup_csr    (sap/solver.h:339-404,
+ *                                              sap/precond.h:1656-2101, :2240-2363)
+ *   SpmvCusp::operator()(v, Av)            -> sap_csr_spmv     (sap/spmv.h:83-129)
+ *   Graph::reorder + Graph::dropOff        -> sap_reorder_csr  (sap/graph.h:446-747,
+ *     (host side of transformToBandedMatrix)                    sap/precond.h:1656-1776)
  *   Solver::solve(spmv, b, x)              -> sap_solve        (sap/solver.h:477-562)
  *   Solver::getStats()                     -> sap_get_stats    (sap/solver.h:182)
  *   Solver::getMonitorCode()               -> sap_monitor_code (sap/solver.h:183-188)
@@ -26,6 +31,9 @@
  *     alive until the last sap_solve that uses the built-in SpMV.
  *   - Banded input: row-major band, A(i,j) at d_band[i*(2k+1) + k + (j-i)]
  *     (sap/banded_matrix.h:74-92).
+ *   - CSR input: zero-based row offsets (n+1), column indices and FP64
+ *     values (nnz), the layout of cusp::csr_matrix<int,double>.  The
+ *     built-in CSR SpMV keeps non-owning pointers to them, as for the band.
  *   - Return codes mirror sap::system_error::Reason (sap/exception.h:16-23):
  *     0 OK, -1 zero pivot, -2 negative DB weight, -3 illegal update,
  *     -4 illegal solve, -5 singular matrix; plus -6 out of device memory,
@@ -166,8 +174,24 @@ int         sap_set_stream(sap_solver* s, void* hip_stream);
 
 int         sap_setup_banded(sap_solver* s, int n, int k, const double* d_band);
 
+/* Solver::setup with a CSR matrix (sap/solver.h:339-404).  With
+ * performReorder the host runs DB (performDB; applyScaling scales), RCM
+ * (when maxBandwidth > 64) and drop-off (dropOffFraction / maxBandwidth)
+ * as transformToBandedMatrix does (sap/precond.h:1656-1776), the band of
+ * the reordered matrix is factored, and every preconditioner application
+ * permutes and scales its vectors (sap/precond.h:1393-1406, :1535-1621).
+ * Without performReorder the matrix is converted to a band of half-width
+ * max|i-j| as is (sap/precond.h:2240-2363).
+ * Returns SAP_ERR_SINGULAR when DB finds no perfect matching,
+ * SAP_ERR_NEG_DB_WEIGHT on a negative reduced cost, SAP_ERR_UNSUPPORTED for
+ * testDB or for performReorder with variableBandwidth (the
+ * variable-bandwidth preconditioner is not provided). */
+int         sap_setup_csr(sap_solver* s, int n, int nnz, const int* d_row_offsets,
+                          const int* d_col_indices, const double* d_values);
+
 /* Solves A x = b.  d_x holds the initial guess on entry and the solution on
- * return.  spmv == NULL uses the built-in banded SpMV of the setup matrix.
+ * return.  spmv == NULL uses the built-in SpMV of the setup matrix (banded or
+ * CSR).
  * nrhs > 1 solves the columns of b/x (leading dimension n) one after another. */
 int         sap_solve(sap_solver* s, sap_spmv_fn spmv, void* ctx,
                       const double* d_b, double* d_x, int nrhs);
@@ -183,6 +207,31 @@ int         sap_precond_apply(sap_solver* s, const double* d_v, double* d_z);
 /* y = A x for a row-major FP64 band on the given HIP stream (NULL: default). */
 int         sap_band_spmv(int n, int k, const double* d_band, const double* d_x,
                           double* d_y, void* hip_stream);
+
+/* y = A x for an FP64 CSR matrix on the given HIP stream (NULL: default). */
+int         sap_csr_spmv(int n, int nnz, const int* d_row_offsets, const int* d_col_indices,
+                         const double* d_values, const double* d_x, double* d_y, void* hip_stream);
+
+/* The host reordering of a CSR setup on its own (host arrays in and out, no
+ * GPU): DB, RCM and drop-off with the options' performDB, applyScaling,
+ * dbFirstStageOnly, maxBandwidth and dropOffFraction, in the precision of
+ * precValueType.  NULL on failure with *err set (SAP_ERR_SINGULAR,
+ * SAP_ERR_NEG_DB_WEIGHT, SAP_ERR_BADARG, SAP_ERR_OOM).
+ *   db_perm[i]   new row of original row i after DB      (dbRowPerm)
+ *   row_perm[i]  new row of original row i, DB then RCM  (m_optPerm)
+ *   col_reord[j] original column of new column j         (m_optReordering)
+ *   row_scale, col_scale: DB scalings (all 1 without DB)
+ *   row_offsets (n+1), col_indices, values (nnz): the reordered, scaled and
+ *   dropped-off matrix whose band is factored. */
+typedef struct sap_reordering sap_reordering;
+sap_reordering* sap_reorder_csr(int n, const int* h_row_offsets, const int* h_col_indices,
+                                const double* h_values, const sap_options* opts, int* err);
+int         sap_reordering_info(const sap_reordering* r, int* k, int* k_db, int* k_reorder,
+                                double* drop_off, int* nnz);
+int         sap_reordering_get(const sap_reordering* r, int* db_perm, int* row_perm, int* col_reord,
+                               double* row_scale, double* col_scale, int* row_offsets,
+                               int* col_indices, double* values);
+void        sap_reordering_free(sap_reordering* r);
 
 /* Kernel-family timing since the last sap_reset_profile (profile = 1). */
 int         sap_get_profile(const sap_solver* s, sap_kernel_profile* out, int max_entries);

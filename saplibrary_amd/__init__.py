@@ -5,7 +5,8 @@ in include/sap_c.h).  This module mirrors the reference's C++ interface
 (spikegpu/SaPLibrary sap/solver.h: ``Options``, ``Stats``,
 ``Solver(numPartitions, opts)`` with ``setup``/``solve``/``getStats``/
 ``getMonitorCode``/``getMonitorMessage``; sap/banded_matrix.h:
-``BandedMatrix``; sap/spmv.h: ``MVBanded``) over device tensors.  PyTorch is
+``BandedMatrix``; cusp::csr_matrix: ``CsrMatrix``; sap/spmv.h: ``MVBanded``,
+``SpmvCusp``) over device tensors.  PyTorch is
 only used for device memory; it is imported first so the library binds to
 the HIP runtime already loaded in the process.
 
@@ -131,7 +132,8 @@ EXPORTS = (
     "sap_default_options", "sap_create", "sap_destroy", "sap_set_stream", "sap_setup_banded", "sap_solve",
     "sap_get_stats", "sap_monitor_code", "sap_monitor_message", "sap_last_error", "sap_precond_apply",
     "sap_band_spmv", "sap_get_profile", "sap_reset_profile", "sap_get_factors", "sap_get_spikes",
-    "sap_num_partitions", "sap_gen_banded", "sap_gen_vector",
+    "sap_num_partitions", "sap_gen_banded", "sap_gen_vector", "sap_setup_csr", "sap_csr_spmv",
+    "sap_reorder_csr", "sap_reordering_info", "sap_reordering_get", "sap_reordering_free",
 )
 
 _lib = None
@@ -171,6 +173,15 @@ def lib():
     L.sap_num_partitions.argtypes = [vp]
     L.sap_gen_banded.argtypes = [i64, i, d, ctypes.c_uint64, vp, vp]
     L.sap_gen_vector.argtypes = [i64, ctypes.c_uint64, vp, vp]
+    L.sap_setup_csr.argtypes = [vp, i, i, vp, vp, vp]
+    L.sap_csr_spmv.argtypes = [i, i, vp, vp, vp, vp, vp, vp]
+    ip, dp = ctypes.POINTER(i), ctypes.POINTER(d)
+    L.sap_reorder_csr.argtypes = [i, ip, ip, dp, ctypes.POINTER(Options), ip]
+    L.sap_reorder_csr.restype = vp
+    L.sap_reordering_info.argtypes = [vp, ip, ip, ip, dp, ip]
+    L.sap_reordering_get.argtypes = [vp, ip, ip, ip, dp, dp, ip, ip, dp]
+    L.sap_reordering_free.argtypes = [vp]
+    L.sap_reordering_free.restype = None
     _lib = L
     return L
 
@@ -178,6 +189,12 @@ def lib():
 def _dev_f64(t, name):
     if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
         raise TypeError(f"{name} must be a contiguous float64 device tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _dev_i32(t, name):
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.int32 or not t.is_cuda or not t.is_contiguous():
+        raise TypeError(f"{name} must be a contiguous int32 device tensor")
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -219,6 +236,99 @@ def band_spmv(A, x, y):
     return y
 
 
+class CsrMatrix:
+    """FP64 CSR matrix on the device with zero-based int32 indices (the
+    cusp::csr_matrix<int, double, device_memory> the reference's Solver::setup
+    and SpmvCusp take)."""
+
+    def __init__(self, row_offsets, column_indices, values, num_cols=None):
+        _dev_i32(row_offsets, "row_offsets")
+        _dev_i32(column_indices, "column_indices")
+        _dev_f64(values, "values")
+        if row_offsets.dim() != 1 or column_indices.shape != values.shape:
+            raise ValueError("inconsistent CSR arrays")
+        self.row_offsets = row_offsets
+        self.column_indices = column_indices
+        self.values = values
+        self.num_rows = row_offsets.shape[0] - 1
+        self.num_cols = self.num_rows if num_cols is None else num_cols
+        self.num_entries = values.shape[0]
+        if self.num_cols != self.num_rows:
+            raise ValueError("the matrix must be square")
+
+    @staticmethod
+    def from_host(row_offsets, column_indices, values, device="cuda"):
+        """From numpy / host arrays (scipy.sparse.csr_matrix fields)."""
+        return CsrMatrix(torch.as_tensor(row_offsets, dtype=torch.int32).to(device).contiguous(),
+                         torch.as_tensor(column_indices, dtype=torch.int32).to(device).contiguous(),
+                         torch.as_tensor(values, dtype=torch.float64).to(device).contiguous())
+
+    def _ptrs(self):
+        return (_dev_i32(self.row_offsets, "row_offsets"), _dev_i32(self.column_indices, "column_indices"),
+                _dev_f64(self.values, "values"))
+
+
+def csr_spmv(A, x, y):
+    """y = A x for a CsrMatrix (the SpmvCusp functor, sap/spmv.h:83-129)."""
+    rp, ci, val = A._ptrs()
+    _check(lib().sap_csr_spmv(A.num_rows, A.num_entries, rp, ci, val, _dev_f64(x, "x"), _dev_f64(y, "y"), None))
+    return y
+
+
+class SpmvCusp:
+    """SPMV functor over a CsrMatrix (sap/spmv.h:83-129)."""
+
+    def __init__(self, A):
+        self.A = A
+        self.num_rows = A.num_rows
+
+    def __call__(self, v, Av):
+        csr_spmv(self.A, v, Av)
+
+
+def reorder_csr(row_offsets, column_indices, values, opts=None):
+    """The host reordering of a CSR setup (DB, RCM, drop-off; sap/graph.h:446-747)
+    on host arrays, without a GPU.  Returns a dict with the permutations
+    (db_perm, row_perm, col_reord), scalings, bandwidths (k, k_db, k_reorder),
+    the actual drop-off fraction and the reordered CSR (rp, ci, val).
+    Raises SapError (Matrix_singular, Negative_DB_weight)."""
+    import numpy as np
+    rp = np.ascontiguousarray(row_offsets, dtype=np.int32)
+    ci = np.ascontiguousarray(column_indices, dtype=np.int32)
+    val = np.ascontiguousarray(values, dtype=np.float64)
+    n = rp.shape[0] - 1
+    o = opts if opts is not None else Options()
+    ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+    err = ctypes.c_int(0)
+    L = lib()
+    r = L.sap_reorder_csr(n, rp.ctypes.data_as(ip), ci.ctypes.data_as(ip), val.ctypes.data_as(dp),
+                          ctypes.byref(o), ctypes.byref(err))
+    if not r:
+        raise SapError(err.value, "reorder_csr failed")
+    r = ctypes.c_void_p(r)
+    try:
+        k, kdb, kre, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        drop = ctypes.c_double()
+        _check(L.sap_reordering_info(r, ctypes.byref(k), ctypes.byref(kdb), ctypes.byref(kre), ctypes.byref(drop),
+                                     ctypes.byref(nnz)))
+        out = {name: np.empty(n, dtype=np.int32) for name in ("db_perm", "row_perm", "col_reord")}
+        out["row_scale"] = np.empty(n)
+        out["col_scale"] = np.empty(n)
+        out["rp"] = np.empty(n + 1, dtype=np.int32)
+        out["ci"] = np.empty(max(nnz.value, 1), dtype=np.int32)
+        out["val"] = np.empty(max(nnz.value, 1))
+        _check(L.sap_reordering_get(r, *(out[f].ctypes.data_as(ip) for f in ("db_perm", "row_perm", "col_reord")),
+                                    out["row_scale"].ctypes.data_as(dp), out["col_scale"].ctypes.data_as(dp),
+                                    out["rp"].ctypes.data_as(ip), out["ci"].ctypes.data_as(ip),
+                                    out["val"].ctypes.data_as(dp)))
+        out["ci"] = out["ci"][:nnz.value]
+        out["val"] = out["val"][:nnz.value]
+        out.update(k=k.value, k_db=kdb.value, k_reorder=kre.value, dropoff=drop.value)
+        return out
+    finally:
+        L.sap_reordering_free(r)
+
+
 class MVBanded:
     """SPMV functor over a BandedMatrix (sap/spmv.h:27-74)."""
 
@@ -249,19 +359,23 @@ class Solver:
             self.h = None
 
     def setup(self, A):
-        """Builds the preconditioner (sap/solver.h:339-404).  Raises SapError."""
-        if not isinstance(A, BandedMatrix):
-            raise TypeError("setup() takes a BandedMatrix")
-        _check(lib().sap_setup_banded(self.h, A.num_rows, A.m_k, _dev_f64(A.band, "A")), self.h)
+        """Builds the preconditioner (sap/solver.h:339-404) from a BandedMatrix
+        or a CsrMatrix.  Raises SapError."""
+        if isinstance(A, BandedMatrix):
+            _check(lib().sap_setup_banded(self.h, A.num_rows, A.m_k, _dev_f64(A.band, "A")), self.h)
+        elif isinstance(A, CsrMatrix):
+            _check(lib().sap_setup_csr(self.h, A.num_rows, A.num_entries, *A._ptrs()), self.h)
+        else:
+            raise TypeError("setup() takes a BandedMatrix or a CsrMatrix")
         self._A = A
         return True
 
     def solve(self, spmv, b, x):
         """Solves A x = b (sap/solver.h:477-562); x holds the initial guess.
-        spmv: None (built-in banded SpMV of the setup matrix), an MVBanded of
-        that matrix, or any callable f(v, Av) on device tensors."""
+        spmv: None (built-in SpMV of the setup matrix), an MVBanded / SpmvCusp
+        of that matrix, or any callable f(v, Av) on device tensors."""
         n = b.shape[0]
-        if spmv is None or (isinstance(spmv, MVBanded) and spmv.A is self._A):
+        if spmv is None or (isinstance(spmv, (MVBanded, SpmvCusp)) and spmv.A is self._A):
             fn = SPMV_FN()
         else:
             def _cb(ctx, pv, pav, nn):
@@ -306,7 +420,7 @@ class Solver:
 
     def factors(self):
         import numpy as np
-        n, k = self._A.num_rows, self._A.m_k
+        n, k = self._A.num_rows, self.getStats().bandwidth
         out = np.empty((n, 2 * k + 1))
         _check(lib().sap_get_factors(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size), self.h)
         return out
@@ -315,7 +429,7 @@ class Solver:
         """(WV, S): spike tips (P-1, 2, k, k) stored RHS-major and the factored
         Schur blocks (P-1, k, k) stored column-major."""
         import numpy as np
-        k = self._A.m_k
+        k = self.getStats().bandwidth
         P = self.numPartitions
         wv = np.empty((max(P - 1, 0), 2, k, k))
         s = np.empty((max(P - 1, 0), k, k))

@@ -149,6 +149,80 @@ __global__ __launch_bounds__(256) void band_to_colmajor_kernel(int64_t n, int k,
     }
 }
 
+// ---------------------------------------------------------------------------
+// CSR entries into the zero-filled column-major band, cast to the band type
+// (the COO scatter copyFromCOOMatrixToBandedMatrix,
+// sap/device/data_transfer.cuh:421-448, driven from the CSR rows: each
+// entry lands on its own band slot, so the order does not matter).
+// ---------------------------------------------------------------------------
+template <class Tin, class T>
+__global__ void csr_to_band_kernel(int64_t n, int k, const int* __restrict__ rp, const int* __restrict__ ci,
+                                   const Tin* __restrict__ vals, T* __restrict__ B)
+{
+    const int64_t w = 2 * k + 1;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+        const int e1 = rp[r + 1];
+        for (int e = rp[r]; e < e1; ++e) {
+            const int64_t c = ci[e];
+            B[c * w + k + r - c] = (T)vals[e];
+        }
+    }
+}
+
+// max |row - col| over the stored entries (the half-bandwidth of
+// convertToBandedMatrix(CSR), sap/precond.h:2253-2257)
+__global__ void csr_bandwidth_kernel(int64_t n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                     int* __restrict__ out)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int m = 0;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride)
+        for (int e = rp[r]; e < rp[r + 1]; ++e) m = max(m, abs((int)r - ci[e]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// ---------------------------------------------------------------------------
+// y = A x for FP64 CSR (the SpmvCusp functor, sap/spmv.h:83-129): G lanes per
+// row, each summing its entries in order, then a fixed shuffle tree.
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ __launch_bounds__(256) void csr_spmv_kernel(int64_t n, const int* __restrict__ rp,
+                                                       const int* __restrict__ ci, const double* __restrict__ val,
+                                                       const double* __restrict__ x, double* __restrict__ y)
+{
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    double s = 0.0;
+    if (row < n)
+        for (int j = rp[row] + lane; j < rp[row + 1]; j += G) s = fma(val[j], x[ci[j]], s);
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_down(s, off, G);
+    if (lane == 0 && row < n) y[row] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Vector permutations of the reordered preconditioner, in the
+// preconditioner type as the reference computes them on PrecVectors:
+//   leftTrans  (scaleAndPermute, sap/precond.h:1582-1598): w[perm[i]] = v[i] * s[i]
+//   rightTrans (permuteAndScale, sap/precond.h:1560-1576): w[perm[i]] = v[i] * s[perm[i]]
+// (scale may be null: permute only).
+// ---------------------------------------------------------------------------
+template <class T, bool SCALE_AT_TARGET>
+__global__ void permute_scale_kernel(int64_t n, const double* __restrict__ v, const int* __restrict__ perm,
+                                     const T* __restrict__ scale, double* __restrict__ w)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int p = perm[i];
+        T x = (T)v[i];
+        if (scale) x = x * scale[SCALE_AT_TARGET ? p : i];
+        w[p] = (double)x;
+    }
+}
+
 }  // namespace
 
 void launch_gen_banded(int64_t n, int k, double dom, uint64_t seed, double* band, hipStream_t st)
@@ -194,5 +268,62 @@ void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipS
 
 template void launch_band_to_colmajor<float>(int64_t, int, const double*, float*, hipStream_t);
 template void launch_band_to_colmajor<double>(int64_t, int, const double*, double*, hipStream_t);
+
+template <class Tin, class T>
+void launch_csr_to_band(int64_t n, int k, const int* rp, const int* ci, const Tin* vals, T* B, hipStream_t st)
+{
+    int64_t g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL((csr_to_band_kernel<Tin, T>), dim3((unsigned)g), dim3(256), 0, st, n, k, rp, ci, vals, B);
+    SAP_LAUNCH_CHECK();
+}
+template void launch_csr_to_band<double, float>(int64_t, int, const int*, const int*, const double*, float*,
+                                                hipStream_t);
+template void launch_csr_to_band<double, double>(int64_t, int, const int*, const int*, const double*, double*,
+                                                 hipStream_t);
+template void launch_csr_to_band<float, float>(int64_t, int, const int*, const int*, const float*, float*,
+                                               hipStream_t);
+
+void launch_csr_bandwidth(int64_t n, const int* rp, const int* ci, int* out, hipStream_t st)
+{
+    int64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(csr_bandwidth_kernel, dim3((unsigned)g), dim3(256), 0, st, n, rp, ci, out);
+    SAP_LAUNCH_CHECK();
+}
+
+void launch_csr_spmv(int64_t n, int64_t nnz, const int* rp, const int* ci, const double* val, const double* x,
+                     double* y, hipStream_t st)
+{
+    const int bs = 256;
+    const double avg = n > 0 ? (double)nnz / (double)n : 0.0;
+    if (avg >= 24) {
+        const int64_t blocks = (n * 32 + bs - 1) / bs;
+        hipLaunchKernelGGL(csr_spmv_kernel<32>, dim3((unsigned)blocks), dim3(bs), 0, st, n, rp, ci, val, x, y);
+    } else if (avg >= 6) {
+        const int64_t blocks = (n * 8 + bs - 1) / bs;
+        hipLaunchKernelGGL(csr_spmv_kernel<8>, dim3((unsigned)blocks), dim3(bs), 0, st, n, rp, ci, val, x, y);
+    } else {
+        const int64_t blocks = (n * 2 + bs - 1) / bs;
+        hipLaunchKernelGGL(csr_spmv_kernel<2>, dim3((unsigned)blocks), dim3(bs), 0, st, n, rp, ci, val, x, y);
+    }
+    SAP_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_permute_scale(int64_t n, const double* v, const int* perm, const T* scale, bool scale_at_target,
+                          double* w, hipStream_t st)
+{
+    int64_t g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    if (scale_at_target)
+        hipLaunchKernelGGL((permute_scale_kernel<T, true>), dim3((unsigned)g), dim3(256), 0, st, n, v, perm, scale, w);
+    else
+        hipLaunchKernelGGL((permute_scale_kernel<T, false>), dim3((unsigned)g), dim3(256), 0, st, n, v, perm, scale, w);
+    SAP_LAUNCH_CHECK();
+}
+template void launch_permute_scale<float>(int64_t, const double*, const int*, const float*, bool, double*, hipStream_t);
+template void launch_permute_scale<double>(int64_t, const double*, const int*, const double*, bool, double*,
+                                           hipStream_t);
 
 }  // namespace sap_amd

@@ -140,15 +140,13 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
     const int64_t w = 2 * k + 1;
     const int64_t len = job.len;
     const int dir = job.dir;
-    const int64_t origin = dir > 0 ? job.start * w : (job.start + len) * w - 1;
     const int wm1 = 2 * k;   // next column, same row
 
     extern __shared__ unsigned char smem_raw[];
     T* const sm = reinterpret_cast<T*>(smem_raw);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: slot logic stays scalar
 
-    // Entry (j0 + dr, j0 + dc) of the partition is P[dir * (dc * 2k + dr)]
-    // with P = base + origin + dir * (j0 * (2k + 1) + k); rem = len - j0.
+    // rem = len - j0 is the number of partition rows left from panel row j0.
     auto inb = [&](int dr, int dc, int rem) { return dr < rem && dc < rem && dr - dc <= k && dc - dr <= k; };
     // Tile loads are buffer loads over the partition's storage: every element
     // is one unconditional load, and entries outside the band or the
@@ -240,7 +238,6 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         T* rpiv = bcur + LY::rp_off;
         T* L11T = sm + zoff + LY::lt_off;
         const int64_t j0 = 32 * p;
-        T* __restrict__ P = base + origin + dir * (j0 * w + k);
         const int rem = (int)min(len - j0, (int64_t)(1 << 30));
         const int nb = min(32, rem);
         const int sh = (int)((2 * p) % KT);

@@ -1,0 +1,702 @@
+// Host-side reordering pipeline for sparse (CSR) input: diagonal boosting
+// (DB, a maximum-product matching with row/column scaling), reverse
+// Cuthill-McKee and element drop-off (sap::Graph, sap/graph.h).
+//
+// This is integer (and, for the DB costs, FP64) host work whose results are
+// permutations, scalings and bandwidths; they follow the reference step by
+// step so that they are identical to the reference's on the same input,
+// including its tie-breaking: the same traversal orders, stable counting
+// sorts, and std::priority_queue with the same comparison (libstdc++'s heap
+// decides among equal keys exactly as it does for the reference).  The
+// matrix values carry the preconditioner type T, as PrecMatrixCsrH does.
+#include "graph.hpp"
+
+#include <chrono>
+
+#include <algorithm>
+#include <cmath>
+#include <queue>
+#include <stdexcept>
+
+namespace sap_amd {
+
+namespace {
+
+constexpr double LOC_INFINITY = 1e37;   // Graph<T>::LOC_INFINITY (sap/graph.h:417)
+
+// (node, key) with the reference's CompareValue: the queue's top is the
+// smallest key (sap/graph.h:394-398).
+template <class K>
+struct NodeKey {
+    int node;
+    K key;
+};
+template <class K>
+struct KeyGreater {
+    bool operator()(const NodeKey<K>& a, const NodeKey<K>& b) const { return a.key > b.key; }
+};
+
+// Stable counting sort of entries by a new row index: entries of one row
+// keep their input order (the reference fills each row from its end while
+// walking the entries backwards, sap/graph.h:1334-1347).
+template <class T>
+void regroup_rows(int n, const std::vector<int>& new_row, std::vector<int>& ci, std::vector<T>& val,
+                  std::vector<int>& rp)
+{
+    const size_t nnz = new_row.size();
+    rp.assign(n + 1, 0);
+    for (size_t i = 0; i < nnz; ++i) rp[new_row[i] + 1]++;
+    for (int i = 0; i < n; ++i) rp[i + 1] += rp[i];
+    std::vector<int> pos(rp.begin(), rp.end() - 1);
+    std::vector<int> ci2(nnz);
+    std::vector<T> v2(nnz);
+    for (size_t i = 0; i < nnz; ++i) {
+        const int d = pos[new_row[i]]++;
+        ci2[d] = ci[i];
+        v2[d] = val[i];
+    }
+    ci.swap(ci2);
+    val.swap(v2);
+}
+
+// ------------------------------------------------------------------- DB
+// Costs c = log(rowmax / |a|) per CSR entry (get_csc_matrix +
+// getResidualValues, sap/graph.h:1829-1857, sap/device/db.cuh:12-37); the
+// row maxima come from a reduce-by-key over the entries, one per non-empty
+// row, in row order.
+template <class T>
+struct DbState {
+    int n;
+    const std::vector<int>& rp;
+    const std::vector<int>& ci;
+    std::vector<double> c;          // reduced-cost base values
+    std::vector<double> rowmax;     // per non-empty row, in order
+    std::vector<double> u, v;       // duals: u by column index, v by CSR row
+    std::vector<int> match;         // match[column] = CSR row
+    std::vector<int> rev_entry;     // rev_entry[row] = entry index of its matched edge
+    std::vector<char> matched, rev_matched;
+    // Dijkstra scratch kept between augmentations (m_DB_*)
+    std::vector<char> inB, visited;
+    std::vector<int> Blist;
+    std::vector<double> dval;
+    std::vector<int> prev, irn;
+
+    DbState(int n_, const std::vector<int>& rp_, const std::vector<int>& ci_) : n(n_), rp(rp_), ci(ci_) {}
+};
+
+template <class T>
+void db_costs(DbState<T>& S, const std::vector<T>& val)
+{
+    const int n = S.n;
+    const size_t nnz = val.size();
+    S.c.resize(nnz);
+    for (size_t j = 0; j < nnz; ++j) S.c[j] = std::fabs((double)val[j]);
+    S.rowmax.assign(n, 0.0);
+    int run = 0;
+    for (int i = 0; i < n; ++i) {
+        if (S.rp[i] == S.rp[i + 1]) continue;
+        double m = S.c[S.rp[i]];
+        for (int j = S.rp[i] + 1; j < S.rp[i + 1]; ++j) m = std::max(m, S.c[j]);
+        S.rowmax[run++] = m;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = S.rp[i]; j < S.rp[i + 1]; ++j) S.c[j] = std::log(S.rowmax[i] / S.c[j]);
+}
+
+// Initial duals and greedy matching (init_reduced_cval, sap/graph.h:1875-2035)
+template <class T>
+void db_init(DbState<T>& S, bool first_stage_only)
+{
+    const int n = S.n;
+    const auto& rp = S.rp;
+    const auto& ci = S.ci;
+    auto& c = S.c;
+    S.u.assign(n, 0.0);
+    {
+        int run = 0;   // reduce-by-key minimum, one value per non-empty row
+        for (int i = 0; i < n; ++i) {
+            if (rp[i] == rp[i + 1]) continue;
+            double m = c[rp[i]];
+            for (int j = rp[i] + 1; j < rp[i + 1]; ++j) m = std::min(m, c[j]);
+            S.u[run++] = m;
+        }
+    }
+    S.v.assign(n, LOC_INFINITY);
+    S.match.assign(n, 0);
+    S.rev_entry.assign(n, 0);
+    S.matched.assign(n, 0);
+    S.rev_matched.assign(n, 0);
+    for (int i = 0; i < n; ++i) {
+        int min_idx = -1;
+        for (int j = rp[i]; j < rp[i + 1]; ++j) {
+            if (c[j] > LOC_INFINITY / 2.0) continue;
+            const double t = c[j] - S.u[ci[j]];
+            if (S.v[i] > t) {
+                S.v[i] = t;
+                min_idx = j;
+            }
+        }
+        if (min_idx >= 0) {
+            const int col = ci[min_idx];
+            if (!S.matched[col]) {
+                S.rev_matched[i] = 1;
+                S.matched[col] = 1;
+                S.match[col] = i;
+                S.rev_entry[i] = min_idx;
+            }
+        }
+    }
+    if (first_stage_only) {
+        const double CMP_THRESHOLD = 1e-10;
+        for (int i = 0; i < n; ++i) {
+            if (S.rev_matched[i]) continue;
+            const int cur_v = (int)S.v[i];   // the reference truncates the dual to int here
+            bool found = false;
+            for (int j = rp[i]; j < rp[i + 1] && !found; ++j) {
+                if (c[j] > LOC_INFINITY / 2.0) continue;
+                const int col = ci[j];
+                if (!S.matched[col]) continue;
+                if (c[j] - S.u[col] - cur_v > CMP_THRESHOLD) continue;
+                const int i2 = S.match[col];
+                for (int j2 = rp[i2]; j2 < rp[i2 + 1]; ++j2) {
+                    if (c[j2] > LOC_INFINITY / 2.0) continue;
+                    const int col2 = ci[j2];
+                    if (S.matched[col2]) continue;
+                    S.matched[col2] = 1;
+                    S.match[col2] = i2;
+                    S.rev_entry[i2] = j2;
+                    S.match[col] = i;
+                    S.rev_matched[i] = 1;
+                    S.rev_entry[i] = j;
+                    found = true;
+                    break;
+                }
+            }
+        }
+        for (int i = 0; i < n; ++i) {
+            if (S.rev_matched[i]) continue;
+            for (int j = rp[i]; j < rp[i + 1]; ++j) {
+                const int col = ci[j];
+                if (!S.matched[col]) {
+                    S.matched[col] = 1;
+                    S.rev_matched[i] = 1;
+                    S.match[col] = i;
+                    S.rev_entry[i] = j;
+                    break;
+                }
+            }
+        }
+        for (int i = 0; i < n; ++i) {
+            if (S.rev_matched[i]) continue;
+            bool found = false;
+            for (int j = rp[i]; j < rp[i + 1] && !found; ++j) {
+                const int col = ci[j];
+                const int i2 = S.match[col];
+                for (int j2 = rp[i2]; j2 < rp[i2 + 1]; ++j2) {
+                    const int col2 = ci[j2];
+                    if (S.matched[col2]) continue;
+                    found = true;
+                    S.matched[col2] = 1;
+                    S.match[col2] = i2;
+                    S.rev_entry[i2] = j2;
+                    S.match[col] = i;
+                    S.rev_matched[i] = 1;
+                    S.rev_entry[i] = j;
+                    break;
+                }
+            }
+        }
+        int last_j = 0;
+        for (int i = 0; i < n; ++i) {
+            if (S.rev_matched[i]) continue;
+            for (int j = last_j; j < n; ++j) {
+                if (S.matched[j]) continue;
+                S.rev_matched[i] = S.matched[j] = 1;
+                S.match[j] = i;
+                last_j = j + 1;
+                break;
+            }
+        }
+        std::fill(S.u.begin(), S.u.end(), 1.0);
+        std::fill(S.v.begin(), S.v.end(), 1.0);
+    } else {
+        for (int i = 0; i < n; ++i) {
+            if (!S.matched[i]) S.u[i] = 0.0;
+            if (!S.rev_matched[i]) S.v[i] = 0.0;
+        }
+    }
+}
+
+// Shortest augmenting path from CSR row `init` (find_shortest_aug_path,
+// sap/graph.h:2045-2173): Dijkstra over reduced costs, then augmentation and
+// dual update.
+template <class T>
+void db_augment(DbState<T>& S, int init)
+{
+    const auto& rp = S.rp;
+    const auto& ci = S.ci;
+    const auto& c = S.c;
+    std::priority_queue<NodeKey<double>, std::vector<NodeKey<double>>, KeyGreater<double>> Q;
+    double lsp = 0.0, lsap = LOC_INFINITY;
+    int cur = init, isap = -1, ksap = -1, b_cnt = 0;
+    S.prev[init] = -1;
+    while (true) {
+        for (int i = rp[cur]; i < rp[cur + 1]; ++i) {
+            const int col = ci[i];
+            if (S.inB[col]) continue;
+            if (c[i] > LOC_INFINITY / 2.0) continue;
+            const double reduced = c[i] - S.u[col] - S.v[cur];
+            if (reduced + 1e-10 < 0) throw Error(SAP_ERR_NEG_DB_WEIGHT, "Negative reduced weight in DB.");
+            const double d_new = lsp + reduced;
+            if (d_new < lsap) {
+                if (!S.matched[col]) {
+                    lsap = d_new;
+                    isap = col;
+                    ksap = i;
+                    S.match[isap] = cur;
+                } else if (d_new < S.dval[col]) {
+                    S.dval[col] = d_new;
+                    S.prev[S.match[col]] = cur;
+                    Q.push({col, d_new});
+                    S.irn[col] = i;
+                }
+            }
+        }
+        NodeKey<double> top{0, 0.0};
+        bool found = false;
+        while (!Q.empty()) {
+            top = Q.top();
+            Q.pop();
+            if (S.visited[top.node]) continue;
+            found = true;
+            break;
+        }
+        if (!found) break;
+        const int t = top.node;
+        S.visited[t] = 1;
+        lsp = top.key;
+        if (lsap <= lsp) {
+            S.visited[t] = 0;
+            S.dval[t] = LOC_INFINITY;
+            break;
+        }
+        S.inB[t] = 1;
+        S.Blist[b_cnt++] = t;
+        cur = S.match[t];
+    }
+    if (lsap < LOC_INFINITY / 2.0) {
+        S.matched[isap] = 1;
+        cur = S.match[isap];
+        S.v[cur] = c[ksap];
+        while (S.prev[cur] >= 0) {
+            S.match[isap] = cur;
+            int next_k = S.rev_entry[cur];
+            const int next_i = ci[next_k];
+            next_k = S.irn[next_i];
+            S.rev_entry[cur] = ksap;
+            cur = S.prev[cur];
+            isap = next_i;
+            ksap = next_k;
+        }
+        S.match[isap] = cur;
+        S.rev_entry[cur] = ksap;
+        S.rev_matched[cur] = 1;
+        for (int i = 0; i < b_cnt; ++i) {
+            const int row = S.Blist[i];
+            const int jv = S.match[row];
+            const int tk = S.rev_entry[jv];
+            S.u[row] += S.dval[row] - lsap;
+            S.v[jv] = c[tk] - S.u[row];
+            S.dval[row] = LOC_INFINITY;
+            S.visited[row] = 0;
+            S.inB[row] = 0;
+        }
+        while (!Q.empty()) {
+            S.dval[Q.top().node] = LOC_INFINITY;
+            Q.pop();
+        }
+    }
+}
+
+// ------------------------------------------------------------------- RCM
+// Symmetric adjacency without self loops, each (i, j) edge inserted as
+// (i, j) and (j, i) in entry order, then grouped by row stably
+// (buildTopology, sap/graph.h:1766-1809).
+void build_topology(int n, const std::vector<int>& rows, const std::vector<int>& cols, std::vector<int>& arp,
+                    std::vector<int>& aci)
+{
+    std::vector<int> r2, c2;
+    r2.reserve(rows.size() * 2);
+    c2.reserve(rows.size() * 2);
+    for (size_t e = 0; e < rows.size(); ++e) {
+        const int from = rows[e], to = cols[e];
+        if (from == to) continue;
+        r2.push_back(from); c2.push_back(to);
+        r2.push_back(to); c2.push_back(from);
+    }
+    std::vector<float> dummy(r2.size());
+    regroup_rows(n, r2, c2, dummy, arp);
+    aci.swap(c2);
+}
+
+int max_edge_length(const std::vector<int>& rows, const std::vector<int>& cols, const int* perm)
+{
+    int bw = 0;
+    for (size_t e = 0; e < rows.size(); ++e) {
+        const int a = perm ? perm[rows[e]] : rows[e], b = perm ? perm[cols[e]] : cols[e];
+        bw = std::max(bw, std::abs(a - b));
+    }
+    return bw;
+}
+
+// RCM (Graph::RCM, sap/graph.h:1373-1583).  As in the reference, the trial
+// loop's level bookkeeping never raises max_level above 0, so at most two
+// trials run; the second start vertex is chosen from the vertices at level 0
+// by the degrees of vertices 0..count-1.  Returns the bandwidth and fills
+// reord (new -> old) and perm (old -> new).
+int rcm(int n, const std::vector<int>& rp, const std::vector<int>& ci, std::vector<int>& reord, std::vector<int>& perm)
+{
+    reord.resize(n);
+    perm.resize(n);
+    for (int i = 0; i < n; ++i) reord[i] = i;
+    std::vector<int> rows(ci.size());
+    for (int i = 0; i < n; ++i)
+        for (int j = rp[i]; j < rp[i + 1]; ++j) rows[j] = i;
+    std::vector<int> deg(n);
+    for (int i = 0; i < n; ++i) deg[i] = rp[i + 1] - rp[i];
+    int bandwidth = max_edge_length(rows, ci, nullptr);
+    std::vector<int> arp, aci;
+    build_topology(n, rows, ci, arp, aci);
+
+    const int MAX_NUM_TRIAL = 5, BANDWIDTH_THRESHOLD = 256;
+    std::vector<char> tried(n, 0);
+    std::vector<int> pushed(n, -1), levels(n, 0), tmp(n), tperm(n);
+    const int max_level = 0;
+    int p_max_level = 0;
+    for (int trial = 0; trial < MAX_NUM_TRIAL; ++trial) {
+        std::queue<int> q;
+        std::priority_queue<NodeKey<int>, std::vector<NodeKey<int>>, KeyGreater<int>> pq;
+        int node;
+        if (trial > 0) {
+            const int max_pos = (int)(std::max_element(levels.begin(), levels.end()) - levels.begin());
+            const int max_count = (int)std::count(levels.begin(), levels.end(), max_level);
+            if (max_count > 1) {
+                std::vector<int> verts;
+                for (int i = 0; i < n; ++i)
+                    if (levels[i] == max_level) verts.push_back(i);
+                const int pos = (int)(std::min_element(deg.begin(), deg.begin() + max_count) - deg.begin());
+                node = verts[pos];
+            } else {
+                node = max_pos;
+            }
+            while (tried[node]) node = (node + 1) % n;
+        } else {
+            node = (int)(std::min_element(deg.begin(), deg.end()) - deg.begin());
+        }
+        tried[node] = 1;
+        levels[node] = 0;
+        pushed[node] = trial;
+        q.push(node);
+        int left = n, j = 0, last = 0;
+        while (left--) {
+            if (q.empty()) {
+                left++;
+                int i;
+                for (i = last; i < n; ++i) {
+                    if (pushed[i] != trial) {
+                        q.push(i);
+                        pushed[i] = trial;
+                        last = i;
+                        break;
+                    }
+                }
+                if (i < n) continue;
+                return -1;
+            }
+            node = q.front();
+            q.pop();
+            tmp[j++] = node;
+            const int lev = levels[node];
+            for (int e = arp[node]; e < arp[node + 1]; ++e) {
+                const int t = aci[e];
+                if (pushed[t] != trial) {
+                    pushed[t] = trial;
+                    pq.push({t, deg[t]});
+                    levels[t] = lev + 1;
+                }
+            }
+            while (!pq.empty()) {
+                q.push(pq.top().node);
+                pq.pop();
+            }
+        }
+        for (int i = 0; i < n; ++i) tperm[tmp[i]] = i;
+        const int bw = max_edge_length(rows, ci, tperm.data());
+        if (bandwidth > bw) {
+            bandwidth = bw;
+            reord = tmp;
+        }
+        if (trial > 0) {
+            if (p_max_level >= max_level) break;
+            const double ratio = 1.0 * (max_level - p_max_level) / p_max_level;
+            if (ratio < 0.01) break;
+            p_max_level = max_level;
+        } else {
+            p_max_level = max_level;
+        }
+        if (bandwidth <= BANDWIDTH_THRESHOLD) break;
+    }
+    for (int i = 0; i < n; ++i) perm[reord[i]] = i;
+    return bandwidth;
+}
+
+}  // namespace
+
+template <class T>
+Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const double* val_in, const GraphOptions& o)
+{
+    Reordering<T> R;
+    R.n = n;
+    // (1) values in the preconditioner type, explicit zeros removed
+    //     (transformToBandedMatrix, sap/precond.h:1690-1710)
+    std::vector<int> rp(n + 1, 0), ci;
+    std::vector<T> val;
+    ci.reserve(rp_in[n]);
+    val.reserve(rp_in[n]);
+    for (int i = 0; i < n; ++i) {
+        for (int j = rp_in[i]; j < rp_in[i + 1]; ++j) {
+            const T v = (T)val_in[j];
+            if (v != (T)0) {
+                ci.push_back(ci_in[j]);
+                val.push_back(v);
+            }
+        }
+        rp[i + 1] = (int)ci.size();
+    }
+    const size_t nnz = ci.size();
+
+    using clock = std::chrono::steady_clock;
+    auto since = [](clock::time_point a) {
+        return std::chrono::duration<double, std::milli>(clock::now() - a).count();
+    };
+    const auto t_reorder = clock::now();
+
+    // (2) DB (Graph::reorder / Graph::DB, sap/graph.h:448-579, :1216-1361)
+    const auto t_db = clock::now();
+    R.db_perm.resize(n);
+    R.row_scale.assign(n, (T)1);
+    R.col_scale.assign(n, (T)1);
+    if (o.doDB) {
+        DbState<T> S(n, rp, ci);
+        db_costs(S, val);
+        db_init(S, o.dbFirstStageOnly);
+        S.inB.assign(n, 0);
+        S.visited.assign(n, 0);
+        S.Blist.assign(n, 0);
+        S.dval.assign(n, LOC_INFINITY);
+        S.prev.assign(n, 0);
+        S.irn.assign(n, 0);
+        for (int i = 0; i < n; ++i)
+            if (!S.rev_matched[i]) db_augment(S, i);
+        for (int i = 0; i < n; ++i)
+            if (!S.matched[i]) throw Error(SAP_ERR_SINGULAR, "Singular matrix found");
+        std::vector<double> cs(n), rs(n);
+        for (int i = 0; i < n; ++i) {
+            cs[i] = std::exp(S.u[i]);
+            rs[i] = std::exp(S.v[i]) / S.rowmax[i];
+        }
+        for (int i = 0; i < n; ++i) R.db_perm[S.match[i]] = i;
+        // the matrix is scaled unless scaling is off or only the first stage
+        // ran; the scalings are returned either way (the preconditioner
+        // applies them to vectors whenever applyScaling is set, as the
+        // reference does)
+        const bool scale = o.scale && !o.dbFirstStageOnly;
+        std::vector<int> new_row(nnz);
+        for (int i = 0; i < n; ++i) {
+            for (int l = rp[i]; l < rp[i + 1]; ++l) {
+                new_row[l] = R.db_perm[i];
+                if (scale) val[l] = (T)(rs[i] * cs[ci[l]]) * val[l];
+            }
+        }
+        for (int i = 0; i < n; ++i) {
+            R.row_scale[i] = (T)rs[i];
+            R.col_scale[i] = (T)cs[i];
+        }
+        regroup_rows(n, new_row, ci, val, rp);
+        R.time_DB = since(t_db);
+    } else {
+        for (int i = 0; i < n; ++i) R.db_perm[i] = i;
+    }
+    {
+        int kdb = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = rp[i]; j < rp[i + 1]; ++j) kdb = std::max(kdb, std::abs(i - ci[j]));
+        R.k_db = kdb;
+    }
+
+    // (3) RCM or identity (sap/graph.h:560-575)
+    std::vector<int> rcm_perm;
+    if (o.doRCM) {
+        R.k_reorder = rcm(n, rp, ci, R.col_reord, rcm_perm);
+        if (R.k_reorder < 0) throw Error(SAP_ERR_HIP, "RCM failed");
+        // permute rows and columns, regroup rows stably (sap/graph.h:1545-1577)
+        std::vector<int> new_row(nnz);
+        for (int i = 0; i < n; ++i)
+            for (int j = rp[i]; j < rp[i + 1]; ++j) {
+                new_row[j] = rcm_perm[i];
+                ci[j] = rcm_perm[ci[j]];
+            }
+        regroup_rows(n, new_row, ci, val, rp);
+    } else {
+        R.k_reorder = R.k_db;
+        R.col_reord.resize(n);
+        rcm_perm.resize(n);
+        for (int i = 0; i < n; ++i) R.col_reord[i] = rcm_perm[i] = i;
+    }
+
+    R.time_reorder = since(t_reorder);
+
+    // (4) drop-off (Graph::dropOff, sap/graph.h:600-747): entries bucketed by
+    //     |row - col| (stable), whole outer diagonals removed while the kept
+    //     sum of squares (in T) stays >= (1 - frac) of the total
+    R.k = R.k_reorder;
+    R.dropoff_actual = 0;
+    if (R.k_reorder > o.maxBandwidth || o.dropOffFrac > 0) {
+        const auto t_drop = clock::now();
+        std::vector<int> bucket(n + 1, 0);
+        for (int i = 0; i < n; ++i)
+            for (int l = rp[i]; l < rp[i + 1]; ++l) bucket[std::abs(i - ci[l])]++;
+        {
+            int run = 0;
+            for (int d = 0; d < n; ++d) { const int c = bucket[d]; bucket[d] = run; run += c; }
+        }
+        std::vector<int> cr(nnz), cc(nnz);
+        std::vector<T> cv(nnz);
+        for (int i = 0; i < n; ++i)
+            for (int l = rp[i]; l < rp[i + 1]; ++l) {
+                const int d = bucket[std::abs(i - ci[l])]++;
+                cr[d] = i;
+                cc[d] = ci[l];
+                cv[d] = val[l];
+            }
+        T norm_in = (T)0;
+        for (size_t l = 0; l < nnz; ++l) norm_in += val[l] * val[l];
+        const T frac = (T)o.dropOffFrac;
+        const T min_norm_out = ((T)1 - frac) * norm_in;
+        T norm_out = norm_in;
+        // walk from the longest edges (the end of the bucketed array)
+        // (an index past the front has no band: the walk stops there, where
+        //  the reference would read outside its array)
+        auto band_of = [&](long e) { return e < 0 ? -1 : std::abs(cr[e] - cc[e]); };
+        long first = (long)nnz - 1, last = first;
+        int final_k = band_of(first);
+        {
+            int bw;
+            while ((bw = band_of(last)) > o.maxBandwidth) last--;
+            T s = (T)0;
+            for (long e = first; e > last; --e) s += cv[e] * cv[e];
+            norm_out -= s;
+            first = last;
+            final_k = bw;
+        }
+        if (norm_out >= min_norm_out) {
+            while (true) {
+                const int bw = band_of(first);
+                if (bw == 0) { final_k = 0; break; }
+                do { last--; } while (band_of(last) == bw);
+                T s = (T)0;
+                for (long e = first; e > last; --e) s += cv[e] * cv[e];
+                if (norm_out - s < min_norm_out) break;
+                norm_out -= s;
+                first = last;
+                final_k = bw;
+            }
+        }
+        R.dropoff_actual = (double)((T)1 - norm_out / norm_in);
+        R.k = final_k;
+        // keep entries [0, first] and regroup: by column (stable), then by row
+        // filling each row from its end (so a row's columns end up descending)
+        const size_t keep = (size_t)(first + 1);
+        std::vector<int> cb(n + 1, 0);
+        for (size_t e = 0; e < keep; ++e) cb[cc[e]]++;
+        { int run = 0; for (int d = 0; d < n; ++d) { const int c = cb[d]; cb[d] = run; run += c; } }
+        std::vector<int> r1(keep), c1(keep);
+        std::vector<T> v1(keep);
+        for (size_t e = 0; e < keep; ++e) {
+            const int d = cb[cc[e]]++;
+            r1[d] = cr[e];
+            c1[d] = cc[e];
+            v1[d] = cv[e];
+        }
+        rp.assign(n + 1, 0);
+        for (size_t e = 0; e < keep; ++e) rp[r1[e] + 1]++;
+        for (int i = 0; i < n; ++i) rp[i + 1] += rp[i];
+        std::vector<int> end(rp.begin() + 1, rp.end());
+        ci.assign(keep, 0);
+        val.assign(keep, (T)0);
+        for (size_t e = 0; e < keep; ++e) {
+            const int d = --end[r1[e]];
+            ci[d] = c1[e];
+            val[d] = v1[e];
+        }
+        R.time_dropOff = since(t_drop);
+    }
+
+    // (5) final row permutation: DB then RCM (combinePermutation,
+    //     sap/precond.h:1626-1636, :2094-2098)
+    R.row_perm.resize(n);
+    for (int i = 0; i < n; ++i) R.row_perm[i] = rcm_perm[R.db_perm[i]];
+    R.rp.swap(rp);
+    R.ci.swap(ci);
+    R.val.swap(val);
+    return R;
+}
+
+BandProfile band_profile(int n, const int* rp, const int* ci, int P)
+{
+    std::vector<int> ks_row(n, 0), ks_col(n, 0);
+    const int part = n / P, rem = n % P;
+    auto part_of = [&](int l) {
+        int p = l / (part + 1);
+        if (p >= rem) p = rem + (l - rem * (part + 1)) / part;
+        return p;
+    };
+    for (int j = 0; j < n; ++j)
+        for (int e = rp[j]; e < rp[j + 1]; ++e) {
+            const int l = ci[e];
+            if (P > 1 && part_of(l) != part_of(j)) continue;   // coupling entries
+            ks_col[l] = std::max(ks_col[l], j - l);
+            ks_row[j] = std::max(ks_row[j], l - j);
+        }
+    // extents may shrink by at most one per row / column; with several
+    // partitions they are also clipped to the partition's end (the first
+    // row of each partition is left as it is)
+    if (P == 1) {
+        for (int i = 1; i < n; ++i) {
+            ks_col[i] = std::max(ks_col[i], ks_col[i - 1] - 1);
+            ks_row[i] = std::max(ks_row[i], ks_row[i - 1] - 1);
+        }
+    } else {
+        int begin = 0;
+        for (int p = 0; p < P; ++p) {
+            const int end = begin + part + (p < rem ? 1 : 0);
+            for (int j = begin + 1; j < end; ++j) {
+                ks_col[j] = std::min(std::max(ks_col[j], ks_col[j - 1] - 1), end - j - 1);
+                ks_row[j] = std::min(std::max(ks_row[j], ks_row[j - 1] - 1), end - j - 1);
+            }
+            begin = end;
+        }
+    }
+    BandProfile b;
+    for (int i = 0; i < n; ++i) {
+        b.sum_row += ks_row[i];
+        b.sum_col += ks_col[i];
+        b.sum_prod += (double)ks_row[i] * ks_col[i];
+    }
+    return b;
+}
+
+template Reordering<float> reorder_csr<float>(int, const int*, const int*, const double*, const GraphOptions&);
+template Reordering<double> reorder_csr<double>(int, const int*, const int*, const double*, const GraphOptions&);
+
+}  // namespace sap_amd

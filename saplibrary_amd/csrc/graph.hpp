@@ -1,0 +1,52 @@
+// Host-side reordering pipeline for CSR input (DB, RCM, drop-off); see
+// graph.hip.
+#pragma once
+
+#include "sap_internal.hpp"
+
+#include <vector>
+
+namespace sap_amd {
+
+struct GraphOptions {
+    bool doDB = true;              // performDB
+    bool scale = true;             // applyScaling
+    bool dbFirstStageOnly = false;
+    bool doRCM = true;             // performReorder && maxBandwidth > 64 (sap/precond.h:1726-1728)
+    int maxBandwidth = 0x7fffffff;
+    double dropOffFrac = 0;
+};
+
+template <class T>
+struct Reordering {
+    int n = 0;
+    int k_db = 0;                  // half-bandwidth after DB (Stats::bandwidthDB)
+    int k_reorder = 0;             // after RCM (Stats::bandwidthReorder)
+    int k = 0;                     // after drop-off (Stats::bandwidth)
+    double dropoff_actual = 0;     // Stats::actualDropOff
+    std::vector<int> db_perm;      // DB row permutation: original row -> row
+    std::vector<int> row_perm;     // final row permutation (DB then RCM), m_optPerm
+    std::vector<int> col_reord;    // RCM reordering: new index -> original, m_optReordering
+    std::vector<T> row_scale, col_scale;   // DB scalings (1 without DB)
+    // reordered (and dropped-off) matrix, CSR, values in T
+    std::vector<int> rp, ci;
+    std::vector<T> val;
+    // host wall times, milliseconds
+    double time_DB = 0, time_reorder = 0, time_dropOff = 0;
+};
+
+// Sums over the per-row / per-column band extents ks_row, ks_col of the
+// assembled band (Graph::assembleBandedMatrix, sap/graph.h:1019-1091 for one
+// partition, :1093-1206 over the diagonal blocks for several), from which
+// Solver::setup derives Stats::nuKf and Stats::flops_LU (sap/solver.h:358-399).
+struct BandProfile {
+    double sum_row = 0, sum_col = 0, sum_prod = 0;
+};
+BandProfile band_profile(int n, const int* rp, const int* ci, int P);
+
+// Throws Error(SAP_ERR_SINGULAR) when DB finds no perfect matching and
+// Error(SAP_ERR_NEG_DB_WEIGHT) on a negative reduced cost.
+template <class T>
+Reordering<T> reorder_csr(int n, const int* rp, const int* ci, const double* val, const GraphOptions& o);
+
+}  // namespace sap_amd

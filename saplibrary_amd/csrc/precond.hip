@@ -3,6 +3,7 @@
 #include "precond.hpp"
 
 #include <algorithm>
+#include <chrono>
 
 namespace sap_amd {
 
@@ -40,8 +41,36 @@ bool Precond<T>::zero_pivots(const T* band, int64_t row0, int64_t rows)
     return h != 0;
 }
 
+namespace {
+
+double wall_now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Events {
+    hipEvent_t e[6];
+    Events()
+    {
+        for (auto& x : e) SAP_HIP_CHECK(hipEventCreate(&x));
+    }
+    ~Events()
+    {
+        for (auto& x : e) (void)hipEventDestroy(x);
+    }
+};
+
+template <class V>
+void upload(DevBuf<V>& d, const std::vector<V>& h, hipStream_t st)
+{
+    d.resize(h.size());
+    if (!h.empty()) SAP_HIP_CHECK(hipMemcpyAsync(d.get(), h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice, st));
+}
+
+}  // namespace
+
 template <class T>
-int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
+void Precond<T>::begin_setup(int64_t n_, int k_)
 {
     n = n_;
     k = k_;
@@ -51,26 +80,8 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
     const int64_t maxP = std::max<int64_t>(1, n / std::max(k + 1, 2 * k));
     P = (int)std::min<int64_t>(cfg.numPartitions, maxP);
     if (P < 1) P = 1;
-    const PartInfo pi{n, P};
     const int64_t w = 2 * k + 1;
     flag_.resize(1);
-
-    hipEvent_t e0, e1, e2, e3, e4, e5;
-    SAP_HIP_CHECK(hipEventCreate(&e0));
-    SAP_HIP_CHECK(hipEventCreate(&e1));
-    SAP_HIP_CHECK(hipEventCreate(&e2));
-    SAP_HIP_CHECK(hipEventCreate(&e3));
-    SAP_HIP_CHECK(hipEventCreate(&e4));
-    SAP_HIP_CHECK(hipEventCreate(&e5));
-    struct EvGuard {
-        hipEvent_t* e;
-        ~EvGuard() { for (int i = 0; i < 6; ++i) (void)hipEventDestroy(e[i]); }
-    };
-    hipEvent_t evs[6] = {e0, e1, e2, e3, e4, e5};
-    EvGuard guard{evs};
-
-    // (1) column-major band in the preconditioner precision
-    SAP_HIP_CHECK(hipEventRecord(e0, stream));
     // 8 elements of slack: the sweeps copy 16-byte chunks covering each
     // column segment, which may reach a few elements past the last column
     B_.resize((size_t)w * n + 8);
@@ -79,19 +90,139 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
     const bool need_ul = lu_ul() && cfg.precondType == SAP_Spike && P > 1 && k > 0;
     if (need_ul) Bul_.resize((size_t)w * n + 8);
     else Bul_.release();
+}
+
+template <class T>
+int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
+{
+    reorder_ = scale_ = false;
+    row_perm_.release(); col_reord_.release(); row_scale_.release(); col_scale_.release();
+    vp_.release(); zp_.release();
+    k_db = k_reorder = 0;
+    dropoff_actual = 0;
+    profile = BandProfile{};
+    begin_setup(n_, k_);
+    Events ev;
+    SAP_HIP_CHECK(hipEventRecord(ev.e[0], stream));
     {
         ProfScope ps(prof, "band_to_colmajor");
         launch_band_to_colmajor<T>(n, k, d_band, B_.get(), stream);
     }
-    SAP_HIP_CHECK(hipEventRecord(e1, stream));
-    time_toBanded = elapsed_ms(e0, e1);
+    SAP_HIP_CHECK(hipEventRecord(ev.e[1], stream));
+    time_toBanded = elapsed_ms(ev.e[0], ev.e[1]);
+    return factor_band();
+}
+
+template <class T>
+int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d_ci, const double* d_val,
+                          const ReorderConfig& rc)
+{
+    Events ev;
+    if (!rc.reorder) {
+        reorder_ = scale_ = false;
+        row_perm_.release(); col_reord_.release(); row_scale_.release(); col_scale_.release();
+        vp_.release(); zp_.release();
+        k_db = k_reorder = 0;
+        dropoff_actual = 0;
+        profile = BandProfile{};
+        // half-bandwidth over every stored entry, explicit zeros included
+        flag_.resize(1);
+        SAP_HIP_CHECK(hipMemsetAsync(flag_.get(), 0, sizeof(int), stream));
+        launch_csr_bandwidth(n_, d_rp, d_ci, flag_.get(), stream);
+        int kk = 0;
+        SAP_HIP_CHECK(hipMemcpyAsync(&kk, flag_.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
+        SAP_HIP_CHECK(hipStreamSynchronize(stream));
+        begin_setup(n_, kk);
+        SAP_HIP_CHECK(hipEventRecord(ev.e[0], stream));
+        {
+            ProfScope ps(prof, "csr_to_band");
+            SAP_HIP_CHECK(hipMemsetAsync(B_.get(), 0, B_.size() * sizeof(T), stream));
+            launch_csr_to_band<double, T>(n, k, d_rp, d_ci, d_val, B_.get(), stream);
+        }
+        SAP_HIP_CHECK(hipEventRecord(ev.e[1], stream));
+        time_toBanded = elapsed_ms(ev.e[0], ev.e[1]);
+        return factor_band();
+    }
+
+    // (1) matrix to the host (transformToBandedMatrix, sap/precond.h:1656-1713)
+    double t = wall_now_ms();
+    std::vector<int> h_rp(n_ + 1), h_ci(nnz);
+    std::vector<double> h_val(nnz);
+    SAP_HIP_CHECK(hipMemcpyAsync(h_rp.data(), d_rp, (n_ + 1) * sizeof(int), hipMemcpyDeviceToHost, stream));
+    if (nnz > 0) {
+        SAP_HIP_CHECK(hipMemcpyAsync(h_ci.data(), d_ci, nnz * sizeof(int), hipMemcpyDeviceToHost, stream));
+        SAP_HIP_CHECK(hipMemcpyAsync(h_val.data(), d_val, nnz * sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+    SAP_HIP_CHECK(hipStreamSynchronize(stream));
+    if (h_rp[0] != 0 || h_rp[n_] != nnz) throw Error(SAP_ERR_BADARG, "CSR row offsets do not span nnz entries");
+    time_transfer = wall_now_ms() - t;
+
+    // (2) DB, RCM (when maxBandwidth > 64, sap/precond.h:1737-1738), drop-off
+    GraphOptions go;
+    go.doDB = rc.doDB;
+    go.scale = rc.scale;
+    go.dbFirstStageOnly = rc.dbFirstStageOnly;
+    go.doRCM = rc.maxBandwidth > 64;
+    go.maxBandwidth = rc.maxBandwidth;
+    go.dropOffFrac = rc.dropOffFraction;
+    Reordering<T> R = reorder_csr<T>((int)n_, h_rp.data(), h_ci.data(), h_val.data(), go);
+    time_DB = R.time_DB;
+    time_reorder += R.time_reorder;
+    time_dropOff = R.time_dropOff;
+    k_db = R.k_db;
+    k_reorder = R.k_reorder;
+    dropoff_actual = R.dropoff_actual;
+
+    // (3) band of the reordered matrix, vectors of the permutations
+    begin_setup(n_, R.k);
+    t = wall_now_ms();
+    profile = band_profile((int)n, R.rp.data(), R.ci.data(), P);
+    time_cpu_assemble = wall_now_ms() - t;
+    t = wall_now_ms();
+    DevBuf<int> rp_d, ci_d;
+    DevBuf<T> val_d;
+    upload(rp_d, R.rp, stream);
+    upload(ci_d, R.ci, stream);
+    upload(val_d, R.val, stream);
+    upload(row_perm_, R.row_perm, stream);
+    upload(col_reord_, R.col_reord, stream);
+    upload(row_scale_, R.row_scale, stream);
+    upload(col_scale_, R.col_scale, stream);
+    vp_.resize(n);
+    zp_.resize(n);
+    SAP_HIP_CHECK(hipStreamSynchronize(stream));
+    time_transfer += wall_now_ms() - t;
+    reorder_ = true;
+    scale_ = rc.scale;
+    SAP_HIP_CHECK(hipEventRecord(ev.e[0], stream));
+    {
+        ProfScope ps(prof, "csr_to_band");
+        SAP_HIP_CHECK(hipMemsetAsync(B_.get(), 0, B_.size() * sizeof(T), stream));
+        launch_csr_to_band<T, T>(n, k, rp_d.get(), ci_d.get(), val_d.get(), B_.get(), stream);
+    }
+    SAP_HIP_CHECK(hipEventRecord(ev.e[1], stream));
+    time_toBanded = elapsed_ms(ev.e[0], ev.e[1]);   // waits for the fill before the uploads are freed
+    return factor_band();
+}
+
+template <class T>
+int Precond<T>::factor_band()
+{
+    const PartInfo pi{n, P};
+    const int64_t w = 2 * k + 1;
+    Events ev;
+    hipEvent_t e1 = ev.e[1], e2 = ev.e[2], e3 = ev.e[3], e4 = ev.e[4], e5 = ev.e[5];
 
     offB_.release(); offC_.release(); WV_.release(); S_.release(); Sinv_.release(); corr_.release();
     g_.resize(n);
     scr0_.resize(n);
     scr1_.release();
 
-    if (k == 0 || cfg.precondType == SAP_None) return 0;
+    if (k == 0 || cfg.precondType == SAP_None) {
+        if (cfg.precondType != SAP_None)
+            actual_nnz = n - count_band_zeros<T>(B_.get(), n, flag_.get(), stream);
+        return 0;
+    }
 
     const int boost = (k >= 70) || cfg.safeFactorization;   // sap/precond.h:4363-4397
     const int boost_last = (k >= 70);
@@ -196,11 +327,29 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
 template <class T>
 void Precond<T>::apply(const double* v, double* z)
 {
-    const PartInfo pi{n, P};
     if (cfg.precondType == SAP_None) {
         launch_copy(n, v, z, stream);
         return;
     }
+    if (!reorder_) {
+        solve_band(v, z);
+        return;
+    }
+    // leftTrans (sap/precond.h:1535-1544), getSRev, rightTrans (:1551-1559)
+    {
+        ProfScope ps(prof, "permute");
+        launch_permute_scale<T>(n, v, row_perm_.get(), scale_ ? row_scale_.get() : nullptr, false, vp_.get(), stream);
+    }
+    solve_band(vp_.get(), zp_.get());
+    ProfScope ps(prof, "permute");
+    launch_permute_scale<T>(n, zp_.get(), col_reord_.get(), scale_ ? col_scale_.get() : nullptr, true, z, stream);
+}
+
+// getSRev (sap/precond.h:1411-1528) for the constant-bandwidth band
+template <class T>
+void Precond<T>::solve_band(const double* v, double* z)
+{
+    const PartInfo pi{n, P};
     if (k == 0) {
         ProfScope ps(prof, "diag_divide");
         launch_diag_divide<T>(B_.get(), n, v, z, stream);

@@ -1,7 +1,8 @@
 // SaP preconditioner (the MI355X counterpart of sap::Precond,
-// sap/precond.h:53-510) for the constant-bandwidth banded path.
+// sap/precond.h:53-510), constant-bandwidth band.
 #pragma once
 
+#include "graph.hpp"
 #include "sap_internal.hpp"
 
 namespace sap_amd {
@@ -11,6 +12,17 @@ struct PrecondConfig {
     int factMethod = SAP_LU_only;
     int precondType = SAP_Spike;
     int safeFactorization = 0;
+};
+
+// Reordering options of a CSR setup, as sap::Solver forwards them to
+// sap::Precond (sap/solver.h:305-307).
+struct ReorderConfig {
+    bool reorder = true;           // performReorder
+    bool doDB = true;              // performDB
+    bool dbFirstStageOnly = false;
+    bool scale = true;             // applyScaling
+    int maxBandwidth = 0x7fffffff;
+    double dropOffFraction = 0;
 };
 
 template <class T>
@@ -27,13 +39,29 @@ public:
     // timings of the last setup, milliseconds (sap::Stats fields)
     double time_toBanded = 0, time_offDiags = 0, time_bandLU = 0, time_bandUL = 0;
     double time_assembly = 0, time_fullLU = 0;
+    // host reordering of CSR setups; time_reorder accumulates over setups
+    // as the reference's m_time_reorder does (sap/precond.h:1752)
+    double time_DB = 0, time_reorder = 0, time_dropOff = 0, time_transfer = 0, time_cpu_assemble = 0;
+    int k_db = 0, k_reorder = 0;    // stay 0 without reordering, as in the reference
+    double dropoff_actual = 0;
+    BandProfile profile;            // ks_row / ks_col sums (empty without reordering)
     int64_t actual_nnz = 0;
 
     // Factors the row-major FP64 band d_band (n x (2k+1)).  Returns 0 or a
     // negative sap_c.h code (SAP_ERR_ZERO_PIVOT).
     int setup(int64_t n_, int k_, const double* d_band);
-    // z = M^{-1} v, FP64 in and out, preconditioner precision inside.
+    // Factors the FP64 CSR matrix (device arrays): with rc.reorder the host
+    // DB / RCM / drop-off pipeline of transformToBandedMatrix
+    // (sap/precond.h:1656-2101), else the plain conversion of
+    // convertToBandedMatrix(CSR) (sap/precond.h:2240-2363).  Throws
+    // Error(SAP_ERR_SINGULAR / SAP_ERR_NEG_DB_WEIGHT) from DB.
+    int setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d_ci, const double* d_val,
+                  const ReorderConfig& rc);
+    // z = M^{-1} v, FP64 in and out, preconditioner precision inside
+    // (solve, sap/precond.h:1393-1406: left permutation and scaling, band
+    // solve, right permutation and scaling when the setup reordered).
     void apply(const double* v, double* z);
+    bool reordered() const { return reorder_; }
 
     bool spike_active() const { return cfg.precondType == SAP_Spike && P > 1 && k > 0; }
     bool lu_ul() const { return cfg.factMethod == SAP_LU_UL; }
@@ -48,6 +76,14 @@ private:
     DevBuf<T> B_, Bul_, offB_, offC_, WV_, S_, Sinv_, g_, corr_, scr0_, scr1_;
     DevBuf<FactorJob> jobs_;
     DevBuf<int> flag_;
+    // reordering: row permutation (DB then RCM), column reordering, scalings
+    bool reorder_ = false, scale_ = false;
+    DevBuf<int> row_perm_, col_reord_;
+    DevBuf<T> row_scale_, col_scale_;
+    DevBuf<double> vp_, zp_;
+    void begin_setup(int64_t n_, int k_);
+    int factor_band();
+    void solve_band(const double* v, double* z);
     void factor(const std::vector<FactorJob>& jobs);
     bool zero_pivots(const T* band, int64_t row0, int64_t rows);
     double elapsed_ms(hipEvent_t a, hipEvent_t b);

@@ -133,6 +133,16 @@ void launch_gen_vector(int64_t n, uint64_t seed, double* x, hipStream_t st);
 void launch_band_spmv(int64_t n, int k, const double* band, const double* x, double* y, hipStream_t st);
 template <class T>
 void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st);
+template <class Tin, class T>   // B zero-filled beforehand
+void launch_csr_to_band(int64_t n, int k, const int* rp, const int* ci, const Tin* vals, T* B, hipStream_t st);
+// atomic max of max|row - col| into *out (zeroed beforehand)
+void launch_csr_bandwidth(int64_t n, const int* rp, const int* ci, int* out, hipStream_t st);
+void launch_csr_spmv(int64_t n, int64_t nnz, const int* rp, const int* ci, const double* val, const double* x,
+                     double* y, hipStream_t st);
+// w[perm[i]] = (T)v[i] * scale[scale_at_target ? perm[i] : i]  (scale may be null)
+template <class T>
+void launch_permute_scale(int64_t n, const double* v, const int* perm, const T* scale, bool scale_at_target,
+                          double* w, hipStream_t st);
 
 // blas.hip (FP64 vectors, results of reductions left in device memory)
 void launch_dot(int64_t n, const double* a, const double* b, double* d_partials, double* d_out, hipStream_t st);

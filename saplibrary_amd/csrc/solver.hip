@@ -3,6 +3,7 @@
 // the extern "C" entry points declared in include/sap_c.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -168,6 +169,12 @@ public:
     int64_t n = 0;
     int k = 0;
     const double* d_band = nullptr;     // non-owning (the MVBanded reference)
+    // non-owning CSR of a CSR setup (the SpmvCusp functor keeps a reference
+    // to the matrix, sap/spmv.h:83-129)
+    const int* d_rp = nullptr;
+    const int* d_ci = nullptr;
+    const double* d_val = nullptr;
+    int64_t nnz = 0;
     sap_spmv_fn user_spmv = nullptr;
     void* user_ctx = nullptr;
     BiCGStabLMonitor bmon;
@@ -254,6 +261,7 @@ public:
         stats.numSpmv++;
         ProfScope ps(&prof, "spmv");
         if (user_spmv) user_spmv(user_ctx, x, y, (int)n);
+        else if (d_rp) launch_csr_spmv(n, nnz, d_rp, d_ci, d_val, x, y, stream);
         else launch_band_spmv(n, k, d_band, x, y, stream);
     }
     void prec(const double* v, double* z)
@@ -529,6 +537,38 @@ public:
         return bmon.code;
     }
 
+    // Solver::setup statistics (sap/solver.h:354-399)
+    template <class PC>
+    void harvest(const PC& p, double t0)
+    {
+        stats.timeSetup = wall_ms() - t0;
+        stats.time_DB = p.time_DB;
+        stats.time_reorder = p.time_reorder;
+        stats.time_dropOff = p.time_dropOff;
+        stats.time_cpu_assemble = p.time_cpu_assemble;
+        stats.time_transfer = p.time_transfer;
+        stats.time_toBanded = p.time_toBanded;
+        stats.time_offDiags = p.time_offDiags;
+        stats.time_bandLU = p.time_bandLU;
+        stats.time_bandUL = p.time_bandUL;
+        stats.time_assembly = p.time_assembly;
+        stats.time_fullLU = p.time_fullLU;
+        stats.numPartitions = p.P;
+        stats.actual_nnz = (int)std::min<int64_t>(p.actual_nnz, INT_MAX);
+        stats.bandwidth = p.k;
+        stats.bandwidthReorder = p.k_reorder;
+        stats.bandwidthDB = p.k_db;
+        stats.actualDropOff = p.dropoff_actual;
+        // nuKf = (2 K n - sum ks_row - sum ks_col) / (2 K n) and flops_LU =
+        // sum ks_row ks_col / (time_bandLU 1e6); the extents are only
+        // assembled by the reordering path (empty otherwise, so nuKf = 1 and
+        // flops_LU = 0 there, as in the reference)
+        stats.nuKf = p.k == 0 ? 0.0
+                              : (2.0 * p.k * (double)n - (p.profile.sum_row + p.profile.sum_col)) /
+                                    (2.0 * p.k * (double)n);
+        stats.flops_LU = stats.time_bandLU == 0 ? 0.0 : p.profile.sum_prod / (stats.time_bandLU * 1e6);
+    }
+
     int setup_banded(int64_t n_, int k_, const double* band)
     {
         const double t0 = wall_ms();
@@ -536,29 +576,39 @@ public:
         n = n_;
         k = k_;
         d_band = band;
+        d_rp = d_ci = nullptr;
+        d_val = nullptr;
+        nnz = 0;
         int rc = pf ? pf->setup(n, k, band) : pd->setup(n, k, band);
         SAP_HIP_CHECK(hipStreamSynchronize(stream));
         if (rc != 0) return rc;
-        stats.timeSetup = wall_ms() - t0;
-        auto fill = [&](auto& p) {
-            stats.time_toBanded = p.time_toBanded;
-            stats.time_offDiags = p.time_offDiags;
-            stats.time_bandLU = p.time_bandLU;
-            stats.time_bandUL = p.time_bandUL;
-            stats.time_assembly = p.time_assembly;
-            stats.time_fullLU = p.time_fullLU;
-            stats.numPartitions = p.P;
-            stats.actual_nnz = (int)std::min<int64_t>(p.actual_nnz, INT_MAX);
-        };
-        if (pf) fill(*pf); else fill(*pd);
-        stats.bandwidth = k;
-        stats.bandwidthReorder = k;
-        stats.bandwidthDB = k;
-        stats.actualDropOff = 0;
-        // Uniform band: ks_row = ks_col = k, so nuKf = 0 (sap/solver.h:360, :391-394)
-        stats.nuKf = 0.0;
-        // flops_LU = sum_i ks_row[i]*ks_col[i] / (time_bandLU * 1e6) (sap/solver.h:361-366, :396-399)
-        stats.flops_LU = stats.time_bandLU > 0 ? (double)k * k * n / (stats.time_bandLU * 1e6) : 0.0;
+        if (pf) harvest(*pf, t0); else harvest(*pd, t0);
+        setup_done = true;
+        return 0;
+    }
+
+    int setup_csr(int64_t n_, int64_t nnz_, const int* rp, const int* ci, const double* val)
+    {
+        const double t0 = wall_ms();
+        setup_done = false;
+        n = n_;
+        d_band = nullptr;
+        d_rp = rp;
+        d_ci = ci;
+        d_val = val;
+        nnz = nnz_;
+        ReorderConfig rc;
+        rc.reorder = opt.performReorder != 0;
+        rc.doDB = opt.performDB != 0;
+        rc.dbFirstStageOnly = opt.dbFirstStageOnly != 0;
+        rc.scale = opt.applyScaling != 0;
+        rc.maxBandwidth = opt.maxBandwidth;
+        rc.dropOffFraction = opt.dropOffFraction;
+        int r = pf ? pf->setup_csr(n, nnz, rp, ci, val, rc) : pd->setup_csr(n, nnz, rp, ci, val, rc);
+        SAP_HIP_CHECK(hipStreamSynchronize(stream));
+        if (r != 0) return r;
+        k = pf ? pf->k : pd->k;
+        if (pf) harvest(*pf, t0); else harvest(*pd, t0);
         setup_done = true;
         return 0;
     }
@@ -746,6 +796,23 @@ int sap_setup_banded(sap_solver* h, int n, int k, const double* d_band)
     return guarded(h, [&] { return h->s->setup_banded(n, k, d_band); });
 }
 
+int sap_setup_csr(sap_solver* h, int n, int nnz, const int* d_row_offsets, const int* d_col_indices,
+                  const double* d_values)
+{
+    if (!h) return SAP_ERR_BADARG;
+    if (n <= 0 || nnz < 0 || !d_row_offsets || (nnz > 0 && (!d_col_indices || !d_values))) {
+        h->err = "bad matrix arguments";
+        return SAP_ERR_BADARG;
+    }
+    const sap_options& o = h->s->opt;
+    if (o.testDB) { h->err = "testDB is not supported"; return SAP_ERR_UNSUPPORTED; }
+    if (o.performReorder && o.variableBandwidth) {
+        h->err = "the variable-bandwidth preconditioner is not implemented; set variableBandwidth = 0";
+        return SAP_ERR_UNSUPPORTED;
+    }
+    return guarded(h, [&] { return h->s->setup_csr(n, nnz, d_row_offsets, d_col_indices, d_values); });
+}
+
 int sap_solve(sap_solver* h, sap_spmv_fn spmv, void* ctx, const double* d_b, double* d_x, int nrhs)
 {
     if (!h) return SAP_ERR_BADARG;
@@ -788,6 +855,17 @@ int sap_band_spmv(int n, int k, const double* d_band, const double* d_x, double*
     if (n <= 0 || k < 0 || !d_band || !d_x || !d_y) return SAP_ERR_BADARG;
     return guarded(nullptr, [&] {
         sap_amd::launch_band_spmv(n, k, d_band, d_x, d_y, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+int sap_csr_spmv(int n, int nnz, const int* d_row_offsets, const int* d_col_indices, const double* d_values,
+                 const double* d_x, double* d_y, void* stream)
+{
+    if (n <= 0 || nnz < 0 || !d_row_offsets || !d_x || !d_y || (nnz > 0 && (!d_col_indices || !d_values)))
+        return SAP_ERR_BADARG;
+    return guarded(nullptr, [&] {
+        sap_amd::launch_csr_spmv(n, nnz, d_row_offsets, d_col_indices, d_values, d_x, d_y, (hipStream_t)stream);
         return 0;
     });
 }
@@ -884,5 +962,87 @@ int sap_gen_vector(int64_t n, uint64_t seed, double* d_x, void* stream)
         return 0;
     });
 }
+
+struct sap_reordering {
+    int precision = SAP_PREC_FLOAT;
+    sap_amd::Reordering<float> f;
+    sap_amd::Reordering<double> d;
+};
+
+sap_reordering* sap_reorder_csr(int n, const int* h_row_offsets, const int* h_col_indices, const double* h_values,
+                                const sap_options* opts, int* err)
+{
+    auto fail = [&](int c) {
+        if (err) *err = c;
+        return (sap_reordering*)nullptr;
+    };
+    if (n <= 0 || !h_row_offsets || (h_row_offsets[n] > 0 && (!h_col_indices || !h_values)))
+        return fail(SAP_ERR_BADARG);
+    sap_options o;
+    if (opts) o = *opts;
+    else sap_default_options(&o);
+    if (o.precValueType != SAP_PREC_FLOAT && o.precValueType != SAP_PREC_DOUBLE) return fail(SAP_ERR_BADARG);
+    sap_amd::GraphOptions go;
+    go.doDB = o.performDB != 0;
+    go.scale = o.applyScaling != 0;
+    go.dbFirstStageOnly = o.dbFirstStageOnly != 0;
+    go.doRCM = o.maxBandwidth > 64;
+    go.maxBandwidth = o.maxBandwidth;
+    go.dropOffFrac = o.dropOffFraction;
+    auto* r = new (std::nothrow) sap_reordering();
+    if (!r) return fail(SAP_ERR_OOM);
+    r->precision = o.precValueType;
+    const int rc = guarded(nullptr, [&] {
+        if (r->precision == SAP_PREC_FLOAT)
+            r->f = sap_amd::reorder_csr<float>(n, h_row_offsets, h_col_indices, h_values, go);
+        else
+            r->d = sap_amd::reorder_csr<double>(n, h_row_offsets, h_col_indices, h_values, go);
+        return 0;
+    });
+    if (rc != 0) {
+        delete r;
+        return fail(rc);
+    }
+    if (err) *err = SAP_OK;
+    return r;
+}
+
+int sap_reordering_info(const sap_reordering* r, int* k, int* k_db, int* k_reorder, double* drop_off, int* nnz)
+{
+    if (!r) return SAP_ERR_BADARG;
+    auto get = [&](const auto& R) {
+        if (k) *k = R.k;
+        if (k_db) *k_db = R.k_db;
+        if (k_reorder) *k_reorder = R.k_reorder;
+        if (drop_off) *drop_off = R.dropoff_actual;
+        if (nnz) *nnz = (int)R.ci.size();
+    };
+    if (r->precision == SAP_PREC_FLOAT) get(r->f); else get(r->d);
+    return SAP_OK;
+}
+
+int sap_reordering_get(const sap_reordering* r, int* db_perm, int* row_perm, int* col_reord, double* row_scale,
+                       double* col_scale, int* row_offsets, int* col_indices, double* values)
+{
+    if (!r) return SAP_ERR_BADARG;
+    auto get = [&](const auto& R) {
+        const size_t n = (size_t)R.n, m = R.ci.size();
+        if (db_perm) std::copy(R.db_perm.begin(), R.db_perm.end(), db_perm);
+        if (row_perm) std::copy(R.row_perm.begin(), R.row_perm.end(), row_perm);
+        if (col_reord) std::copy(R.col_reord.begin(), R.col_reord.end(), col_reord);
+        for (size_t i = 0; i < n; ++i) {
+            if (row_scale) row_scale[i] = (double)R.row_scale[i];
+            if (col_scale) col_scale[i] = (double)R.col_scale[i];
+        }
+        if (row_offsets) std::copy(R.rp.begin(), R.rp.end(), row_offsets);
+        if (col_indices) std::copy(R.ci.begin(), R.ci.end(), col_indices);
+        if (values)
+            for (size_t e = 0; e < m; ++e) values[e] = (double)R.val[e];
+    };
+    if (r->precision == SAP_PREC_FLOAT) get(r->f); else get(r->d);
+    return SAP_OK;
+}
+
+void sap_reordering_free(sap_reordering* r) { delete r; }
 
 }  // extern "C"

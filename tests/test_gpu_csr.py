@@ -1,0 +1,191 @@
+"""Parity of the CSR path (through the C ABI) with the oracle.
+
+  * CSR SpMV: FP64, within 1e-13 of the largest |y| (scipy's product, a
+    different summation order).
+  * CSR setup without reordering: the band it builds is the band of the
+    matrix, so factors and M^{-1} v are bit-identical to the banded setup.
+  * CSR setup with DB / RCM / drop-off: the statistics (bandwidths, drop-off
+    fraction, partitions) equal the oracle's, and M^{-1} v equals the
+    oracle's chain (its reordering, its banded preconditioner on the
+    reordered band, the permutations and scalings in the preconditioner
+    type) within 1e-10 of the largest |z| in FP64; in FP32 within 3e-5 of
+    it or 4x the oracle's own FP32 error, whichever is larger (these
+    matrices are far from diagonally dominant, so FP32 rounding is amplified
+    by the conditioning of the diagonal blocks).
+  * End to end: FP64 residual of the solution on a matrix of the sparse
+    workload's kind (random band of width 300, ~12 entries per row, weak
+    diagonals, symmetric random permutation).
+"""
+import numpy as np
+import pytest
+
+from oracle import graph_oracle as G
+from tests.csr_cases import sparse_band
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def sap():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import saplibrary_amd
+    saplibrary_amd.lib()
+    return saplibrary_amd
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def csr_of_band(band):
+    n, w = band.shape
+    k = (w - 1) // 2
+    rp, ci, val = [0], [], []
+    for i in range(n):
+        for j in range(max(0, i - k), min(n, i + k + 1)):
+            ci.append(j)
+            val.append(band[i, k + j - i])
+        rp.append(len(ci))
+    return np.array(rp, dtype=np.int32), np.array(ci, dtype=np.int32), np.array(val)
+
+
+@pytest.mark.parametrize("n,per_row", [(5000, 3), (4000, 12), (3000, 40), (7, 2)])
+def test_csr_spmv(sap, n, per_row):
+    import scipy.sparse as sp
+    rp, ci, val = sparse_band(n, max(4, 3 * per_row), per_row, 1)
+    A = sap.CsrMatrix.from_host(rp, ci, val)
+    x = np.random.default_rng(2).uniform(-1, 1, n)
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    sap.csr_spmv(A, dev(x), y)
+    ref = sp.csr_matrix((val, ci, rp), shape=(n, n)) @ x
+    np.testing.assert_allclose(host(y), ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n,k,P,fact", [(3000, 20, 4, "LU_UL"), (4000, 50, 5, "LU_only"), (500, 3, 1, "LU_UL")])
+def test_csr_without_reordering_is_the_banded_setup(sap, oracle, n, k, P, fact):
+    band = oracle.gen_banded(n, k, 0.9, 41)
+    rp, ci, val = csr_of_band(band)
+    f = sap.LU_UL if fact == "LU_UL" else sap.LU_only
+    opts = sap.Options(factMethod=f, performReorder=0, performDB=0, applyScaling=0, variableBandwidth=0)
+    g_csr = sap.Solver(P, opts)
+    g_csr.setup(sap.CsrMatrix.from_host(rp, ci, val))
+    g_band = sap.Solver(P, sap.Options(factMethod=f))
+    g_band.setup(sap.BandedMatrix(dev(band)))
+    st = g_csr.getStats()
+    assert st.bandwidth == k and st.bandwidthDB == 0 and st.bandwidthReorder == 0
+    assert g_csr.numPartitions == g_band.numPartitions
+    np.testing.assert_array_equal(g_csr.factors(), g_band.factors())
+    v = dev(oracle.gen_vector(n, 42))
+    z1 = torch.empty(n, dtype=torch.float64, device="cuda")
+    z2 = torch.empty_like(z1)
+    g_csr.precond_apply(v, z1)
+    g_band.precond_apply(v, z2)
+    np.testing.assert_array_equal(host(z1), host(z2))
+
+
+CASES = [
+    # n, width, per_row, seed, P, fact, reorder options
+    (1500, 60, 6, 51, 4, "LU_UL", dict()),
+    (1200, 40, 5, 52, 3, "LU_only", dict(drop_off=0.01)),
+    (1000, 50, 4, 53, 1, "LU_UL", dict(scale=False)),
+    (1400, 30, 5, 54, 5, "LU_UL", dict(do_db=False)),
+    (1300, 80, 6, 55, 2, "LU_UL", dict(max_bw=20)),
+]
+
+
+def _opts(sap, o, fact, prec):
+    return sap.Options(
+        factMethod=sap.LU_UL if fact == "LU_UL" else sap.LU_only, variableBandwidth=0,
+        performDB=int(o.get("do_db", True)), applyScaling=int(o.get("scale", True)),
+        maxBandwidth=int(o.get("max_bw", 2 ** 31 - 1)), dropOffFraction=float(o.get("drop_off", 0.0)),
+        precValueType=sap.PREC_FLOAT if prec == "float" else sap.PREC_DOUBLE)
+
+
+@pytest.mark.parametrize("prec", ["float", "double"])
+@pytest.mark.parametrize("case", CASES, ids=[f"n{c[0]}-{c[3]}" for c in CASES])
+def test_csr_reordered_apply_matches_oracle(sap, oracle, case, prec):
+    n, width, per_row, seed, P, fact, o = case
+    rp, ci, val = sparse_band(n, width, per_row, seed)
+    ref = G.reorder(rp, ci, val, prec=prec, **o)
+    T = np.float32 if prec == "float" else np.float64
+    k = ref["k"]
+    band = G.csr_to_band(n, k, ref["rp"], ref["ci"], ref["val"])
+    cpu = oracle.Solver(P, fact=fact, prec_double=(prec == "double"))
+    assert cpu.setup(band) == 0
+
+    g = sap.Solver(P, _opts(sap, o, fact, prec))
+    g.setup(sap.CsrMatrix.from_host(rp, ci, val))
+    st = g.getStats()
+    assert (st.bandwidth, st.bandwidthDB, st.bandwidthReorder) == (k, ref["k_db"], ref["k_reorder"])
+    assert st.actualDropOff == ref["dropoff"]
+    assert g.numPartitions == cpu.num_partitions
+
+    v = oracle.gen_vector(n, seed + 100)
+    scale = o.get("scale", True)
+
+    def chain(solver, T):
+        w = np.zeros(n)
+        x = v.astype(T) * (ref["row_scale"].astype(T) if scale else T(1))
+        w[ref["row_perm"]] = x.astype(np.float64)
+        zr = solver.apply(w)
+        out = np.zeros(n)
+        y = zr.astype(T) * (ref["col_scale"][ref["col_reord"]].astype(T) if scale else T(1))
+        out[ref["col_reord"]] = y.astype(np.float64)
+        return out
+
+    expect = chain(cpu, T)
+    z = torch.empty(n, dtype=torch.float64, device="cuda")
+    g.precond_apply(dev(v), z)
+    if prec == "double":
+        atol = 1e-10 * np.abs(expect).max()
+    else:
+        # FP32 rounding, amplified by the conditioning of the diagonal
+        # blocks: the GPU may differ from the oracle by 3e-5 of max |z|, or
+        # by 4x the oracle's own FP32 error (against the same preconditioner
+        # applied in FP64), whichever is larger
+        cpu64 = oracle.Solver(P, fact=fact, prec_double=True)
+        assert cpu64.setup(band) == 0
+        err32 = np.abs(expect - chain(cpu64, np.float64)).max()
+        atol = max(3e-5 * np.abs(expect).max(), 4 * err32)
+    np.testing.assert_allclose(host(z), expect, rtol=0, atol=atol)
+
+
+def test_csr_solve_sparse_workload(sap):
+    import scipy.sparse as sp
+    n = 30000
+    rp, ci, val = sparse_band(n, 300, 11, 61)
+    A = sp.csr_matrix((val, ci, rp), shape=(n, n))
+    xt = np.random.default_rng(62).uniform(-1, 1, n)
+    b = A @ xt
+    # (two partitions: this matrix is far from diagonally dominant and its
+    # reordered band is wide, so truncated SPIKE stops converging beyond
+    # P = 2 -- the oracle's SaP behaves the same on the reordered band)
+    opts = sap.Options(variableBandwidth=0, relTol=1e-10, refine=8, factMethod=sap.LU_only)
+    g = sap.Solver(2, opts)
+    g.setup(sap.CsrMatrix.from_host(rp, ci, val))
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert g.solve(None, dev(b), x), g.getMonitorMessage()
+    xs = host(x)
+    assert np.linalg.norm(b - A @ xs) / np.linalg.norm(b) <= 1e-10
+    st = g.getStats()
+    assert st.bandwidthDB > st.bandwidthReorder >= st.bandwidth > 0
+    assert st.numSpmv > 0 and g.numPartitions == 2
+
+
+def test_csr_options_rejected(sap):
+    rp, ci, val = sparse_band(200, 10, 3, 70)
+    A = sap.CsrMatrix.from_host(rp, ci, val)
+    with pytest.raises(sap.SapError) as ei:
+        sap.Solver(2, sap.Options()).setup(A)          # variableBandwidth = 1 by default
+    assert ei.value.reason == -9
+    with pytest.raises(sap.SapError) as ei:
+        sap.Solver(2, sap.Options(variableBandwidth=0, testDB=1)).setup(A)
+    assert ei.value.reason == -9
