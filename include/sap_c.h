@@ -16,6 +16,8 @@
  *   SpmvCusp::operator()(v, Av)            -> sap_csr_spmv     (sap/spmv.h:83-129)
  *   Graph::reorder + Graph::dropOff        -> sap_reorder_csr  (sap/graph.h:446-747,
  *     (host side of transformToBandedMatrix)                    sap/precond.h:1656-1776)
+ *   Solver::update(entries)                -> sap_update       (sap/solver.h:421-462,
+ *                                              sap/precond.h:751-900)
  *   Solver::solve(spmv, b, x)              -> sap_solve        (sap/solver.h:477-562)
  *   Solver::getStats()                     -> sap_get_stats    (sap/solver.h:182)
  *   Solver::getMonitorCode()               -> sap_monitor_code (sap/solver.h:183-188)
@@ -188,6 +190,17 @@ int         sap_setup_banded(sap_solver* s, int n, int k, const double* d_band);
  * variable-bandwidth preconditioner is not provided). */
 int         sap_setup_csr(sap_solver* s, int n, int nnz, const int* d_row_offsets,
                           const int* d_col_indices, const double* d_values);
+
+/* Solver::update: refactors with new values of the setup matrix, same
+ * pattern and same reordering (sap/solver.h:421-462).  d_entries holds count
+ * doubles in the layout of the last setup: the CSR values in their order
+ * (the reference's entries), or, for a banded setup, the row-major band
+ * (an extension: the reference tracks only CSR setups).  Needs
+ * trackReordering; SAP_ERR_ILLEGAL_UPDATE before a setup or without it.
+ * Returns 1 when done and 0 when count does not match the setup matrix
+ * (the reference's false).  The built-in SpMV then uses d_entries
+ * (non-owning, like the setup matrix). */
+int         sap_update(sap_solver* s, const double* d_entries, int64_t count);
 
 /* Solves A x = b.  d_x holds the initial guess on entry and the solution on
  * return.  spmv == NULL uses the built-in SpMV of the setup matrix (banded or

@@ -568,6 +568,7 @@ def reorder(rp, ci, val, do_db=True, scale=True, first_stage_only=False, max_bw=
     # (2) DB (Graph::reorder, sap/graph.h:494-520; Graph::DB :1214-1361)
     row_scale = [T(1)] * n
     col_scale = [T(1)] * n
+    rs = cs = [1.0] * n
     db_perm = list(range(n))
     if do_db:
         match, u, v, mx = _db(n, rp, ci, val, first_stage_only)
@@ -620,6 +621,9 @@ def reorder(rp, ci, val, do_db=True, scale=True, first_stage_only=False, max_bw=
         col_reord=np.array(col_reord, dtype=np.int32),
         row_scale=np.array(row_scale, dtype=T).astype(np.float64),
         col_scale=np.array(col_scale, dtype=T).astype(np.float64),
+        # the scalings before rounding to T (an entry's scale factor is
+        # T(row_scale_d[i] * col_scale_d[j]), sap/graph.h:1307)
+        row_scale_d=np.array(rs, dtype=np.float64), col_scale_d=np.array(cs, dtype=np.float64),
         rp=np.array(rp, dtype=np.int32), ci=np.array(ci, dtype=np.int32),
         val=np.array(val, dtype=T).astype(np.float64),
     )

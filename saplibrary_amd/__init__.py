@@ -133,7 +133,7 @@ EXPORTS = (
     "sap_get_stats", "sap_monitor_code", "sap_monitor_message", "sap_last_error", "sap_precond_apply",
     "sap_band_spmv", "sap_get_profile", "sap_reset_profile", "sap_get_factors", "sap_get_spikes",
     "sap_num_partitions", "sap_gen_banded", "sap_gen_vector", "sap_setup_csr", "sap_csr_spmv",
-    "sap_reorder_csr", "sap_reordering_info", "sap_reordering_get", "sap_reordering_free",
+    "sap_reorder_csr", "sap_reordering_info", "sap_reordering_get", "sap_reordering_free", "sap_update",
 )
 
 _lib = None
@@ -174,6 +174,7 @@ def lib():
     L.sap_gen_banded.argtypes = [i64, i, d, ctypes.c_uint64, vp, vp]
     L.sap_gen_vector.argtypes = [i64, ctypes.c_uint64, vp, vp]
     L.sap_setup_csr.argtypes = [vp, i, i, vp, vp, vp]
+    L.sap_update.argtypes = [vp, vp, i64]
     L.sap_csr_spmv.argtypes = [i, i, vp, vp, vp, vp, vp, vp]
     ip, dp = ctypes.POINTER(i), ctypes.POINTER(d)
     L.sap_reorder_csr.argtypes = [i, ip, ip, dp, ctypes.POINTER(Options), ip]
@@ -369,6 +370,22 @@ class Solver:
             raise TypeError("setup() takes a BandedMatrix or a CsrMatrix")
         self._A = A
         return True
+
+    def update(self, entries):
+        """Refactors with new values of the setup matrix (sap/solver.h:421-462):
+        a float64 device tensor of the CSR values (CsrMatrix setup) or of the
+        n x (2k+1) band (BandedMatrix setup).  Returns False when the number
+        of entries does not match; raises SapError (Illegal_update without
+        trackReordering or before setup).  The built-in SpMV then uses
+        ``entries``, which must stay alive."""
+        rc = lib().sap_update(self.h, _dev_f64(entries, "entries"), entries.numel())
+        _check(rc, self.h)
+        if rc == 1:
+            if isinstance(self._A, CsrMatrix):
+                self._A = CsrMatrix(self._A.row_offsets, self._A.column_indices, entries)
+            elif isinstance(self._A, BandedMatrix):
+                self._A = BandedMatrix(entries.view(self._A.band.shape))
+        return rc == 1
 
     def solve(self, spmv, b, x):
         """Solves A x = b (sap/solver.h:477-562); x holds the initial guess.

@@ -185,6 +185,26 @@ __global__ void csr_bandwidth_kernel(int64_t n, const int* __restrict__ rp, cons
 }
 
 // ---------------------------------------------------------------------------
+// New values of a tracked matrix into the zero-filled band: entry e goes to
+// band slot map[e] (-1: not in the band) times its DB scale factor, in the
+// band type (Precond::update, sap/precond.h:751-775: scatter_if of
+// entries * scaleMap through bandedMatMap).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void update_band_kernel(int64_t nnz, const double* __restrict__ entries,
+                                   const int64_t* __restrict__ map, const T* __restrict__ scale, T* __restrict__ B)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += stride) {
+        const int64_t slot = map[e];
+        if (slot < 0) continue;
+        T v = (T)entries[e];
+        if (scale) v = v * scale[e];
+        B[slot] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // y = A x for FP64 CSR (the SpmvCusp functor, sap/spmv.h:83-129): G lanes per
 // row, each summing its entries in order, then a fixed shuffle tree.
 // ---------------------------------------------------------------------------
@@ -283,6 +303,19 @@ template void launch_csr_to_band<double, double>(int64_t, int, const int*, const
                                                  hipStream_t);
 template void launch_csr_to_band<float, float>(int64_t, int, const int*, const int*, const float*, float*,
                                                hipStream_t);
+
+template <class T>
+void launch_update_band(int64_t nnz, const double* entries, const int64_t* map, const T* scale, T* B, hipStream_t st)
+{
+    if (nnz <= 0) return;
+    int64_t g = (nnz + 255) / 256;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(update_band_kernel<T>, dim3((unsigned)g), dim3(256), 0, st, nnz, entries, map, scale, B);
+    SAP_LAUNCH_CHECK();
+}
+template void launch_update_band<float>(int64_t, const double*, const int64_t*, const float*, float*, hipStream_t);
+template void launch_update_band<double>(int64_t, const double*, const int64_t*, const double*, double*,
+                                         hipStream_t);
 
 void launch_csr_bandwidth(int64_t n, const int* rp, const int* ci, int* out, hipStream_t st)
 {

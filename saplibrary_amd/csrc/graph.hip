@@ -39,24 +39,27 @@ struct KeyGreater {
 // Stable counting sort of entries by a new row index: entries of one row
 // keep their input order (the reference fills each row from its end while
 // walking the entries backwards, sap/graph.h:1334-1347).
+// `id` (optional) is carried along with the entries.
 template <class T>
 void regroup_rows(int n, const std::vector<int>& new_row, std::vector<int>& ci, std::vector<T>& val,
-                  std::vector<int>& rp)
+                  std::vector<int>& rp, std::vector<int>* id = nullptr)
 {
     const size_t nnz = new_row.size();
     rp.assign(n + 1, 0);
     for (size_t i = 0; i < nnz; ++i) rp[new_row[i] + 1]++;
     for (int i = 0; i < n; ++i) rp[i + 1] += rp[i];
     std::vector<int> pos(rp.begin(), rp.end() - 1);
-    std::vector<int> ci2(nnz);
+    std::vector<int> ci2(nnz), id2(id ? nnz : 0);
     std::vector<T> v2(nnz);
     for (size_t i = 0; i < nnz; ++i) {
         const int d = pos[new_row[i]]++;
         ci2[d] = ci[i];
         v2[d] = val[i];
+        if (id) id2[d] = (*id)[i];
     }
     ci.swap(ci2);
     val.swap(v2);
+    if (id) id->swap(id2);
 }
 
 // ------------------------------------------------------------------- DB
@@ -459,16 +462,18 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
     R.n = n;
     // (1) values in the preconditioner type, explicit zeros removed
     //     (transformToBandedMatrix, sap/precond.h:1690-1710)
-    std::vector<int> rp(n + 1, 0), ci;
+    std::vector<int> rp(n + 1, 0), ci, id;   // id: entry index in the input arrays
     std::vector<T> val;
     ci.reserve(rp_in[n]);
     val.reserve(rp_in[n]);
+    id.reserve(rp_in[n]);
     for (int i = 0; i < n; ++i) {
         for (int j = rp_in[i]; j < rp_in[i + 1]; ++j) {
             const T v = (T)val_in[j];
             if (v != (T)0) {
                 ci.push_back(ci_in[j]);
                 val.push_back(v);
+                id.push_back(j);
             }
         }
         rp[i + 1] = (int)ci.size();
@@ -522,7 +527,13 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
             R.row_scale[i] = (T)rs[i];
             R.col_scale[i] = (T)cs[i];
         }
-        regroup_rows(n, new_row, ci, val, rp);
+        if (scale) {
+            // scale factor of every input entry, for value updates
+            R.entry_scale.assign(rp_in[n], (T)1);
+            for (int i = 0; i < n; ++i)
+                for (int l = rp[i]; l < rp[i + 1]; ++l) R.entry_scale[id[l]] = (T)(rs[i] * cs[ci[l]]);
+        }
+        regroup_rows(n, new_row, ci, val, rp, &id);
         R.time_DB = since(t_db);
     } else {
         for (int i = 0; i < n; ++i) R.db_perm[i] = i;
@@ -546,7 +557,7 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
                 new_row[j] = rcm_perm[i];
                 ci[j] = rcm_perm[ci[j]];
             }
-        regroup_rows(n, new_row, ci, val, rp);
+        regroup_rows(n, new_row, ci, val, rp, &id);
     } else {
         R.k_reorder = R.k_db;
         R.col_reord.resize(n);
@@ -570,7 +581,7 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
             int run = 0;
             for (int d = 0; d < n; ++d) { const int c = bucket[d]; bucket[d] = run; run += c; }
         }
-        std::vector<int> cr(nnz), cc(nnz);
+        std::vector<int> cr(nnz), cc(nnz), cid(nnz);
         std::vector<T> cv(nnz);
         for (int i = 0; i < n; ++i)
             for (int l = rp[i]; l < rp[i + 1]; ++l) {
@@ -578,6 +589,7 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
                 cr[d] = i;
                 cc[d] = ci[l];
                 cv[d] = val[l];
+                cid[d] = id[l];
             }
         T norm_in = (T)0;
         for (size_t l = 0; l < nnz; ++l) norm_in += val[l] * val[l];
@@ -620,13 +632,14 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
         std::vector<int> cb(n + 1, 0);
         for (size_t e = 0; e < keep; ++e) cb[cc[e]]++;
         { int run = 0; for (int d = 0; d < n; ++d) { const int c = cb[d]; cb[d] = run; run += c; } }
-        std::vector<int> r1(keep), c1(keep);
+        std::vector<int> r1(keep), c1(keep), i1(keep);
         std::vector<T> v1(keep);
         for (size_t e = 0; e < keep; ++e) {
             const int d = cb[cc[e]]++;
             r1[d] = cr[e];
             c1[d] = cc[e];
             v1[d] = cv[e];
+            i1[d] = cid[e];
         }
         rp.assign(n + 1, 0);
         for (size_t e = 0; e < keep; ++e) rp[r1[e] + 1]++;
@@ -634,10 +647,12 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
         std::vector<int> end(rp.begin() + 1, rp.end());
         ci.assign(keep, 0);
         val.assign(keep, (T)0);
+        id.assign(keep, 0);
         for (size_t e = 0; e < keep; ++e) {
             const int d = --end[r1[e]];
             ci[d] = c1[e];
             val[d] = v1[e];
+            id[d] = i1[e];
         }
         R.time_dropOff = since(t_drop);
     }
@@ -649,6 +664,7 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
     R.rp.swap(rp);
     R.ci.swap(ci);
     R.val.swap(val);
+    R.entry_orig.swap(id);
     return R;
 }
 

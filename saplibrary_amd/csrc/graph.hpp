@@ -31,6 +31,11 @@ struct Reordering {
     // reordered (and dropped-off) matrix, CSR, values in T
     std::vector<int> rp, ci;
     std::vector<T> val;
+    // for value updates (the reference's tracked maps, sap/graph.h:1216-1361):
+    // entry_orig[e] = index in the input arrays of reordered entry e;
+    // entry_scale[j] = DB scale factor of input entry j (empty: all 1)
+    std::vector<int> entry_orig;
+    std::vector<T> entry_scale;
     // host wall times, milliseconds
     double time_DB = 0, time_reorder = 0, time_dropOff = 0;
 };

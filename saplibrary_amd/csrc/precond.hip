@@ -102,6 +102,10 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
     dropoff_actual = 0;
     profile = BandProfile{};
     begin_setup(n_, k_);
+    src_ = Source::band;
+    src_count_ = n_ * (2 * (int64_t)k_ + 1);
+    upd_map_.release();
+    upd_scale_.release();
     Events ev;
     SAP_HIP_CHECK(hipEventRecord(ev.e[0], stream));
     {
@@ -133,6 +137,12 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
         SAP_HIP_CHECK(hipMemcpyAsync(&kk, flag_.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
         SAP_HIP_CHECK(hipStreamSynchronize(stream));
         begin_setup(n_, kk);
+        src_ = Source::csr;
+        src_count_ = nnz;
+        src_rp_ = d_rp;
+        src_ci_ = d_ci;
+        upd_map_.release();
+        upd_scale_.release();
         SAP_HIP_CHECK(hipEventRecord(ev.e[0], stream));
         {
             ProfScope ps(prof, "csr_to_band");
@@ -188,6 +198,26 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
     upload(col_reord_, R.col_reord, stream);
     upload(row_scale_, R.row_scale, stream);
     upload(col_scale_, R.col_scale, stream);
+    src_ = Source::csr_reordered;
+    src_count_ = nnz;
+    std::vector<int64_t> map;
+    if (rc.track) {
+        // input entry -> band slot of where reordering put it (-1 when it
+        // was dropped or was an explicit zero)
+        const int64_t w = 2 * (int64_t)k + 1;
+        map.assign(nnz, -1);
+        for (int r = 0; r < (int)n; ++r)
+            for (int e = R.rp[r]; e < R.rp[r + 1]; ++e) {
+                const int64_t c = R.ci[e];
+                map[R.entry_orig[e]] = c * w + k + r - c;
+            }
+        upload(upd_map_, map, stream);
+        if (rc.scale && !R.entry_scale.empty()) upload(upd_scale_, R.entry_scale, stream);
+        else upd_scale_.release();
+    } else {
+        upd_map_.release();
+        upd_scale_.release();
+    }
     vp_.resize(n);
     zp_.resize(n);
     SAP_HIP_CHECK(hipStreamSynchronize(stream));
@@ -203,6 +233,41 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
     SAP_HIP_CHECK(hipEventRecord(ev.e[1], stream));
     time_toBanded = elapsed_ms(ev.e[0], ev.e[1]);   // waits for the fill before the uploads are freed
     return factor_band();
+}
+
+template <class T>
+bool Precond<T>::update(const double* d_entries, int64_t count, int* rc)
+{
+    *rc = 0;
+    if (src_ == Source::none) throw Error(SAP_ERR_ILLEGAL_UPDATE, "Illegal call to update() before setup().");
+    if (src_ == Source::csr_reordered && !upd_map_.get())
+        throw Error(SAP_ERR_ILLEGAL_UPDATE, "Illegal call to update() with reordering tracking disabled.");
+    if (count != src_count_) return false;
+    if (cfg.precondType == SAP_None) return true;
+    const double t0 = std::chrono::duration<double, std::milli>(
+        std::chrono::steady_clock::now().time_since_epoch()).count();
+    time_transfer = 0;
+    time_offDiags = time_bandLU = time_bandUL = time_assembly = time_fullLU = 0;
+    Events ev;
+    SAP_HIP_CHECK(hipEventRecord(ev.e[0], stream));
+    {
+        ProfScope ps(prof, "update_band");
+        if (src_ == Source::band) {
+            launch_band_to_colmajor<T>(n, k, d_entries, B_.get(), stream);
+        } else {
+            SAP_HIP_CHECK(hipMemsetAsync(B_.get(), 0, B_.size() * sizeof(T), stream));
+            if (src_ == Source::csr)
+                launch_csr_to_band<double, T>(n, k, src_rp_, src_ci_, d_entries, B_.get(), stream);
+            else
+                launch_update_band<T>(count, d_entries, upd_map_.get(), upd_scale_.get(), B_.get(), stream);
+        }
+    }
+    SAP_HIP_CHECK(hipEventRecord(ev.e[1], stream));
+    time_toBanded = elapsed_ms(ev.e[0], ev.e[1]);
+    time_cpu_assemble = std::chrono::duration<double, std::milli>(
+        std::chrono::steady_clock::now().time_since_epoch()).count() - t0;
+    *rc = factor_band();
+    return true;
 }
 
 template <class T>

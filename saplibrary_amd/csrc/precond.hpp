@@ -23,6 +23,7 @@ struct ReorderConfig {
     bool scale = true;             // applyScaling
     int maxBandwidth = 0x7fffffff;
     double dropOffFraction = 0;
+    bool track = false;            // trackReordering: keep what update() needs
 };
 
 template <class T>
@@ -62,6 +63,12 @@ public:
     // solve, right permutation and scaling when the setup reordered).
     void apply(const double* v, double* z);
     bool reordered() const { return reorder_; }
+    // Refactors with new values of the matrix given to the last setup, same
+    // pattern (Precond::update, sap/precond.h:751-900): count doubles in
+    // the layout of that setup (the row-major band, or the CSR values in
+    // their order).  Returns false when count does not match; throws
+    // Error(SAP_ERR_ILLEGAL_UPDATE) before a setup.
+    bool update(const double* d_entries, int64_t count, int* rc);
 
     bool spike_active() const { return cfg.precondType == SAP_Spike && P > 1 && k > 0; }
     bool lu_ul() const { return cfg.factMethod == SAP_LU_UL; }
@@ -81,6 +88,13 @@ private:
     DevBuf<int> row_perm_, col_reord_;
     DevBuf<T> row_scale_, col_scale_;
     DevBuf<double> vp_, zp_;
+    // update(): what the last setup was, and its value maps
+    enum class Source { none, band, csr, csr_reordered } src_ = Source::none;
+    int64_t src_count_ = 0;                 // doubles an update() takes
+    const int* src_rp_ = nullptr;           // CSR pattern of a plain CSR setup (caller's)
+    const int* src_ci_ = nullptr;
+    DevBuf<int64_t> upd_map_;               // input entry -> band slot (-1: dropped)
+    DevBuf<T> upd_scale_;                   // DB scale factor per input entry
     void begin_setup(int64_t n_, int k_);
     int factor_band();
     void solve_band(const double* v, double* z);

@@ -135,6 +135,10 @@ template <class T>
 void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st);
 template <class Tin, class T>   // B zero-filled beforehand
 void launch_csr_to_band(int64_t n, int k, const int* rp, const int* ci, const Tin* vals, T* B, hipStream_t st);
+// B[map[e]] = (T)entries[e] * scale[e] for map[e] >= 0 (scale may be null)
+template <class T>
+void launch_update_band(int64_t nnz, const double* entries, const int64_t* map, const T* scale, T* B,
+                        hipStream_t st);
 // atomic max of max|row - col| into *out (zeroed beforehand)
 void launch_csr_bandwidth(int64_t n, const int* rp, const int* ci, int* out, hipStream_t st);
 void launch_csr_spmv(int64_t n, int64_t nnz, const int* rp, const int* ci, const double* val, const double* x,

@@ -604,6 +604,7 @@ public:
         rc.scale = opt.applyScaling != 0;
         rc.maxBandwidth = opt.maxBandwidth;
         rc.dropOffFraction = opt.dropOffFraction;
+        rc.track = opt.trackReordering != 0;
         int r = pf ? pf->setup_csr(n, nnz, rp, ci, val, rc) : pd->setup_csr(n, nnz, rp, ci, val, rc);
         SAP_HIP_CHECK(hipStreamSynchronize(stream));
         if (r != 0) return r;
@@ -611,6 +612,41 @@ public:
         if (pf) harvest(*pf, t0); else harvest(*pd, t0);
         setup_done = true;
         return 0;
+    }
+
+    // Solver::update (sap/solver.h:421-462): 1 done, 0 when the number of
+    // entries does not match the setup matrix, negative on error.
+    int update(const double* entries, int64_t count)
+    {
+        if (!setup_done) throw Error(SAP_ERR_ILLEGAL_UPDATE, "Illegal call to update() before setup().");
+        if (!opt.trackReordering)
+            throw Error(SAP_ERR_ILLEGAL_UPDATE, "Illegal call to update() with reordering tracking disabled.");
+        const double t0 = wall_ms();
+        int rc = 0;
+        const bool done = pf ? pf->update(entries, count, &rc) : pd->update(entries, count, &rc);
+        SAP_HIP_CHECK(hipStreamSynchronize(stream));
+        if (!done) return 0;
+        if (rc != 0) {
+            setup_done = false;
+            return rc;
+        }
+        stats.timeUpdate = wall_ms() - t0;
+        auto fill = [&](const auto& p) {
+            stats.time_reorder = 0;
+            stats.time_cpu_assemble = p.time_cpu_assemble;
+            stats.time_transfer = p.time_transfer;
+            stats.time_toBanded = p.time_toBanded;
+            stats.time_offDiags = p.time_offDiags;
+            stats.time_bandLU = p.time_bandLU;
+            stats.time_bandUL = p.time_bandUL;
+            stats.time_assembly = p.time_assembly;
+            stats.time_fullLU = p.time_fullLU;
+        };
+        if (pf) fill(*pf); else fill(*pd);
+        // the built-in SpMV follows the updated values (non-owning, as at setup)
+        if (d_rp) d_val = entries;
+        else d_band = entries;
+        return 1;
     }
 
     int solve(sap_spmv_fn fn, void* ctx, const double* d_b, double* d_x)
@@ -811,6 +847,13 @@ int sap_setup_csr(sap_solver* h, int n, int nnz, const int* d_row_offsets, const
         return SAP_ERR_UNSUPPORTED;
     }
     return guarded(h, [&] { return h->s->setup_csr(n, nnz, d_row_offsets, d_col_indices, d_values); });
+}
+
+int sap_update(sap_solver* h, const double* d_entries, int64_t count)
+{
+    if (!h) return SAP_ERR_BADARG;
+    if (!d_entries || count < 0) { h->err = "bad update arguments"; return SAP_ERR_BADARG; }
+    return guarded(h, [&] { return h->s->update(d_entries, count); });
 }
 
 int sap_solve(sap_solver* h, sap_spmv_fn spmv, void* ctx, const double* d_b, double* d_x, int nrhs)

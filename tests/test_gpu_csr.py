@@ -189,3 +189,107 @@ def test_csr_options_rejected(sap):
     with pytest.raises(sap.SapError) as ei:
         sap.Solver(2, sap.Options(variableBandwidth=0, testDB=1)).setup(A)
     assert ei.value.reason == -9
+
+
+# ------------------------------------------------------------------- update
+def _perturb(val, seed):
+    return val * (1 + np.random.default_rng(seed).uniform(-0.05, 0.05, val.size))
+
+
+@pytest.mark.parametrize("prec", ["float", "double"])
+@pytest.mark.parametrize("o", [dict(), dict(drop_off=0.01), dict(scale=False)], ids=["db-rcm", "drop", "noscale"])
+def test_update_reordered_csr_matches_oracle(sap, oracle, o, prec):
+    """update() keeps the setup's permutations and scale factors and only
+    refills the band (sap/precond.h:751-775): the oracle's expected band is
+    the new value of every entry that setup kept, times its scale factor
+    T(r_i c_j), at its reordered place."""
+    n, P = 1200, 3
+    rp, ci, val = sparse_band(n, 50, 5, 81)
+    T = np.float32 if prec == "float" else np.float64
+    ref = G.reorder(rp, ci, val, prec=prec, **o)
+    k = ref["k"]
+    opts = _opts(sap, o, "LU_UL", prec)
+    opts.trackReordering = 1
+    g = sap.Solver(P, opts)
+    A = sap.CsrMatrix.from_host(rp, ci, val)
+    g.setup(A)
+    val2 = _perturb(val, 82)
+    assert g.update(dev(val2))
+
+    scale = o.get("scale", True)
+    kept = set(zip(np.repeat(np.arange(n), np.diff(ref["rp"])).tolist(), ref["ci"].tolist()))
+    colperm = np.empty(n, dtype=np.int64)
+    colperm[ref["col_reord"]] = np.arange(n)
+    band = np.zeros((n, 2 * k + 1))
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    for i, j, a in zip(rows, ci, val2):
+        r, c = int(ref["row_perm"][i]), int(colperm[j])
+        if (r, c) not in kept:
+            continue
+        f = T(ref["row_scale_d"][i] * ref["col_scale_d"][j]) if scale else T(1)
+        band[r, k + c - r] = float(T(a) * f)
+    cpu = oracle.Solver(P, fact="LU_UL", prec_double=(prec == "double"))
+    assert cpu.setup(band) == 0
+
+    v = oracle.gen_vector(n, 83)
+    w = np.zeros(n)
+    w[ref["row_perm"]] = (v.astype(T) * (ref["row_scale"].astype(T) if scale else T(1))).astype(np.float64)
+    zr = cpu.apply(w)
+    expect = np.zeros(n)
+    y = zr.astype(T) * (ref["col_scale"][ref["col_reord"]].astype(T) if scale else T(1))
+    expect[ref["col_reord"]] = y.astype(np.float64)
+    z = torch.empty(n, dtype=torch.float64, device="cuda")
+    g.precond_apply(dev(v), z)
+    if prec == "double":
+        atol = 1e-10 * np.abs(expect).max()
+    else:
+        cpu64 = oracle.Solver(P, fact="LU_UL", prec_double=True)
+        assert cpu64.setup(band) == 0
+        w64 = np.zeros(n)
+        w64[ref["row_perm"]] = v * (ref["row_scale"] if scale else 1.0)
+        z64 = np.zeros(n)
+        z64[ref["col_reord"]] = cpu64.apply(w64) * (ref["col_scale"][ref["col_reord"]] if scale else 1.0)
+        atol = max(3e-5 * np.abs(expect).max(), 4 * np.abs(expect - z64).max())
+    np.testing.assert_allclose(host(z), expect, rtol=0, atol=atol)
+    # the built-in SpMV now multiplies by the new values
+    import scipy.sparse as sp
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    b = sp.csr_matrix((val2, ci, rp), shape=(n, n)) @ np.ones(n)
+    g.solve(None, dev(b), x)
+    assert g.getStats().numSpmv > 0 and g.getStats().timeUpdate > 0
+
+
+def test_update_banded_and_plain_csr_equal_a_fresh_setup(sap, oracle):
+    n, k, P = 4000, 30, 4
+    band1 = oracle.gen_banded(n, k, 0.9, 91)
+    band2 = oracle.gen_banded(n, k, 1.1, 92)
+    opts = sap.Options(factMethod=sap.LU_UL, trackReordering=1)
+    g = sap.Solver(P, opts)
+    g.setup(sap.BandedMatrix(dev(band1)))
+    assert g.update(dev(band2))
+    fresh = sap.Solver(P, sap.Options(factMethod=sap.LU_UL))
+    fresh.setup(sap.BandedMatrix(dev(band2)))
+    np.testing.assert_array_equal(g.factors(), fresh.factors())
+
+    rp, ci, val = csr_of_band(band1)
+    _, _, val2 = csr_of_band(band2)
+    copts = sap.Options(factMethod=sap.LU_UL, performReorder=0, performDB=0, applyScaling=0,
+                        variableBandwidth=0, trackReordering=1)
+    gc = sap.Solver(P, copts)
+    gc.setup(sap.CsrMatrix.from_host(rp, ci, val))
+    assert gc.update(dev(val2))
+    np.testing.assert_array_equal(gc.factors(), fresh.factors())
+    assert not gc.update(dev(val2[:-1]))           # wrong number of entries: false
+
+
+def test_update_errors(sap, oracle):
+    band = oracle.gen_banded(500, 5, 1.0, 93)
+    g = sap.Solver(2, sap.Options(trackReordering=1))
+    with pytest.raises(sap.SapError) as ei:
+        g.update(dev(band))                         # before setup
+    assert ei.value.reason == -3
+    g2 = sap.Solver(2, sap.Options())
+    g2.setup(sap.BandedMatrix(dev(band)))
+    with pytest.raises(sap.SapError) as ei:
+        g2.update(dev(band))                        # trackReordering off
+    assert ei.value.reason == -3
