@@ -1,6 +1,7 @@
 // Header-only C++ facade with the reference's interface
-// (spikegpu/SaPLibrary sap/solver.h:52-225, sap/banded_matrix.h, sap/spmv.h)
-// over the C ABI of include/sap_c.h.  Array types are any device array with
+// (spikegpu/SaPLibrary sap/solver.h:52-225, sap/banded_matrix.h, sap/spmv.h,
+// and the cusp::csr_matrix input of Solver::setup) over the C ABI of
+// include/sap_c.h.  Array types are any device array with
 // `value_type`, `size()` and `data()` returning a device pointer (for example
 // sap::DeviceArray below); the reference used cusp::array1d<..., device_memory>.
 #pragma once
@@ -73,12 +74,42 @@ template <class Array>
 class MVBanded {
 public:
     explicit MVBanded(const BandedMatrix<Array>& A) : m_A(A), num_rows(A.num_rows) {}
-    void operator()(const Array& v, Array& Av) const
+    void operator()(const Array& v, Array& Av) const { spmv_raw(v.data(), Av.data(), num_rows); }
+    void spmv_raw(const double* v, double* Av, int n) const
     {
-        int rc = sap_band_spmv(m_A.num_rows, m_A.m_k, m_A.band->data(), v.data(), Av.data(), nullptr);
+        int rc = sap_band_spmv(n, m_A.m_k, m_A.band->data(), v, Av, nullptr);
         if (rc < 0) throw system_error(rc, "sap_band_spmv failed");
     }
     const BandedMatrix<Array>& m_A;
+    int num_rows;
+};
+
+// FP64 CSR matrix on the device with zero-based int indices, the fields of
+// cusp::csr_matrix<int, double, cusp::device_memory>.
+template <class IndexArray, class ValueArray>
+struct CsrMatrix {
+    int num_rows = 0, num_cols = 0, num_entries = 0;
+    const IndexArray* row_offsets = nullptr;
+    const IndexArray* column_indices = nullptr;
+    const ValueArray* values = nullptr;
+    CsrMatrix(int n, int nnz, const IndexArray& rp, const IndexArray& ci, const ValueArray& val)
+        : num_rows(n), num_cols(n), num_entries(nnz), row_offsets(&rp), column_indices(&ci), values(&val) {}
+};
+
+// SPMV functor over a CsrMatrix (SpmvCusp, sap/spmv.h:83-129).
+template <class Matrix>
+class SpmvCusp {
+public:
+    explicit SpmvCusp(const Matrix& A) : m_A(A), num_rows(A.num_rows) {}
+    template <class Array>
+    void operator()(const Array& v, Array& Av) const { spmv_raw(v.data(), Av.data(), num_rows); }
+    void spmv_raw(const double* v, double* Av, int n) const
+    {
+        int rc = sap_csr_spmv(n, m_A.num_entries, m_A.row_offsets->data(), m_A.column_indices->data(),
+                              m_A.values->data(), v, Av, nullptr);
+        if (rc < 0) throw system_error(rc, "sap_csr_spmv failed");
+    }
+    const Matrix& m_A;
     int num_rows;
 };
 
@@ -101,6 +132,25 @@ public:
     {
         check(sap_setup_banded(m_h, M.num_rows, M.m_k, M.band->data()));
         return true;
+    }
+
+    // CSR input: DB / RCM / drop-off as the options ask (needs
+    // variableBandwidth = false when performReorder is set).
+    template <class I, class V>
+    bool setup(const CsrMatrix<I, V>& M)
+    {
+        check(sap_setup_csr(m_h, M.num_rows, M.num_entries, M.row_offsets->data(), M.column_indices->data(),
+                            M.values->data()));
+        return true;
+    }
+
+    // New values, same pattern (sap/solver.h:421-462); needs trackReordering.
+    template <class Array1>
+    bool update(const Array1& entries)
+    {
+        int rc = sap_update(m_h, entries.data(), (int64_t)entries.size());
+        check(rc);
+        return rc == 1;
     }
 
     // Any SPMV functor with operator()(const Array&, Array&); MVBanded over
@@ -146,12 +196,11 @@ private:
         View vin{const_cast<double*>(v), (size_t)n}, vout{Av, (size_t)n};
         op(vin, vout);
     }
-    // MVBanded and other functors bound to Array itself: call the band SpMV directly
+    // MVBanded, SpmvCusp and other functors bound to Array itself: their raw entry
     template <class Op>
     static void invoke(Op& op, const double* v, double* Av, int n, long)
     {
-        int rc = sap_band_spmv(n, op.m_A.m_k, op.m_A.band->data(), v, Av, nullptr);
-        if (rc < 0) throw system_error(rc, "sap_band_spmv failed");
+        op.spmv_raw(v, Av, n);
     }
     void check(int rc) const
     {

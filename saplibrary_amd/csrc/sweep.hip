@@ -18,10 +18,14 @@
 // SaP-D (or P = 1) runs pass 2 alone.
 //
 // Triangular sweeps: one workgroup per (partition, direction) walks the rows
-// in blocks of 64.  Wavefront 0 solves the 64 x 64 unit-lower diagonal block
-// with a lane-broadcast chain; then every thread updates one of the next k
-// rows with the block's 64 multipliers.  The pending values of the next
-// 2*64 + k rows live in an LDS ring.
+// in blocks; wavefront 0 solves each block's unit-lower diagonal block with
+// a lane-broadcast chain, the other wavefronts apply the block to the next k
+// rows, and the pending values live in an LDS ring.  Three variants:
+//   MODE 2 (k <= 192 + BS, the default): multipliers and right-hand side
+//          staged HBM -> LDS by producer wavefronts several blocks ahead;
+//   MODE 1 (same k range): multipliers prefetched into registers one block
+//          ahead;
+//   MODE 0 (any k): the plain blocked sweep with blocks of 64.
 #include "sap_internal.hpp"
 
 #include <algorithm>

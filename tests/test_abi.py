@@ -91,7 +91,16 @@ def test_cpp_facade_compiles(tmp_path):
         "  sap::Solver<Vec, float> solver(2, opts);\n"
         "  sap::MVBanded<Vec> spmv(A);\n"
         "  solver.setup(A);\n"
-        "  return solver.solve(spmv, b, x) ? 0 : 1;\n"
+        "  bool ok = solver.solve(spmv, b, x);\n"
+        "  sap::DeviceArray<int> rp(11), ci(28);\n"
+        "  Vec val(28);\n"
+        "  sap::CsrMatrix<sap::DeviceArray<int>, Vec> C(10, 28, rp, ci, val);\n"
+        "  opts.variableBandwidth = false; opts.trackReordering = true;\n"
+        "  sap::Solver<Vec, double> s2(2, opts);\n"
+        "  sap::SpmvCusp<sap::CsrMatrix<sap::DeviceArray<int>, Vec> > cs(C);\n"
+        "  s2.setup(C);\n"
+        "  s2.update(val);\n"
+        "  return (ok && s2.solve(cs, b, x)) ? 0 : 1;\n"
         "}\n")
     r = subprocess.run([hipcc, "-std=c++17", "-I", os.path.join(ROOT, "include"), "-c", str(src), "-o",
                         str(tmp_path / "facade.o")], capture_output=True, text=True)
