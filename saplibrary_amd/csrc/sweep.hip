@@ -58,6 +58,27 @@ __device__ inline BandView<T> make_view(const T* B, int64_t s, int64_t len, int 
     return v;
 }
 
+// v if bit j of m is set, else +0 (bitwise: a signed one-bit field extract
+// and an AND, so no compare and no scalar mask register per element)
+__device__ inline float keep_if(float v, unsigned m, int j)
+{
+    return __int_as_float(__float_as_int(v) & __builtin_amdgcn_sbfe((int)m, j, 1));
+}
+__device__ inline double keep_if(double v, unsigned m, int j)
+{
+    // the 32-bit field mask applied to each half (hipcc drops the sign
+    // extension of the field extract when it is widened to 64 bits first)
+    const unsigned sm = (unsigned)__builtin_amdgcn_sbfe((int)m, j, 1);
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)b & sm, hi = (unsigned)(b >> 32) & sm;
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// bits 0 .. n-1 set (n clamped to [0, 32])
+__device__ inline unsigned bits_below(int n)
+{
+    return n <= 0 ? 0u : n >= 32 ? ~0u : (1u << n) - 1u;
+}
+
 __device__ inline float bcast(float v, int j)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
@@ -272,6 +293,7 @@ __device__ __forceinline__ void lower_sweep_fast(const BandView<T>& V, int64_t l
 // vmcnt count for its own copies before the barrier that publishes them.
 // ---------------------------------------------------------------------------
 constexpr int SW_PROD = 4;           // producer wavefronts
+constexpr int SW_NI_MAX = 8;         // load instructions per producer per block (k <= 192 + BS)
 constexpr int SW_THREADS = 512;
 constexpr int SW_DIN = 8;            // right-hand side steps in flight
 constexpr int SW_NIN = SW_DIN + 1;   // staging slots of 64 dwords
@@ -353,20 +375,43 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
     const int dwm = (int)(V.dir * V.w);
     const int kadj = V.dir < 0 ? k : 0;
 
-    // producer: chunks of block b into buffer b % nbuf (columns past the end
-    // and padding lanes re-read the block's first chunk: in bounds, never used)
-    auto produce = [&](int64_t b, int lane) {
-        T* dst = bufs + (size_t)(b % nbuf) * bufe;
-        const int64_t c0 = b * BS;
-        const int64_t lo_first = seg_lo(c0) & ~(int64_t)(E - 1);
-        for (int i = 0; i < G.ni; ++i) {
+    // producer: chunks of block b into buffer slot = b % nbuf (columns past
+    // the end and padding lanes re-read the block's first chunk: in bounds,
+    // never used)
+    // The chunk a producer lane copies in load i of a block is the same for
+    // every block up to a uniform shift: seg_lo(c0 + j) = seg_lo(j) +
+    // dir * w * c0, and w * c0 is a multiple of E.  So the offsets (for block
+    // 0) and columns are tabulated once per lane.
+    int64_t poff[SW_NI_MAX];
+    int pcol[SW_NI_MAX];
+    const int64_t first0 = seg_lo(0) & ~(int64_t)(E - 1);
+    if (wave >= 4) {
+        const int lane = threadIdx.x & 63;
+#pragma unroll
+        for (int i = 0; i < SW_NI_MAX; ++i) {
             const int rnd = i * SW_PROD + (wave - 4);       // 64-chunk round of this block
             const int g = rnd * 64 + lane;
             const int j = g / G.nch, ch = g - j * G.nch;
-            int64_t off = lo_first;
-            if (g < BS * G.nch && c0 + j < len) off = (seg_lo(c0 + j) & ~(int64_t)(E - 1)) + (int64_t)ch * E;
-            __builtin_amdgcn_global_load_lds(V.base + off, dst + (size_t)rnd * 64 * E, 16, 0, 0);
+            const bool real = i < G.ni && g < BS * G.nch;
+            poff[i] = real ? (seg_lo(j) & ~(int64_t)(E - 1)) + (int64_t)ch * E : first0;
+            pcol[i] = real ? j : 0;
         }
+    }
+    const int64_t blk_step = (int64_t)V.dir * V.w * BS;
+    auto produce = [&](int64_t b, int slot, int lane) {
+        T* dst = bufs + (size_t)slot * bufe;
+        const int64_t shift = b * blk_step;
+        const int64_t rem = len - b * BS;                // columns of this block inside the view
+#pragma unroll
+        for (int i = 0; i < SW_NI_MAX; ++i) {
+            if (i < G.ni) {
+                const int rnd = i * SW_PROD + (wave - 4);
+                // columns past the end re-read the block's first chunk
+                const int64_t off = (pcol[i] < rem ? poff[i] : first0) + shift;
+                __builtin_amdgcn_global_load_lds(V.base + off, dst + (size_t)rnd * 64 * E, 16, 0, 0);
+            }
+        }
+        (void)lane;
     };
     auto produced_after = [&](int64_t b) -> int {   // this wave's block loads issued for blocks > b
         const int64_t last = min(nblk - 1, b + nbuf - 3);
@@ -383,14 +428,31 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
     };
 
     if (wave >= 4) {
-        for (int64_t b = 0; b < min(nblk, (int64_t)nbuf - 2); ++b) produce(b, threadIdx.x & 63);
+        for (int64_t b = 0; b < min(nblk, (int64_t)nbuf - 2); ++b) produce(b, (int)b, threadIdx.x & 63);
         wait_vmcnt(produced_after(0));
     } else if (wave == 1) {
         for (int64_t st = 0; st < min(nstream, (int64_t)SW_DIN); ++st) stream(st, threadIdx.x & 63);
     }
     lds_barrier();
 
-    for (int64_t b = 0; b <= nblk; ++b) {
+    // Wave 0's band masks, bit j for block column j (lanes 0..BS-1):
+    //   ma0: diagonal block, row i, column j < i with i - j <= k;
+    //   mb0: sub-diagonal block, row BS + i, column j with BS + i - j <= k.
+    //   mf0: far row t of waves 1-3 (row 2 BS + t of the previous block),
+    //        column j with 2 BS + t - j <= k.
+    unsigned ma0 = 0u, mb0 = 0u;
+    {
+        const int i = threadIdx.x & 63;
+        if (i < BS) {
+            ma0 = bits_below(i) & ~bits_below(i - k);
+            mb0 = ~bits_below(BS + i - k);
+        }
+    }
+    const unsigned mf0 = ~bits_below(2 * BS + ((int)threadIdx.x - 64) - k);
+    // slot = b % nbuf, kept as a 32-bit counter (a 64-bit modulo per step
+    // costs more than the step's arithmetic)
+    int slot = 0;
+    for (int64_t b = 0; b <= nblk; ++b, slot = slot + 1 == nbuf ? 0 : slot + 1) {
         // lane-derived values re-materialised per step (no hoisted predicates)
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
@@ -404,20 +466,21 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
                 // colb(j) + dir * i with colb(j) = j * segl + off_j - dir * j
                 // (wave-uniform); rows outside the band read a neighbouring
                 // LDS word (inside the buffers or their guard) and are masked
-                const T* buf = bufs + (size_t)(b % nbuf) * bufe;
+                const T* buf = bufs + (size_t)slot * bufe;
                 const int lo0 = (int)seg_lo(c0);
                 const int i = lane;
                 const int di = V.dir * i;
                 const bool rv = i < nb;
                 const bool sv = i < BS && c0 + BS + i < len;
+                // masks of this step (rows past the end, columns past nb)
+                const unsigned ma = rv ? ma0 : 0u;
+                const unsigned mb = sv ? (mb0 & (nb >= 32 ? ~0u : (1u << nb) - 1u)) : 0u;
                 T a[BS], sb[BS];
 #pragma unroll
                 for (int j = 0; j < BS; ++j) {
                     const int colb = j * G.segl + ((lo0 + j * dwm) & (E - 1)) + kadj - V.dir * j;
-                    const T va = buf[colb + di];
-                    const T vb = buf[colb + V.dir * BS + di];
-                    a[j] = (rv && j < i && i - j <= k) ? va : (T)0;
-                    sb[j] = (sv && j < nb && BS + i - j <= k) ? vb : (T)0;
+                    a[j] = keep_if(buf[colb + di], ma, j);
+                    sb[j] = keep_if(buf[colb + V.dir * BS + di], mb, j);
                 }
                 const int ii = i < BS ? i : 0;
                 const T dv = buf[ii * G.segl + ((lo0 + ii * dwm) & (E - 1)) + kadj];
@@ -446,7 +509,7 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
             const int t = tid - 64;   // far-row index
             if (b >= 1) {
                 const int64_t cp = c0 - BS;
-                const T* buf = bufs + (size_t)((b - 1) % nbuf) * bufe;
+                const T* buf = bufs + (size_t)(slot == 0 ? nbuf - 1 : slot - 1) * bufe;
                 const int lo0 = (int)seg_lo(cp);
                 const T* xp = xblk2 + (1 - par) * BS;
                 const int64_t r = cp + 2 * BS + t;
@@ -457,11 +520,8 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
                 for (int j = 0; j < BS; j += 2) {
                     const int colb0 = j * G.segl + ((lo0 + j * dwm) & (E - 1)) + kadj - V.dir * j;
                     const int colb1 = (j + 1) * G.segl + ((lo0 + (j + 1) * dwm) & (E - 1)) + kadj - V.dir * (j + 1);
-                    const int q0 = 2 * BS + t - j;
-                    const T v0 = buf[colb0 + dt];
-                    const T v1 = buf[colb1 + dt];
-                    s0_ += (q0 <= k ? v0 : (T)0) * xp[j];
-                    s1_ += (q0 - 1 <= k ? v1 : (T)0) * xp[j + 1];
+                    s0_ += keep_if(buf[colb0 + dt], mf0, j) * xp[j];
+                    s1_ += keep_if(buf[colb1 + dt], mf0, j + 1) * xp[j + 1];
                 }
                 const T acc = s0_ + s1_;
                 if (t < BS) acc2[(1 - par) * BS + t] = fv ? acc : (T)0;
@@ -479,7 +539,7 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
                 }
             }
         } else {
-            if (b + nbuf - 2 < nblk) produce(b + nbuf - 2, lane);
+            if (b + nbuf - 2 < nblk) produce(b + nbuf - 2, slot >= 2 ? slot - 2 : slot + nbuf - 2, lane);
             wait_vmcnt(produced_after(b + 1));
         }
         lds_barrier();
@@ -724,7 +784,7 @@ SweepCfg sweep_cfg(int k)
     const size_t guard = 256 * sizeof(T);   // masked reads past the last buffer stay inside the allocation
     int nbuf = (int)std::min<size_t>(8, (LDS_MAX - ring - stage - guard) / bb);
     nbuf = std::min(nbuf, 63 / G.ni + 3);
-    if (nbuf < 4) return {1, 0, ring, 256};
+    if (nbuf < 4 || G.ni > SW_NI_MAX) return {1, 0, ring, 256};
     return {2, nbuf, ring + stage + nbuf * bb + guard, SW_THREADS};
 }
 
