@@ -33,7 +33,9 @@ namespace {
 //       threads U12 = L11^-1 A12 column by column (unit L11);
 //   C.  trailing tiles += (-L21) U12 on the matrix cores
 //       (v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64, the 32 panel columns
-//       split over the four lane groups).
+//       split over the four lane groups).  Tiles are kept transposed in the
+//       accumulators (the update is acc^T += U12^T (-L21)^T), which makes each
+//       tile load read whole 16-row column segments.
 // Outside the band every stored value is an exact zero, so the padded tiles
 // never create fill-in.  The wavefronts exchange data only through LDS (each
 // tile load is consumed by the wavefront that issued it; factor stores are
@@ -127,6 +129,37 @@ __device__ inline void buffer_store<double>(double v, __amdgpu_buffer_rsrc_t r, 
 __device__ inline float pivot_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
 __device__ inline double pivot_rcp(double v) { return 1.0 / v; }
 
+// All ones when x < 0, else 0: one field extract of the sign bit.  Range
+// predicates are combined as sign masks of differences (ANDed), so no
+// per-element compare writes a scalar mask register.
+__device__ inline int neg_mask(int x) { return __builtin_amdgcn_sbfe(x, 31, 1); }
+// the sign of v flipped where m is all ones
+__device__ inline float flip_sign(float v, int m) { return __int_as_float(__float_as_int(v) ^ (m & (int)0x80000000)); }
+__device__ inline double flip_sign(double v, int m)
+{
+    const unsigned long long sb = (unsigned long long)((unsigned)m & 0x80000000u) << 32;
+    return __longlong_as_double((long long)((unsigned long long)__double_as_longlong(v) ^ sb));
+}
+// a where m is all ones, else b
+__device__ inline float pick(float a, float b, int m)
+{
+    return __int_as_float((__float_as_int(a) & m) | (__float_as_int(b) & ~m));
+}
+__device__ inline double pick(double a, double b, int m)
+{
+    const long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+    const unsigned lo = ((unsigned)x & (unsigned)m) | ((unsigned)y & ~(unsigned)m);
+    const unsigned hi = ((unsigned)(x >> 32) & (unsigned)m) | ((unsigned)(y >> 32) & ~(unsigned)m);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// (x + KT - sh) mod KT for 0 <= x, sh < KT
+template <int KT>
+__device__ inline int rot(int x, int sh)
+{
+    const int t = x - sh;
+    return t < 0 ? t + KT : t;
+}
+
 template <class T, int KT>
 __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict__ B0, T* __restrict__ B1, int k,
                                                                    const FactorJob* __restrict__ jobs)
@@ -146,8 +179,11 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
     T* const sm = reinterpret_cast<T*>(smem_raw);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: slot logic stays scalar
 
-    // rem = len - j0 is the number of partition rows left from panel row j0.
-    auto inb = [&](int dr, int dc, int rem) { return dr < rem && dc < rem && dr - dc <= k && dc - dr <= k; };
+    // rem = len - j0 is the number of partition rows left from panel row j0;
+    // entry (dr, dc) of the panel's window is stored iff this mask is set
+    auto inb = [&](int dr, int dc, int rem) {
+        return neg_mask((dr - rem) & (dc - rem) & (dr - dc - k - 1) & (dc - dr - k - 1));
+    };
     // Tile loads are buffer loads over the partition's storage: every element
     // is one unconditional load, and entries outside the band or the
     // partition use an offset past the end of the buffer, which reads zero.
@@ -157,12 +193,17 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(base + job.start * w), 0, (int)(len * w * (int64_t)sizeof(T)), 0x00020000);
     const int rel0 = dir > 0 ? 0 : (int)(len * w - 1);
+    // Tiles are held transposed: accumulator register q of lane l holds
+    // A(dr0 + l % 16, dc0 + row(l, q)), so for each register the 16 lanes of
+    // a group read 16 consecutive rows of one column (one contiguous segment
+    // per lane group).
     auto load_tile = [&](acc_t& t, int ob, int rem, int dr0, int dc0, int lr, int lane) {
-        const int dc = dc0 + lr;
+        const int dr = dr0 + lr;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int dr = dr0 + MF::row(lane, q);
-            const int off = inb(dr, dc, rem) ? (ob + dir * (dc * wm1 + dr)) * (int)sizeof(T) : -1;
+            const int dc = dc0 + MF::row(lane, q);
+            const int m = inb(dr, dc, rem);
+            const int off = (((ob + dir * (dc * wm1 + dr)) * (int)sizeof(T)) & m) | ~m;
             t[q] = buffer_load<T>(rsrc, off);
         }
     };
@@ -181,27 +222,28 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         const int pnb = min(32, prem);
         const T* qPn = bq + LY::pn_off;
         const T* qUb = bq + LY::ub_off;
-        const T* qpv = bq + LY::pv_off;
+        const T* qrp = bq + LY::rp_off;   // pivot reciprocals (0 past the partition's end)
         T v1[N1], v2[N2];
         int o1[N1], o2[N2];
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
             const int e = tid + 512 * i;
             const int c = e / HR, r = e - c * HR;
-            const bool ok = e < 32 * HR && c < pnb && r < prem && r - c <= k && c - r <= k;
-            T v = qPn[(ok ? r * PS + c : 0)];
-            const T pv = qpv[(r < 32 ? r : 0)];
-            v = r >= 32 ? -v : (r < c ? v / pv : v);
-            v1[i] = v;
-            o1[i] = ok ? (obq + dir * (c * wm1 + r)) * (int)sizeof(T) : -1;
+            const int ok = neg_mask((e - 32 * HR) & (c - pnb) & (r - prem) & (r - c - k - 1) & (c - r - k - 1));
+            const int lrow = ~neg_mask(r - 32);   // L21 rows, kept negated in LDS
+            const int upper = neg_mask(r - c);    // U' of the diagonal block: scaled by 1 / pivot
+            const T v = qPn[(r * PS + c) & ok];
+            const T f = qrp[r & ~lrow & 31];
+            v1[i] = flip_sign(v * pick(f, (T)1, upper), lrow);
+            o1[i] = (((obq + dir * (c * wm1 + r)) * (int)sizeof(T)) & ok) | ~ok;
         }
 #pragma unroll
         for (int i = 0; i < N2; ++i) {
             const int e = tid + 512 * i;
             const int cc = e >> 5, m = e & 31;
-            const bool ok = e < 32 * NTR && m < pnb && 32 + cc < prem && 32 + cc - m <= k;
-            v2[i] = qUb[(ok ? cc * PS + m : 0)] / qpv[m];
-            o2[i] = ok ? (obq + dir * ((32 + cc) * wm1 + m)) * (int)sizeof(T) : -1;
+            const int ok = neg_mask((e - 32 * NTR) & (m - pnb) & (32 + cc - prem) & (32 + cc - m - k - 1));
+            v2[i] = qUb[(cc * PS + m) & ok] * qrp[m];
+            o2[i] = (((obq + dir * ((32 + cc) * wm1 + m)) * (int)sizeof(T)) & ok) | ~ok;
         }
 #pragma unroll
         for (int i = 0; i < N1; ++i) buffer_store<T>(v1[i], rsrc, o1[i]);
@@ -243,6 +285,13 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         const int sh = (int)((2 * p) % KT);
         const int obp = rel0 + dir * (int)(j0 * w + k);
 
+        // Every load and store of the previous panel has long completed (B1,
+        // B2 and C ran since); saying so with an explicit wait clears the
+        // compiler's record of loads pending across the loop back-edge, which
+        // would otherwise make each slot below wait for the loads just issued
+        // by the slots before it (vmcnt bits [3:0] and [15:14] = 0, expcnt and
+        // lgkmcnt at their maxima).
+        __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8));
         // A. leaving tiles to LDS, then their slots load the entering tiles,
         //    then the previous panel's factors go out: every global access of
         //    the panel is issued here, so loads and stores drain during B1 and
@@ -253,13 +302,13 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         for (int j = 0; j < SPW; ++j) {
             const int s = wave + 8 * j;
             if (s < NS) {
-                const int rho = (s / KT - sh + KT) % KT, gam = (s % KT - sh + KT) % KT;
+                const int rho = rot<KT>(s / KT, sh), gam = rot<KT>(s % KT, sh);
                 if (gam < 2) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) Pn[(16 * rho + MF::row(lane, q)) * PS + 16 * gam + lr] = acc[j][q];
+                    for (int q = 0; q < 4; ++q) Pn[(16 * rho + lr) * PS + 16 * gam + MF::row(lane, q)] = acc[j][q];
                 } else if (rho < 2) {
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) UbT[(16 * (gam - 2) + lr) * PS + 16 * rho + MF::row(lane, q)] = acc[j][q];
+                    for (int q = 0; q < 4; ++q) UbT[(16 * (gam - 2) + MF::row(lane, q)) * PS + 16 * rho + lr] = acc[j][q];
                 }
             }
         }
@@ -267,7 +316,7 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         for (int j = 0; j < SPW; ++j) {
             const int s = wave + 8 * j;
             if (s < NS) {
-                const int rho = (s / KT - sh + KT) % KT, gam = (s % KT - sh + KT) % KT;
+                const int rho = rot<KT>(s / KT, sh), gam = rot<KT>(s % KT, sh);
                 if (gam < 2 || rho < 2) {
                     const int rn = rho < 2 ? rho + KT - 2 : rho - 2;
                     const int cn = gam < 2 ? gam + KT - 2 : gam - 2;
@@ -349,7 +398,7 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         for (int j = 0; j < SPW; ++j) {
             const int s = wave + 8 * j;
             if (s < NS) {
-                const int rho = (s / KT - sh + KT) % KT, gam = (s % KT - sh + KT) % KT;
+                const int rho = rot<KT>(s / KT, sh), gam = rot<KT>(s % KT, sh);
                 if (rho >= 2 && gam >= 2) {
                     const T* ap = Pn + (16 * rho + lr) * PS + 8 * lg;
                     const T* bp = UbT + (16 * (gam - 2) + lr) * PS + 8 * lg;
@@ -357,7 +406,8 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
 #pragma unroll
                     for (int q = 0; q < 8; ++q) { av[q] = ap[q]; bv[q] = bp[q]; }
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) acc[j] = MF::mma(av[q], bv[q], acc[j]);
+                    // transposed tiles: acc^T += U12^T (-L21)^T
+                    for (int q = 0; q < 8; ++q) acc[j] = MF::mma(bv[q], av[q], acc[j]);
                 }
             }
         }
