@@ -603,6 +603,106 @@ __global__ __launch_bounds__(256) void schur_lu_kernel(T* __restrict__ S, int k,
     }
 }
 
+// LU of S_i and S_i^{-1} in one workgroup of 1024 threads with S_i held in
+// LDS (k^2 elements: k <= 200 in FP32, 141 in FP64).  The same operations in
+// the same order as schur_lu_kernel and schur_inverse_kernel (so the same
+// results), without their per-step trips through global memory:
+//   LU: per step j the pivot (boosted with safeFactorization), the
+//       multipliers of column j, then the trailing update with wavefronts
+//       over columns and lanes over rows (each lane keeps its rows'
+//       multipliers in registers);
+//   U rows divided by their pivots;
+//   inverse: one wavefront per identity column, the column vector in
+//       registers (lane l holds rows l, l + 64, ...), forward L, D^{-1},
+//       backward U' with the pivot entries broadcast by readlane.
+template <class T>
+__device__ inline T wave_bcast(const T (&y)[4], int m)
+{
+    const T v = m < 64 ? y[0] : m < 128 ? y[1] : m < 192 ? y[2] : y[3];
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), m & 63));
+    } else {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), m & 63);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), m & 63);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void schur_lu_inv_kernel(T* __restrict__ S, T* __restrict__ Sinv, int k, int safe)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* M = reinterpret_cast<T*>(smem_raw);   // S_i, column-major, leading dimension k
+    const size_t kk = (size_t)k * k;
+    T* A = S + (size_t)blockIdx.x * kk;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nw = blockDim.x >> 6;
+    for (size_t e = tid; e < kk; e += blockDim.x) M[e] = A[e];
+    __syncthreads();
+    for (int j = 0; j < k; ++j) {
+        T piv = M[(size_t)j * k + j];
+        if (safe) piv = boost_pivot(piv);
+        for (int t = j + 1 + tid; t < k; t += blockDim.x) M[(size_t)j * k + t] /= piv;
+        __syncthreads();   // multipliers written, every thread has read the pivot
+        if (tid == 0) M[(size_t)j * k + j] = piv;
+        T lm[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int t = j + 1 + lane + 64 * q;
+            lm[q] = t < k ? M[(size_t)j * k + t] : (T)0;
+        }
+        for (int c = j + 1 + wave; c < k; c += nw) {
+            const T u = M[(size_t)c * k + j];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int t = j + 1 + lane + 64 * q;
+                if (t < k) M[(size_t)c * k + t] -= lm[q] * u;
+            }
+        }
+        __syncthreads();
+    }
+    for (size_t e = tid; e < kk; e += blockDim.x) {
+        const int r = (int)(e % k), c = (int)(e / k);
+        if (r < c) M[e] /= M[(size_t)r * k + r];
+    }
+    __syncthreads();
+    for (size_t e = tid; e < kk; e += blockDim.x) A[e] = M[e];
+    T* X = Sinv + (size_t)blockIdx.x * kk;
+    for (int r = wave; r < k; r += nw) {
+        T y[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) y[q] = (lane + 64 * q == r) ? (T)1 : (T)0;
+        for (int m = 0; m < k - 1; ++m) {   // unit lower L
+            const T ym = wave_bcast(y, m);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int t = lane + 64 * q;
+                if (t > m && t < k) y[q] -= M[(size_t)m * k + t] * ym;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int t = lane + 64 * q;
+            if (t < k) y[q] /= M[(size_t)t * k + t];
+        }
+        for (int m = k - 1; m > 0; --m) {   // unit upper U'
+            const T ym = wave_bcast(y, m);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int t = lane + 64 * q;
+                if (t < m) y[q] -= M[(size_t)m * k + t] * ym;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int t = lane + 64 * q;
+            if (t < k) X[(size_t)r * k + t] = y[q];
+        }
+    }
+}
+
 // S_i^{-1} from its L D U' factors: RC identity columns per workgroup.
 template <class T>
 __global__ __launch_bounds__(256) void schur_inverse_kernel(const T* __restrict__ S, int k, T* __restrict__ Sinv)
@@ -734,6 +834,14 @@ void launch_reduced_factor(const T* WV, int k, int nb, T* S, T* Sinv, int safe, 
     const unsigned tiles = (unsigned)((k + GT - 1) / GT);
     hipLaunchKernelGGL(schur_gemm_kernel<T>, dim3(nb, tiles, tiles), dim3(256), 0, st, WV, k, S);
     SAP_LAUNCH_CHECK();
+    const size_t lds_lu = sizeof(T) * (size_t)k * k;
+    if (lds_lu <= LDS_LIMIT && k <= 256) {
+        SAP_HIP_CHECK(hipFuncSetAttribute((const void*)schur_lu_inv_kernel<T>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_lu));
+        hipLaunchKernelGGL(schur_lu_inv_kernel<T>, dim3(nb), dim3(1024), lds_lu, st, S, Sinv, k, safe);
+        SAP_LAUNCH_CHECK();
+        return;
+    }
     hipLaunchKernelGGL(schur_lu_kernel<T>, dim3(nb), dim3(256), 0, st, S, k, safe);
     SAP_LAUNCH_CHECK();
     const size_t lds = sizeof(T) * (size_t)k * (RC + 1);
