@@ -16,6 +16,9 @@
  *   SpmvCusp::operator()(v, Av)            -> sap_csr_spmv     (sap/spmv.h:83-129)
  *   Graph::reorder + Graph::dropOff        -> sap_reorder_csr  (sap/graph.h:446-747,
  *     (host side of transformToBandedMatrix)                    sap/precond.h:1656-1776)
+ *   Graph::assembleOffDiagMatrices,        -> sap_reordering_assemble_var
+ *     secondLevelReordering, partitionedRCM,   (sap/graph.h:756-1206, :1593-1757)
+ *     assembleBandedMatrix
  *   Solver::update(entries)                -> sap_update       (sap/solver.h:421-462,
  *                                              sap/precond.h:751-900)
  *   Solver::solve(spmv, b, x)              -> sap_solve        (sap/solver.h:477-562)
@@ -245,6 +248,25 @@ int         sap_reordering_get(const sap_reordering* r, int* db_perm, int* row_p
                                double* row_scale, double* col_scale, int* row_offsets,
                                int* col_indices, double* values);
 void        sap_reordering_free(sap_reordering* r);
+
+/* The variable-bandwidth assembly of the reordered matrix into
+ * numPartitions partitions (clamped to max(n/(k+1), 1) as
+ * transformToBandedMatrix does, sap/precond.h:1777-1783), on the host:
+ * Graph::assembleOffDiagMatrices, secondLevelReordering + partitionedRCM and
+ * assembleBandedMatrix (sap/graph.h:756-1206, :1593-1757).  Returns the
+ * partition count used (1: no variable-bandwidth assembly, as the reference
+ * falls back to the constant band) or a negative code. */
+int         sap_reordering_assemble_var(sap_reordering* r, int numPartitions);
+/* Its results (NULL skips an array); P partitions, k the half-bandwidth:
+ *   ks (P), boffsets (P+1), first_rows (P-1), second_reorder / second_perm (n),
+ *   widths_left / widths_right (P-1), perms_left / perms_right ((P-1) k),
+ *   wv / offdiags (2 k^2 (P-1)), ks_col / ks_row (n), and the diagonal blocks
+ *   after the second-level permutation as CSR (n+1, nnz_d, nnz_d). */
+int         sap_reordering_var_get(const sap_reordering* r, int* ks, int64_t* boffsets, int* first_rows,
+                                   int* second_reorder, int* second_perm, int* widths_left,
+                                   int* widths_right, int* perms_left, int* perms_right, double* wv,
+                                   double* offdiags, int* ks_col, int* ks_row, int* diag_row_offsets,
+                                   int* diag_col_indices, double* diag_values);
 
 /* Kernel-family timing since the last sap_reset_profile (profile = 1). */
 int         sap_get_profile(const sap_solver* s, sap_kernel_profile* out, int max_entries);

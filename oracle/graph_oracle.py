@@ -673,3 +673,214 @@ def csr_to_band(n, k, rp, ci, val):
         for e in range(rp[i], rp[i + 1]):
             band[i, k + int(ci[e]) - i] = val[e]
     return band
+
+
+# ----------------------------------------------------- variable-bandwidth assembly
+def _part_of(l, part_size, rem):
+    p = l // (part_size + 1)
+    if p >= rem:
+        p = rem + (l - rem * (part_size + 1)) // part_size
+    return p
+
+
+def _partitioned_rcm(n, drp, rows, cols, eb, ee, nb, ne, reord, perm, buf):
+    """Graph::partitionedRCM (sap/graph.h:1593-1757) on rows [nb, ne) of the
+    diagonal-block matrix (entries [eb, ee))."""
+    for i in range(nb, ne):
+        reord[i] = i
+    opt_bw = max((abs(rows[e] - cols[e]) for e in range(eb, ee)), default=0)
+    # buildTopology over this partition's entries
+    r2, c2 = [], []
+    for e in range(eb, ee):
+        if rows[e] != cols[e]:
+            r2 += [rows[e], cols[e]]
+            c2 += [cols[e], rows[e]]
+    trp = [0] * (n + 1)
+    for r in r2:
+        trp[r] += 1
+    run = 0
+    for i in range(nb, ne + 1):
+        run += trp[i]
+        trp[i] = run
+    tci = [0] * len(r2)
+    for e in range(len(r2) - 1, -1, -1):
+        trp[r2[e]] -= 1
+        tci[trp[r2[e]]] = c2[e]
+    tried = [False] * n
+    pushed = [-1] * ne
+    levels = [0] * n
+    # ori_degrees: the transform covers ne - nb - 1 rows (the last stays 0)
+    degs = [0] * (ne - nb)
+    for i in range(nb, ne - 1):
+        degs[i - nb] = drp[i + 1] - drp[i]
+    max_level = p_max_level = 0
+    for trial in range(5):
+        q, qh = [], 0
+        pq = LibstdcxxPQ()
+        if trial > 0:
+            seg = levels[nb:ne]
+            max_pos = nb + seg.index(max(seg))
+            max_count = seg.count(max_level)
+            if max_count > 1:
+                verts = [i for i in range(nb, ne) if levels[i] == max_level]
+                val = degs[:max_count]
+                node = verts[val.index(min(val))]
+            else:
+                node = max_pos
+            while tried[node]:
+                node = (node + 1) % n
+        else:
+            node = degs.index(min(degs)) + nb
+        tried[node] = True
+        levels[node] = 0
+        pushed[node] = trial
+        q.append(node)
+        left, j, last = ne - nb, nb, nb
+        while left > 0:
+            left -= 1
+            if qh == len(q):
+                left += 1
+                i = last
+                while i < ne:
+                    if pushed[i] != trial:
+                        q.append(i)
+                        pushed[i] = trial
+                        last = i
+                        break
+                    i += 1
+                if i < ne:
+                    continue
+                raise RuntimeError("partitioned RCM: no vertex left")
+            node = q[qh]
+            qh += 1
+            buf[j] = node
+            j += 1
+            for e in range(trp[node], trp[node + 1]):
+                t = tci[e]
+                if pushed[t] != trial:
+                    pushed[t] = trial
+                    pq.push(t, trp[t + 1] - trp[t])
+                    levels[t] = levels[node] + 1
+                    max_level = levels[t]
+            while not pq.empty():
+                q.append(pq.top()[0])
+                pq.pop()
+        for i in range(nb, ne):
+            perm[buf[i]] = i
+        bw = max((abs(perm[rows[e]] - perm[cols[e]]) for e in range(eb, ee)), default=0)
+        if opt_bw > bw:
+            opt_bw = bw
+            reord[nb:ne] = buf[nb:ne]
+        if trial > 0:
+            if p_max_level >= max_level:
+                break
+            if 1.0 * (max_level - p_max_level) / p_max_level < 0.01:
+                break
+            p_max_level = max_level
+        else:
+            p_max_level = max_level
+        if opt_bw <= 128:
+            break
+    for i in range(nb, ne):
+        perm[reord[i]] = i
+
+
+def assemble_var(n, k, P, rp, ci, val, prec="float"):
+    """The variable-bandwidth assembly of transformToBandedMatrix
+    (sap/precond.h:1785-1800): Graph::assembleOffDiagMatrices
+    (sap/graph.h:756-898), secondLevelReordering (:908-1008) with
+    partitionedRCM, assembleBandedMatrix for several partitions
+    (:1093-1206).  rp/ci/val: the reordered matrix (values already in T);
+    P: the clamped partition count (> 1)."""
+    T = _T(prec)
+    part, rem = n // P, n % P
+    kk = k * k
+    WV = [T(0)] * (2 * kk * (P - 1))
+    off = [T(0)] * (2 * kk * (P - 1))
+    wl, wr = [0] * (P - 1), [0] * (P - 1)
+    pl, pr = [-1] * ((P - 1) * k), [-1] * ((P - 1) * k)
+    rl, rr = [-1] * ((P - 1) * k), [-1] * ((P - 1) * k)
+    exists = [False] * n
+    drows, dcols, dvals = [], [], []
+    for j in range(n):
+        for e in range(rp[j], rp[j + 1]):
+            l = int(ci[e])
+            cp, rpn = _part_of(l, part, rem), _part_of(j, part, rem)
+            v = T(val[e])
+            if cp == rpn:
+                drows.append(j)
+                dcols.append(l)
+                dvals.append(v)
+            elif cp > rpn:
+                exists[j] = True
+                end_row = part * rpn + (rpn + part + 1 if rpn < rem else rem + part)
+                start_col = part * cp + (cp if cp < rem else rem)
+                s = rpn * k + l - start_col
+                if rr[s] < 0:
+                    rr[s] = wr[rpn]
+                    pr[rpn * k + wr[rpn]] = l - start_col
+                    wr[rpn] += 1
+                base = rpn * 2 * kk
+                off[base + (j + k - end_row) * k + (l - start_col)] = v
+                WV[base + (j + k - end_row) + rr[s] * k] = v
+            else:
+                start_row = part * rpn + (rpn if rpn < rem else rem)
+                end_col = part * cp + (cp + part + 1 if cp < rem else rem + part)
+                s = (rpn - 1) * k + l - end_col + k
+                if rl[s] < 0:
+                    rl[s] = k - 1 - wl[rpn - 1]
+                    pl[(rpn - 1) * k + k - 1 - wl[rpn - 1]] = l - end_col + k
+                    wl[rpn - 1] += 1
+                base = (cp * 2 + 1) * kk
+                off[base + (j - start_row) * k + (l - end_col + k)] = v
+                WV[base + (j - start_row) + rl[s] * k] = v
+    drp = [0] * (n + 1)
+    for r in drows:
+        drp[r + 1] += 1
+    for i in range(n):
+        drp[i + 1] += drp[i]
+    # second-level reordering
+    sre, sp, buf = [0] * n, [0] * n, [0] * n
+    nb = eb = 0
+    for i in range(P):
+        ne = nb + part + (1 if i < rem else 0)
+        ee = drp[ne]
+        _partitioned_rcm(n, drp, drows, dcols, eb, ee, nb, ne, sre, sp, buf)
+        nb, eb = ne, ee
+    nr = [sp[r] for r in drows]
+    nc = [sp[c] for c in dcols]
+    drp2, dci2, dval2 = _regroup_rows_backward(n, nr, nc, dvals)
+    first = [0] * (P - 1)
+    last_row = 0
+    for i in range(P - 1):
+        last_row += part + (1 if i < rem else 0)
+        first[i] = last_row
+        for j in range(last_row - k, last_row):
+            if exists[j] and first[i] > sp[j]:
+                first[i] = sp[j]
+    # assembleBandedMatrix (several partitions)
+    ks = [0] * P
+    kcol, krow = [0] * n, [0] * n
+    for j in range(n):
+        for e in range(drp2[j], drp2[j + 1]):
+            l = dci2[e]
+            cp = _part_of(l, part, rem)
+            ks[cp] = max(ks[cp], abs(l - j))
+            kcol[l] = max(kcol[l], j - l)
+            krow[j] = max(krow[j], l - j)
+    bo = [0] * (P + 1)
+    for i in range(P):
+        bo[i + 1] = bo[i] + (part + (1 if i < rem else 0)) * (2 * ks[i] + 1)
+    begin = 0
+    for i in range(P):
+        end = begin + part + (1 if i < rem else 0)
+        for j in range(begin + 1, end):
+            kcol[j] = min(max(kcol[j], kcol[j - 1] - 1), end - j - 1)
+            krow[j] = min(max(krow[j], krow[j - 1] - 1), end - j - 1)
+        begin = end
+    f64 = lambda a: np.array(a, dtype=T).astype(np.float64)
+    i32 = lambda a: np.array(a, dtype=np.int32)
+    return dict(ks=i32(ks), boffsets=np.array(bo, dtype=np.int64), first_rows=i32(first), second_reorder=i32(sre),
+                second_perm=i32(sp), widths_left=i32(wl), widths_right=i32(wr), perms_left=i32(pl),
+                perms_right=i32(pr), wv=f64(WV), offdiags=f64(off), ks_col=i32(kcol), ks_row=i32(krow),
+                drp=i32(drp2), dci=i32(dci2), dval=f64(dval2))

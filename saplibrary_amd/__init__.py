@@ -134,6 +134,7 @@ EXPORTS = (
     "sap_band_spmv", "sap_get_profile", "sap_reset_profile", "sap_get_factors", "sap_get_spikes",
     "sap_num_partitions", "sap_gen_banded", "sap_gen_vector", "sap_setup_csr", "sap_csr_spmv",
     "sap_reorder_csr", "sap_reordering_info", "sap_reordering_get", "sap_reordering_free", "sap_update",
+    "sap_reordering_assemble_var", "sap_reordering_var_get",
 )
 
 _lib = None
@@ -183,6 +184,9 @@ def lib():
     L.sap_reordering_get.argtypes = [vp, ip, ip, ip, dp, dp, ip, ip, dp]
     L.sap_reordering_free.argtypes = [vp]
     L.sap_reordering_free.restype = None
+    L.sap_reordering_assemble_var.argtypes = [vp, i]
+    L.sap_reordering_var_get.argtypes = [vp, ip, ctypes.POINTER(i64), ip, ip, ip, ip, ip, ip, ip, dp, dp, ip, ip,
+                                         ip, ip, dp]
     _lib = L
     return L
 
@@ -287,7 +291,7 @@ class SpmvCusp:
         csr_spmv(self.A, v, Av)
 
 
-def reorder_csr(row_offsets, column_indices, values, opts=None):
+def reorder_csr(row_offsets, column_indices, values, opts=None, numPartitions=None):
     """The host reordering of a CSR setup (DB, RCM, drop-off; sap/graph.h:446-747)
     on host arrays, without a GPU.  Returns a dict with the permutations
     (db_perm, row_perm, col_reord), scalings, bandwidths (k, k_db, k_reorder),
@@ -325,6 +329,32 @@ def reorder_csr(row_offsets, column_indices, values, opts=None):
         out["ci"] = out["ci"][:nnz.value]
         out["val"] = out["val"][:nnz.value]
         out.update(k=k.value, k_db=kdb.value, k_reorder=kre.value, dropoff=drop.value)
+        if numPartitions is not None:
+            P = L.sap_reordering_assemble_var(r, numPartitions)
+            _check(P)
+            out["P"] = P
+            if P > 1:
+                kk = k.value
+                var = dict(ks=np.empty(P, dtype=np.int32), boffsets=np.empty(P + 1, dtype=np.int64),
+                           first_rows=np.empty(P - 1, dtype=np.int32), second_reorder=np.empty(n, dtype=np.int32),
+                           second_perm=np.empty(n, dtype=np.int32), widths_left=np.empty(P - 1, dtype=np.int32),
+                           widths_right=np.empty(P - 1, dtype=np.int32),
+                           perms_left=np.empty((P - 1) * kk, dtype=np.int32),
+                           perms_right=np.empty((P - 1) * kk, dtype=np.int32),
+                           wv=np.empty(2 * kk * kk * (P - 1)), offdiags=np.empty(2 * kk * kk * (P - 1)),
+                           ks_col=np.empty(n, dtype=np.int32), ks_row=np.empty(n, dtype=np.int32),
+                           drp=np.empty(n + 1, dtype=np.int32), dci=np.empty(nnz.value, dtype=np.int32),
+                           dval=np.empty(max(nnz.value, 1)))
+                names = ("ks", "boffsets", "first_rows", "second_reorder", "second_perm", "widths_left",
+                         "widths_right", "perms_left", "perms_right", "wv", "offdiags", "ks_col", "ks_row", "drp",
+                         "dci", "dval")
+                ptrs = [var[f].ctypes.data_as(ctypes.POINTER(ctypes.c_int64) if f == "boffsets" else
+                                              dp if var[f].dtype == np.float64 else ip) for f in names]
+                _check(L.sap_reordering_var_get(r, *ptrs))
+                nd = int(var["drp"][-1])
+                var["dci"] = var["dci"][:nd]
+                var["dval"] = var["dval"][:nd]
+                out["var"] = var
         return out
     finally:
         L.sap_reordering_free(r)

@@ -668,6 +668,262 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
     return R;
 }
 
+namespace {
+
+// partition of index l: the first n mod P partitions hold n/P + 1 rows
+inline int part_of(int l, int part, int rem)
+{
+    int p = l / (part + 1);
+    if (p >= rem) p = rem + (l - rem * (part + 1)) / part;
+    return p;
+}
+
+// Per-partition RCM (Graph::partitionedRCM, sap/graph.h:1593-1757) over the
+// diagonal-block graph: rows [nb, ne), their entries [eb, ee).
+void partitioned_rcm(int n, const std::vector<int>& drp, const std::vector<int>& rows, const std::vector<int>& cols,
+                     int eb, int ee, int nb, int ne, std::vector<int>& reord, std::vector<int>& perm,
+                     std::vector<int>& buf)
+{
+    for (int i = nb; i < ne; ++i) reord[i] = i;
+    int opt_bw = 0;
+    for (int e = eb; e < ee; ++e) opt_bw = std::max(opt_bw, std::abs(rows[e] - cols[e]));
+    // topology: both directions of every off-diagonal entry, grouped by
+    // row (backward fill), offsets absolute by node
+    std::vector<int> trp(n + 1, 0), tci;
+    {
+        std::vector<int> r2, c2;
+        for (int e = eb; e < ee; ++e) {
+            if (rows[e] == cols[e]) continue;
+            r2.push_back(rows[e]); c2.push_back(cols[e]);
+            r2.push_back(cols[e]); c2.push_back(rows[e]);
+        }
+        for (int r : r2) trp[r]++;
+        int run = 0;
+        for (int i = nb; i <= ne; ++i) { run += trp[i]; trp[i] = run; }
+        tci.assign(r2.size(), 0);
+        for (int e = (int)r2.size() - 1; e >= 0; --e) tci[--trp[r2[e]]] = c2[e];
+    }
+    const int MAX_NUM_TRIAL = 5, BANDWIDTH_THRESHOLD = 128;
+    std::vector<char> tried(n, 0);
+    // degrees of the partition's rows in the diagonal-block matrix; the
+    // reference's transform covers ne - nb - 1 of them, so the last row's
+    // degree stays 0 (sap/graph.h:1627)
+    std::vector<int> pushed(ne, -1), levels(n, 0), degs(ne - nb, 0);
+    for (int i = nb; i < ne - 1; ++i) degs[i - nb] = drp[i + 1] - drp[i];
+    int max_level = 0, p_max_level = 0;   // the reference leaves them unset until the first push
+    for (int trial = 0; trial < MAX_NUM_TRIAL; ++trial) {
+        std::queue<int> q;
+        std::priority_queue<NodeKey<int>, std::vector<NodeKey<int>>, KeyGreater<int>> pq;
+        int node;
+        if (trial > 0) {
+            const int max_pos = (int)(std::max_element(levels.begin() + nb, levels.begin() + ne) - levels.begin());
+            const int max_count = (int)std::count(levels.begin() + nb, levels.begin() + ne, max_level);
+            if (max_count > 1) {
+                std::vector<int> verts;
+                for (int i = nb; i < ne; ++i)
+                    if (levels[i] == max_level) verts.push_back(i);
+                const int pos = (int)(std::min_element(degs.begin(), degs.begin() + max_count) - degs.begin());
+                node = verts[pos];
+            } else {
+                node = max_pos;
+            }
+            while (tried[node]) node = (node + 1) % n;
+        } else {
+            node = (int)(std::min_element(degs.begin(), degs.end()) - degs.begin()) + nb;
+        }
+        tried[node] = 1;
+        levels[node] = 0;
+        pushed[node] = trial;
+        q.push(node);
+        int left = ne - nb, j = nb, last = nb;
+        while (left--) {
+            if (q.empty()) {
+                left++;
+                int i;
+                for (i = last; i < ne; ++i) {
+                    if (pushed[i] != trial) {
+                        q.push(i);
+                        pushed[i] = trial;
+                        last = i;
+                        break;
+                    }
+                }
+                if (i < ne) continue;
+                throw Error(SAP_ERR_HIP, "partitioned RCM ran out of vertices");
+            }
+            node = q.front();
+            q.pop();
+            buf[j++] = node;
+            for (int e = trp[node]; e < trp[node + 1]; ++e) {
+                const int t = tci[e];
+                if (pushed[t] != trial) {
+                    pushed[t] = trial;
+                    pq.push({t, trp[t + 1] - trp[t]});
+                    max_level = levels[t] = levels[node] + 1;
+                }
+            }
+            while (!pq.empty()) {
+                q.push(pq.top().node);
+                pq.pop();
+            }
+        }
+        for (int i = nb; i < ne; ++i) perm[buf[i]] = i;
+        int bw = 0;
+        for (int e = eb; e < ee; ++e) bw = std::max(bw, std::abs(perm[rows[e]] - perm[cols[e]]));
+        if (opt_bw > bw) {
+            opt_bw = bw;
+            std::copy(buf.begin() + nb, buf.begin() + ne, reord.begin() + nb);
+        }
+        if (trial > 0) {
+            if (p_max_level >= max_level) break;
+            const double ratio = 1.0 * (max_level - p_max_level) / p_max_level;
+            if (ratio < 0.01) break;
+            p_max_level = max_level;
+        } else {
+            p_max_level = max_level;
+        }
+        if (opt_bw <= BANDWIDTH_THRESHOLD) break;
+    }
+    for (int i = nb; i < ne; ++i) perm[reord[i]] = i;
+}
+
+}  // namespace
+
+template <class T>
+VarBand<T> assemble_var(int n, int k, int P, const std::vector<int>& rp, const std::vector<int>& ci,
+                        const std::vector<T>& val)
+{
+    VarBand<T> V;
+    V.n = n;
+    V.k = k;
+    V.P = P;
+    const int part = n / P, rem = n % P;
+    const size_t kk = (size_t)k * k;
+
+    // (1) Graph::assembleOffDiagMatrices (sap/graph.h:756-898)
+    V.WV.assign(2 * kk * (P - 1), (T)0);
+    V.offDiags.assign(2 * kk * (P - 1), (T)0);
+    V.widths_left.assign(P - 1, 0);
+    V.widths_right.assign(P - 1, 0);
+    V.perms_left.assign((size_t)(P - 1) * k, -1);
+    V.perms_right.assign((size_t)(P - 1) * k, -1);
+    std::vector<int> reord_left((size_t)(P - 1) * k, -1), reord_right((size_t)(P - 1) * k, -1);
+    V.exists.assign(n, 0);
+    std::vector<int> drows;
+    V.dci.clear();
+    V.dval.clear();
+    for (int j = 0; j < n; ++j) {
+        for (int e = rp[j]; e < rp[j + 1]; ++e) {
+            const int l = ci[e];
+            const int cp = part_of(l, part, rem), rpn = part_of(j, part, rem);
+            if (cp == rpn) {
+                drows.push_back(j);
+                V.dci.push_back(l);
+                V.dval.push_back(val[e]);
+            } else if (cp > rpn) {   // V / B block: row in partition rpn, column in cp
+                V.exists[j] = 1;
+                const int end_row = part * rpn + (rpn < rem ? rpn + part + 1 : rem + part);
+                const int start_col = part * cp + (cp < rem ? cp : rem);
+                const size_t slot = (size_t)rpn * k + (l - start_col);
+                if (reord_right[slot] < 0) {
+                    reord_right[slot] = V.widths_right[rpn];
+                    V.perms_right[(size_t)rpn * k + V.widths_right[rpn]] = l - start_col;
+                    V.widths_right[rpn]++;
+                }
+                const size_t base = (size_t)rpn * 2 * kk;
+                V.offDiags[base + (size_t)(j + k - end_row) * k + (l - start_col)] = val[e];
+                V.WV[base + (size_t)(j + k - end_row) + (size_t)reord_right[slot] * k] = val[e];
+            } else {                 // W / C block: row in partition rpn, column in cp < rpn
+                const int start_row = part * rpn + (rpn < rem ? rpn : rem);
+                const int end_col = part * cp + (cp < rem ? cp + part + 1 : rem + part);
+                const size_t slot = (size_t)(rpn - 1) * k + (l - end_col + k);
+                if (reord_left[slot] < 0) {
+                    reord_left[slot] = k - 1 - V.widths_left[rpn - 1];
+                    V.perms_left[(size_t)(rpn - 1) * k + k - 1 - V.widths_left[rpn - 1]] = l - end_col + k;
+                    V.widths_left[rpn - 1]++;
+                }
+                const size_t base = (size_t)(cp * 2 + 1) * kk;
+                V.offDiags[base + (size_t)(j - start_row) * k + (l - end_col + k)] = val[e];
+                V.WV[base + (size_t)(j - start_row) + (size_t)reord_left[slot] * k] = val[e];
+            }
+        }
+    }
+    V.drp.assign(n + 1, 0);
+    for (int r : drows) V.drp[r + 1]++;
+    for (int i = 0; i < n; ++i) V.drp[i + 1] += V.drp[i];
+
+    // (2) secondLevelReordering (sap/graph.h:908-1008)
+    V.second_reorder.assign(n, 0);
+    V.second_perm.assign(n, 0);
+    {
+        std::vector<int> buf(n, 0);
+        int nb = 0, eb = 0;
+        for (int i = 0; i < P; ++i) {
+            const int ne = nb + part + (i < rem ? 1 : 0);
+            const int ee = V.drp[ne];
+            partitioned_rcm(n, V.drp, drows, V.dci, eb, ee, nb, ne, V.second_reorder, V.second_perm, buf);
+            nb = ne;
+            eb = ee;
+        }
+        // permute the diagonal blocks and regroup their rows (backward fill)
+        const size_t dn = V.dci.size();
+        std::vector<int> nr(dn), nc(dn);
+        for (size_t e = 0; e < dn; ++e) {
+            nr[e] = V.second_perm[drows[e]];
+            nc[e] = V.second_perm[V.dci[e]];
+        }
+        regroup_rows(n, nr, nc, V.dval, V.drp);
+        V.dci.swap(nc);
+    }
+    V.first_rows.assign(P - 1, 0);
+    {
+        int last_row = 0;
+        for (int i = 0; i < P - 1; ++i) {
+            last_row += part + (i < rem ? 1 : 0);
+            V.first_rows[i] = last_row;
+            for (int j = last_row - k; j < last_row; ++j)
+                if (j >= 0 && V.exists[j] && V.first_rows[i] > V.second_perm[j]) V.first_rows[i] = V.second_perm[j];
+        }
+    }
+
+    // (3) assembleBandedMatrix, several partitions (sap/graph.h:1093-1206)
+    V.ks.assign(P, 0);
+    for (int j = 0; j < n; ++j)
+        for (int e = V.drp[j]; e < V.drp[j + 1]; ++e) {
+            const int l = V.dci[e];
+            const int cp = part_of(l, part, rem);
+            V.ks[cp] = std::max(V.ks[cp], std::abs(l - j));
+        }
+    V.boffsets.assign(P + 1, 0);
+    for (int i = 0; i < P; ++i)
+        V.boffsets[i + 1] = V.boffsets[i] + (int64_t)(part + (i < rem ? 1 : 0)) * (2 * V.ks[i] + 1);
+    V.ks_col.assign(n, 0);
+    V.ks_row.assign(n, 0);
+    for (int j = 0; j < n; ++j)
+        for (int e = V.drp[j]; e < V.drp[j + 1]; ++e) {
+            const int l = V.dci[e];
+            V.ks_col[l] = std::max(V.ks_col[l], j - l);
+            V.ks_row[j] = std::max(V.ks_row[j], l - j);
+        }
+    {
+        int begin = 0;
+        for (int i = 0; i < P; ++i) {
+            const int end = begin + part + (i < rem ? 1 : 0);
+            for (int j = begin + 1; j < end; ++j) {
+                V.ks_col[j] = std::min(std::max(V.ks_col[j], V.ks_col[j - 1] - 1), end - j - 1);
+                V.ks_row[j] = std::min(std::max(V.ks_row[j], V.ks_row[j - 1] - 1), end - j - 1);
+            }
+            begin = end;
+        }
+    }
+    return V;
+}
+
+template VarBand<float> assemble_var<float>(int, int, int, const std::vector<int>&, const std::vector<int>&,
+                                            const std::vector<float>&);
+template VarBand<double> assemble_var<double>(int, int, int, const std::vector<int>&, const std::vector<int>&,
+                                              const std::vector<double>&);
+
 BandProfile band_profile(int n, const int* rp, const int* ci, int P)
 {
     std::vector<int> ks_row(n, 0), ks_col(n, 0);

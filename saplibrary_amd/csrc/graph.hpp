@@ -40,6 +40,35 @@ struct Reordering {
     double time_DB = 0, time_reorder = 0, time_dropOff = 0;
 };
 
+// The variable-bandwidth assembly of a reordered matrix into P partitions
+// (transformToBandedMatrix's variable-bandwidth branch, sap/precond.h:
+// 1785-1800, and Graph::assembleOffDiagMatrices, secondLevelReordering +
+// partitionedRCM, assembleBandedMatrix, sap/graph.h:756-1206, :1593-1757).
+template <class T>
+struct VarBand {
+    int n = 0, k = 0, P = 1;
+    // coupling blocks: WV holds them with compressed columns (the spike
+    // right-hand sides), offDiags as they are; 2 k^2 per boundary
+    std::vector<T> WV, offDiags;
+    std::vector<int> widths_left, widths_right;   // used columns per boundary
+    std::vector<int> perms_left, perms_right;     // (P-1) k column maps, -1 unused
+    std::vector<char> exists;                     // rows with a coupling entry to the right
+    std::vector<int> second_reorder, second_perm; // per-partition RCM (new -> old, old -> new)
+    std::vector<int> first_rows;                  // P-1
+    std::vector<int> ks;                          // per-partition half-bandwidths
+    std::vector<int64_t> boffsets;                // P+1 band offsets
+    std::vector<int> ks_col, ks_row;              // per column / row extents
+    // diagonal-block entries after the second-level permutation (CSR in T)
+    std::vector<int> drp, dci;
+    std::vector<T> dval;
+};
+
+// rp/ci/val: the reordered, dropped-off matrix (Reordering::rp/ci/val), k its
+// half-bandwidth, P the partition count (already clamped, > 1).
+template <class T>
+VarBand<T> assemble_var(int n, int k, int P, const std::vector<int>& rp, const std::vector<int>& ci,
+                        const std::vector<T>& val);
+
 // Sums over the per-row / per-column band extents ks_row, ks_col of the
 // assembled band (Graph::assembleBandedMatrix, sap/graph.h:1019-1091 for one
 // partition, :1093-1206 over the diagonal blocks for several), from which

@@ -1010,6 +1010,9 @@ struct sap_reordering {
     int precision = SAP_PREC_FLOAT;
     sap_amd::Reordering<float> f;
     sap_amd::Reordering<double> d;
+    bool has_var = false;
+    sap_amd::VarBand<float> vf;
+    sap_amd::VarBand<double> vd;
 };
 
 sap_reordering* sap_reorder_csr(int n, const int* h_row_offsets, const int* h_col_indices, const double* h_values,
@@ -1083,6 +1086,53 @@ int sap_reordering_get(const sap_reordering* r, int* db_perm, int* row_perm, int
             for (size_t e = 0; e < m; ++e) values[e] = (double)R.val[e];
     };
     if (r->precision == SAP_PREC_FLOAT) get(r->f); else get(r->d);
+    return SAP_OK;
+}
+
+int sap_reordering_assemble_var(sap_reordering* r, int numPartitions)
+{
+    if (!r || numPartitions < 1) return SAP_ERR_BADARG;
+    return guarded(nullptr, [&] {
+        auto run = [&](const auto& R, auto& V) {
+            // partition clamp of the reordering path (sap/precond.h:1777-1783)
+            const int P = std::min(numPartitions, std::max(R.n / (R.k + 1), 1));
+            r->has_var = P > 1 && R.k > 0;
+            if (r->has_var) V = sap_amd::assemble_var(R.n, R.k, P, R.rp, R.ci, R.val);
+            return r->has_var ? P : 1;
+        };
+        return r->precision == SAP_PREC_FLOAT ? run(r->f, r->vf) : run(r->d, r->vd);
+    });
+}
+
+int sap_reordering_var_get(const sap_reordering* r, int* ks, int64_t* boffsets, int* first_rows,
+                           int* second_reorder, int* second_perm, int* widths_left, int* widths_right,
+                           int* perms_left, int* perms_right, double* wv, double* offdiags, int* ks_col,
+                           int* ks_row, int* diag_row_offsets, int* diag_col_indices, double* diag_values)
+{
+    if (!r || !r->has_var) return SAP_ERR_BADARG;
+    auto get = [&](const auto& V) {
+        auto cp = [](const auto& src, auto* dst) {
+            if (dst)
+                for (size_t i = 0; i < src.size(); ++i) dst[i] = src[i];
+        };
+        cp(V.ks, ks);
+        cp(V.boffsets, boffsets);
+        cp(V.first_rows, first_rows);
+        cp(V.second_reorder, second_reorder);
+        cp(V.second_perm, second_perm);
+        cp(V.widths_left, widths_left);
+        cp(V.widths_right, widths_right);
+        cp(V.perms_left, perms_left);
+        cp(V.perms_right, perms_right);
+        cp(V.WV, wv);
+        cp(V.offDiags, offdiags);
+        cp(V.ks_col, ks_col);
+        cp(V.ks_row, ks_row);
+        cp(V.drp, diag_row_offsets);
+        cp(V.dci, diag_col_indices);
+        cp(V.dval, diag_values);
+    };
+    if (r->precision == SAP_PREC_FLOAT) get(r->vf); else get(r->vd);
     return SAP_OK;
 }
 

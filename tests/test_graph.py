@@ -285,3 +285,94 @@ def test_product_reordering_rejects_singular(sap):
     with pytest.raises(sap.SapError) as ei:
         sap.reorder_csr(rp, ci, val)
     assert ei.value.reason == -5
+
+
+# ------------------------------------------------ variable-bandwidth assembly
+VAR_CASES = [(2000, 60, 6, 51, 8), (1500, 40, 5, 52, 5), (1200, 90, 7, 53, 3), (900, 30, 4, 54, 16)]
+
+
+def _var_ref(n, width, per_row, seed, P, prec="float"):
+    rp, ci, val = sparse_band(n, width, per_row, seed)
+    ref = G.reorder(rp, ci, val, prec=prec)
+    Pc = min(P, max(n // (ref["k"] + 1), 1))
+    return rp, ci, val, ref, Pc, G.assemble_var(n, ref["k"], Pc, ref["rp"], ref["ci"], ref["val"], prec=prec)
+
+
+@pytest.mark.parametrize("case", VAR_CASES, ids=[f"n{c[0]}P{c[4]}" for c in VAR_CASES])
+def test_var_assembly_invariants(case):
+    n, width, per_row, seed, P = case
+    rp, ci, val, ref, P, v = _var_ref(n, width, per_row, seed, P, prec="double")
+    k = ref["k"]
+    part, rem = n // P, n % P
+    sizes = np.array([part + (1 if i < rem else 0) for i in range(P)])
+    starts = np.r_[0, np.cumsum(sizes)]
+    # band offsets: prefix sums of n_p (2 k_p + 1)
+    assert np.array_equal(v["boffsets"], np.r_[0, np.cumsum(sizes * (2 * v["ks"].astype(np.int64) + 1))])
+    # the second-level permutation maps every partition onto itself
+    sp, sr = v["second_perm"], v["second_reorder"]
+    assert np.array_equal(sp[sr], np.arange(n))
+    pof = np.searchsorted(starts, np.arange(n), side="right") - 1
+    assert np.array_equal(pof[sp], pof)
+    # every entry of the reordered matrix is either in a diagonal block (at
+    # its permuted place) or in a coupling block (as it is), nowhere else
+    rows = np.repeat(np.arange(n), np.diff(ref["rp"]))
+    cols = ref["ci"]
+    same = pof[rows] == pof[cols]
+    D = {}
+    drows = np.repeat(np.arange(n), np.diff(v["drp"]))
+    for r, c, x in zip(drows, v["dci"], v["dval"]):
+        D[(int(r), int(c))] = x
+    assert len(D) == int(same.sum())
+    for r, c, x in zip(rows[same], cols[same], ref["val"][same]):
+        assert D[(int(sp[r]), int(sp[c]))] == x
+    kk = k * k
+    nrb = 0
+    for r, c, x in zip(rows[~same], cols[~same], ref["val"][~same]):
+        pr, pc = pof[r], pof[c]
+        if pc > pr:   # B_i: bottom rows of partition pr, first columns of pc
+            idx = pr * 2 * kk + (r + k - starts[pr + 1]) * k + (c - starts[pc])
+        else:         # C_{i+1}: top rows of partition pr, last columns of pc
+            idx = (pc * 2 + 1) * kk + (r - starts[pr]) * k + (c - starts[pc + 1] + k)
+        assert v["offdiags"][idx] == x
+        nrb += 1
+    assert np.count_nonzero(v["offdiags"]) == nrb
+    # WV holds the same blocks with their used columns packed: right blocks
+    # from the left, left blocks from the right, perms giving the column
+    for i in range(P - 1):
+        for side, widths, perms in ((0, v["widths_right"], v["perms_right"]), (1, v["widths_left"], v["perms_left"])):
+            base = (2 * i + side) * kk
+            Off = v["offdiags"][base:base + kk].reshape(k, k)           # [row][col]
+            W = v["wv"][base:base + kk].reshape(k, k)                    # [slot][row]
+            used = np.nonzero(np.any(Off != 0, axis=0))[0]
+            assert widths[i] == used.size
+            slots = range(widths[i]) if side == 0 else range(k - widths[i], k)
+            pc = perms[i * k:(i + 1) * k]
+            assert sorted(int(pc[s]) for s in slots) == sorted(used.tolist())
+            for s in slots:
+                assert np.array_equal(W[s], Off[:, pc[s]])
+    # half-bandwidth of each diagonal block, and the per-partition RCM never
+    # widens it
+    for p in range(P):
+        m = (pof[drows] == p)
+        assert v["ks"][p] == (np.max(np.abs(drows[m] - v["dci"][m])) if m.any() else 0)
+        m0 = same & (pof[rows] == p)
+        assert v["ks"][p] <= (np.max(np.abs(rows[m0] - cols[m0])) if m0.any() else 0)
+    # first_rows: no later than the partition end, and at the smallest
+    # permuted row among the last k rows with a coupling entry to the right
+    exists = np.zeros(n, dtype=bool)
+    exists[rows[~same & (pof[cols] > pof[rows])]] = True
+    for i in range(P - 1):
+        lr = starts[i + 1]
+        cand = [int(sp[j]) for j in range(lr - k, lr) if exists[j]]
+        assert v["first_rows"][i] == min([lr] + cand)
+
+
+@pytest.mark.parametrize("prec", ["float", "double"])
+@pytest.mark.parametrize("case", VAR_CASES, ids=[f"n{c[0]}P{c[4]}" for c in VAR_CASES])
+def test_product_var_assembly_is_bit_exact(sap, case, prec):
+    n, width, per_row, seed, P = case
+    rp, ci, val, ref, Pc, v = _var_ref(n, width, per_row, seed, P, prec=prec)
+    got = sap.reorder_csr(rp, ci, val, _options(sap, {}, prec), numPartitions=P)
+    assert got["P"] == Pc
+    for f, x in v.items():
+        assert np.array_equal(got["var"][f], x), f
