@@ -134,8 +134,7 @@ public:
         return true;
     }
 
-    // CSR input: DB / RCM / drop-off as the options ask (needs
-    // variableBandwidth = false when performReorder is set).
+    // CSR input: DB / RCM / drop-off as the options ask.
     template <class I, class V>
     bool setup(const CsrMatrix<I, V>& M)
     {

@@ -187,10 +187,13 @@ int         sap_setup_banded(sap_solver* s, int n, int k, const double* d_band);
  * permutes and scales its vectors (sap/precond.h:1393-1406, :1535-1621).
  * Without performReorder the matrix is converted to a band of half-width
  * max|i-j| as is (sap/precond.h:2240-2363).
+ * With variableBandwidth (and more than one partition) the preconditioner
+ * is the variable-bandwidth path's truncated-SPIKE operator, factored
+ * LU_only with every pivot boosted, on the constant-width band of the
+ * reordered matrix.
  * Returns SAP_ERR_SINGULAR when DB finds no perfect matching,
  * SAP_ERR_NEG_DB_WEIGHT on a negative reduced cost, SAP_ERR_UNSUPPORTED for
- * testDB or for performReorder with variableBandwidth (the
- * variable-bandwidth preconditioner is not provided). */
+ * testDB. */
 int         sap_setup_csr(sap_solver* s, int n, int nnz, const int* d_row_offsets,
                           const int* d_col_indices, const double* d_values);
 

@@ -95,7 +95,7 @@ void Precond<T>::begin_setup(int64_t n_, int k_)
 template <class T>
 int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
 {
-    reorder_ = scale_ = false;
+    reorder_ = scale_ = var_mode_ = false;
     row_perm_.release(); col_reord_.release(); row_scale_.release(); col_scale_.release();
     vp_.release(); zp_.release();
     k_db = k_reorder = 0;
@@ -122,6 +122,7 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
                           const ReorderConfig& rc)
 {
     Events ev;
+    var_mode_ = false;
     if (!rc.reorder) {
         reorder_ = scale_ = false;
         row_perm_.release(); col_reord_.release(); row_scale_.release(); col_scale_.release();
@@ -183,10 +184,33 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
     k_reorder = R.k_reorder;
     dropoff_actual = R.dropoff_actual;
 
-    // (3) band of the reordered matrix, vectors of the permutations
+    // (3) variable bandwidth (sap/precond.h:1777-1800): with more than one
+    //     partition (clamped to n / (k+1)) and k > 0 the reference builds the
+    //     variable-bandwidth preconditioner.  It is the same truncated-SPIKE
+    //     operator (diagonal blocks, their k x k spike tips, the reduced
+    //     blocks, purification with the coupling blocks); this build factors
+    //     the diagonal blocks on the constant-width band of the reordered
+    //     matrix instead of after the per-partition second-level RCM, LU_only
+    //     with every pivot boosted as the reference's variable-bandwidth
+    //     kernels do.  The host assembly runs for its band extents
+    //     (Stats::nuKf, flops_LU).
+    const int Pvar = std::min(cfg.numPartitions, std::max((int)(n_ / (R.k + 1)), 1));
+    var_mode_ = rc.variable && Pvar > 1 && R.k > 0;
+
+    // (4) band of the reordered matrix, vectors of the permutations
     begin_setup(n_, R.k);
     t = wall_now_ms();
-    profile = band_profile((int)n, R.rp.data(), R.ci.data(), P);
+    if (var_mode_) {
+        const VarBand<T> VB = assemble_var((int)n, R.k, Pvar, R.rp, R.ci, R.val);
+        profile = BandProfile{};
+        for (int i = 0; i < (int)n; ++i) {
+            profile.sum_row += VB.ks_row[i];
+            profile.sum_col += VB.ks_col[i];
+            profile.sum_prod += (double)VB.ks_row[i] * VB.ks_col[i];
+        }
+    } else {
+        profile = band_profile((int)n, R.rp.data(), R.ci.data(), P);
+    }
     time_cpu_assemble = wall_now_ms() - t;
     t = wall_now_ms();
     DevBuf<int> rp_d, ci_d;
@@ -289,8 +313,10 @@ int Precond<T>::factor_band()
         return 0;
     }
 
-    const int boost = (k >= 70) || cfg.safeFactorization;   // sap/precond.h:4363-4397
-    const int boost_last = (k >= 70);
+    // sap/precond.h:4363-4397; the variable-bandwidth kernels boost every
+    // pivot (sap/device/factor_band_var.cuh)
+    const int boost = (k >= 70) || cfg.safeFactorization || var_mode_;
+    const int boost_last = (k >= 70) || var_mode_;
 
     if (cfg.precondType == SAP_Block || P == 1) {
         std::vector<FactorJob> jobs;

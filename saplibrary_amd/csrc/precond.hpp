@@ -24,6 +24,7 @@ struct ReorderConfig {
     int maxBandwidth = 0x7fffffff;
     double dropOffFraction = 0;
     bool track = false;            // trackReordering: keep what update() needs
+    bool variable = false;         // variableBandwidth (LU_only, every pivot boosted)
 };
 
 template <class T>
@@ -71,7 +72,8 @@ public:
     bool update(const double* d_entries, int64_t count, int* rc);
 
     bool spike_active() const { return cfg.precondType == SAP_Spike && P > 1 && k > 0; }
-    bool lu_ul() const { return cfg.factMethod == SAP_LU_UL; }
+    bool lu_ul() const { return cfg.factMethod == SAP_LU_UL && !var_mode_; }
+    bool variable_mode() const { return var_mode_; }
 
     // diagnostics
     const DevBuf<T>& factors() const { return B_; }
@@ -85,6 +87,9 @@ private:
     DevBuf<int> flag_;
     // reordering: row permutation (DB then RCM), column reordering, scalings
     bool reorder_ = false, scale_ = false;
+    // variable-bandwidth setup: the preconditioner of the variable-bandwidth
+    // path is factored LU_only with every pivot boosted
+    bool var_mode_ = false;
     DevBuf<int> row_perm_, col_reord_;
     DevBuf<T> row_scale_, col_scale_;
     DevBuf<double> vp_, zp_;

@@ -605,6 +605,7 @@ public:
         rc.maxBandwidth = opt.maxBandwidth;
         rc.dropOffFraction = opt.dropOffFraction;
         rc.track = opt.trackReordering != 0;
+        rc.variable = opt.variableBandwidth != 0;
         int r = pf ? pf->setup_csr(n, nnz, rp, ci, val, rc) : pd->setup_csr(n, nnz, rp, ci, val, rc);
         SAP_HIP_CHECK(hipStreamSynchronize(stream));
         if (r != 0) return r;
@@ -842,10 +843,6 @@ int sap_setup_csr(sap_solver* h, int n, int nnz, const int* d_row_offsets, const
     }
     const sap_options& o = h->s->opt;
     if (o.testDB) { h->err = "testDB is not supported"; return SAP_ERR_UNSUPPORTED; }
-    if (o.performReorder && o.variableBandwidth) {
-        h->err = "the variable-bandwidth preconditioner is not implemented; set variableBandwidth = 0";
-        return SAP_ERR_UNSUPPORTED;
-    }
     return guarded(h, [&] { return h->s->setup_csr(n, nnz, d_row_offsets, d_col_indices, d_values); });
 }
 

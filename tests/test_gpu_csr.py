@@ -168,7 +168,7 @@ def test_csr_solve_sparse_workload(sap):
     # (two partitions: this matrix is far from diagonally dominant and its
     # reordered band is wide, so truncated SPIKE stops converging beyond
     # P = 2 -- the oracle's SaP behaves the same on the reordered band)
-    opts = sap.Options(variableBandwidth=0, relTol=1e-10, refine=8, factMethod=sap.LU_only)
+    opts = sap.Options(relTol=1e-10, refine=8)      # reordering, DB, scaling, variable bandwidth
     g = sap.Solver(2, opts)
     g.setup(sap.CsrMatrix.from_host(rp, ci, val))
     x = torch.zeros(n, dtype=torch.float64, device="cuda")
@@ -180,12 +180,52 @@ def test_csr_solve_sparse_workload(sap):
     assert st.numSpmv > 0 and g.numPartitions == 2
 
 
+@pytest.mark.parametrize("prec", ["float", "double"])
+def test_csr_variable_bandwidth_matches_oracle(sap, oracle, prec):
+    """Default options (variableBandwidth, LU_only): the truncated-SPIKE
+    operator of the reordered matrix, so the oracle's LU_only preconditioner
+    on the reordered band within the tolerances above; nuKf from the
+    variable-bandwidth band extents (sap/solver.h:358-394)."""
+    n, P = 1500, 4
+    rp, ci, val = sparse_band(n, 60, 6, 56)
+    ref = G.reorder(rp, ci, val, prec=prec)
+    T = np.float32 if prec == "float" else np.float64
+    k = ref["k"]
+    band = G.csr_to_band(n, k, ref["rp"], ref["ci"], ref["val"])
+    cpu = oracle.Solver(P, fact="LU_only", prec_double=(prec == "double"))
+    assert cpu.setup(band) == 0
+    g = sap.Solver(P, sap.Options(precValueType=sap.PREC_FLOAT if prec == "float" else sap.PREC_DOUBLE))
+    g.setup(sap.CsrMatrix.from_host(rp, ci, val))
+    st = g.getStats()
+    assert st.bandwidth == k and g.numPartitions == cpu.num_partitions
+    Pv = min(P, max(n // (k + 1), 1))
+    var = G.assemble_var(n, k, Pv, ref["rp"], ref["ci"], ref["val"], prec=prec)
+    nukf = (2.0 * k * n - (var["ks_row"].sum() + var["ks_col"].sum())) / (2.0 * k * n)
+    assert st.nuKf == pytest.approx(nukf, rel=1e-12)
+    v = oracle.gen_vector(n, 57)
+    w = np.zeros(n)
+    w[ref["row_perm"]] = (v.astype(T) * ref["row_scale"].astype(T)).astype(np.float64)
+    zr = cpu.apply(w)
+    expect = np.zeros(n)
+    expect[ref["col_reord"]] = (zr.astype(T) * ref["col_scale"][ref["col_reord"]].astype(T)).astype(np.float64)
+    z = torch.empty(n, dtype=torch.float64, device="cuda")
+    g.precond_apply(dev(v), z)
+    if prec == "double":
+        atol = 1e-10 * np.abs(expect).max()
+    else:
+        cpu64 = oracle.Solver(P, fact="LU_only", prec_double=True)
+        assert cpu64.setup(band) == 0
+        w64 = np.zeros(n)
+        w64[ref["row_perm"]] = v * ref["row_scale"]
+        z64 = np.zeros(n)
+        z64[ref["col_reord"]] = cpu64.apply(w64) * ref["col_scale"][ref["col_reord"]]
+        atol = max(3e-5 * np.abs(expect).max(), 4 * np.abs(expect - z64).max())
+    np.testing.assert_allclose(host(z), expect, rtol=0, atol=atol)
+
+
 def test_csr_options_rejected(sap):
     rp, ci, val = sparse_band(200, 10, 3, 70)
     A = sap.CsrMatrix.from_host(rp, ci, val)
-    with pytest.raises(sap.SapError) as ei:
-        sap.Solver(2, sap.Options()).setup(A)          # variableBandwidth = 1 by default
-    assert ei.value.reason == -9
     with pytest.raises(sap.SapError) as ei:
         sap.Solver(2, sap.Options(variableBandwidth=0, testDB=1)).setup(A)
     assert ei.value.reason == -9
