@@ -92,8 +92,9 @@ struct ResLds {
     static constexpr int pv_off = ub_off + NTR * PS;      // pivots [32]
     static constexpr int rp_off = pv_off + 32;            // their reciprocals [32]
     static constexpr int buf_elems = rp_off + 32;
-    static constexpr int lt_off = 2 * buf_elems;          // L11 transposed [32][33]
-    static constexpr size_t bytes = sizeof(T) * (size_t)(lt_off + 32 * 33);
+    static constexpr int LTS = 36;                        // L11T row stride (16-byte aligned rows)
+    static constexpr int lt_off = 2 * buf_elems;          // L11 transposed [32][LTS]
+    static constexpr size_t bytes = sizeof(T) * (size_t)(lt_off + 32 * LTS);
 };
 
 template <class T>
@@ -124,6 +125,44 @@ __device__ inline void buffer_store<double>(double v, __amdgpu_buffer_rsrc_t r, 
     typedef unsigned int u2 __attribute__((ext_vector_type(2)));
     const u2 x = {(unsigned)b, (unsigned)(b >> 32)};
     __builtin_amdgcn_raw_buffer_store_b64(x, r, off, 0, 0);
+}
+
+// 32 values of a 16-byte aligned LDS row (float: 16-byte loads), from
+// element 4 * (FROM / 4) on; the rest of v is left as it is
+template <class T>
+__device__ inline void lds_row_from(const T* p, T (&v)[32], int from)
+{
+    if constexpr (sizeof(T) == 4) {
+        const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (4 * i + 3 >= from) {
+                const float4 x = q[i];
+                v[4 * i] = x.x; v[4 * i + 1] = x.y; v[4 * i + 2] = x.z; v[4 * i + 3] = x.w;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 32; ++c)
+            if (c >= from) v[c] = p[c];
+    }
+}
+template <int FROM, class T>
+__device__ inline void lds_row(const T* p, T (&v)[32])
+{
+    lds_row_from<T>(p, v, FROM);
+}
+__device__ inline void lds_store_row(float* p, const float (&v)[32])
+{
+    float4* q = reinterpret_cast<float4*>(p);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) q[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+}
+__device__ inline void lds_store_neg_row(float* p, const float (&v)[32])
+{
+    float4* q = reinterpret_cast<float4*>(p);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) q[i] = make_float4(-v[4 * i], -v[4 * i + 1], -v[4 * i + 2], -v[4 * i + 3]);
 }
 
 __device__ inline float pivot_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
@@ -160,14 +199,17 @@ __device__ inline int rot(int x, int sh)
     return t < 0 ? t + KT : t;
 }
 
-template <class T, int KT>
-__global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict__ B0, T* __restrict__ B1, int k,
+template <class T, int KT, int NW>
+__global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __restrict__ B0, T* __restrict__ B1, int k,
                                                                    const FactorJob* __restrict__ jobs)
 {
     using MF = Mfma16<T>;
     using acc_t = typename MF::acc_t;
     using LY = ResLds<T, KT>;
-    constexpr int PS = LY::PS, NTR = LY::NTR, NS = KT * KT, SPW = (NS + 7) / 8, HR = 32 + NTR;
+    constexpr int PS = LY::PS, NTR = LY::NTR, NS = KT * KT, SPW = (NS + NW - 1) / NW, HR = 32 + NTR;
+    constexpr int NT = 64 * NW;         // threads
+    constexpr int UOFF = 64 * (NW / 2); // first thread of the U12 solves in B2
+    static_assert(NTR <= UOFF && UOFF + NTR <= NT, "B2 thread ranges");
     const FactorJob job = jobs[blockIdx.x];
     T* __restrict__ base = job.buf ? B1 : B0;
     const int64_t w = 2 * k + 1;
@@ -215,7 +257,7 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
     // an offset past the end of the buffer and are dropped, so every panel
     // issues the same number of stores (pp < 0: none land).
     auto store_panel = [&](int64_t pp, const T* bq, int tid) {
-        constexpr int N1 = (32 * HR + 511) / 512, N2 = (32 * NTR + 511) / 512;
+        constexpr int N1 = (32 * HR + NT - 1) / NT, N2 = (32 * NTR + NT - 1) / NT;
         const int64_t pj0 = 32 * max(pp, (int64_t)0);
         const int obq = rel0 + dir * (int)(pj0 * w + k);
         const int prem = pp < 0 ? 0 : (int)min(len - pj0, (int64_t)(1 << 30));
@@ -227,7 +269,7 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         int o1[N1], o2[N2];
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
-            const int e = tid + 512 * i;
+            const int e = tid + NT * i;
             const int c = e / HR, r = e - c * HR;
             const int ok = neg_mask((e - 32 * HR) & (c - pnb) & (r - prem) & (r - c - k - 1) & (c - r - k - 1));
             const int lrow = ~neg_mask(r - 32);   // L21 rows, kept negated in LDS
@@ -239,7 +281,7 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         }
 #pragma unroll
         for (int i = 0; i < N2; ++i) {
-            const int e = tid + 512 * i;
+            const int e = tid + NT * i;
             const int cc = e >> 5, m = e & 31;
             const int ok = neg_mask((e - 32 * NTR) & (m - pnb) & (32 + cc - prem) & (32 + cc - m - k - 1));
             v2[i] = qUb[(cc * PS + m) & ok] * qrp[m];
@@ -258,7 +300,7 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         const int rem = (int)min(len, (int64_t)(1 << 30));
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
-            const int s = wave + 8 * j;
+            const int s = wave + NW * j;
             if (s < NS) load_tile(acc[j], ob, rem, 16 * (s / KT), 16 * (s % KT), lane & 15, lane);
         }
     }
@@ -300,7 +342,7 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         //    waits for every load issued before it.)
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
-            const int s = wave + 8 * j;
+            const int s = wave + NW * j;
             if (s < NS) {
                 const int rho = rot<KT>(s / KT, sh), gam = rot<KT>(s % KT, sh);
                 if (gam < 2) {
@@ -314,7 +356,7 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         }
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
-            const int s = wave + 8 * j;
+            const int s = wave + NW * j;
             if (s < NS) {
                 const int rho = rot<KT>(s / KT, sh), gam = rot<KT>(s % KT, sh);
                 if (gam < 2 || rho < 2) {
@@ -355,14 +397,50 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
 #pragma unroll
                 for (int c = 0; c < 32; ++c) {
                     Pn[r * PS + c] = row[c];
-                    if (c < r) L11T[c * 33 + r] = row[c];
+                    if (c < r) L11T[c * LY::LTS + r] = row[c];
                 }
             }
         }
         lds_barrier();
 
         // B2. L21 rows (stored negated: the trailing update adds) and U12 columns
-        if (tid < NTR) {
+        if constexpr (sizeof(T) == 4) {
+            // FP32: the rows of U11 (L11^T) come in 16-byte loads one step
+            // ahead of their use, so the LDS latency hides behind the
+            // previous step's multiply-adds
+            if (tid < NTR) {
+                const int t = tid;
+                T a[32], cur[32], nxt[32];
+                lds_row<0>(Pn + (32 + t) * PS, a);
+                lds_row<1>(Pn, cur);
+#pragma unroll
+                for (int jj = 0; jj < 32; ++jj) {   // rpiv = 0 past the partition's end
+                    if (jj < 31) lds_row_from<T>(Pn + (jj + 1) * PS, nxt, jj + 2);
+                    const T l = a[jj] * rpiv[jj];
+                    a[jj] = l;
+#pragma unroll
+                    for (int c = jj + 1; c < 32; ++c) a[c] -= l * cur[c];
+#pragma unroll
+                    for (int c = 0; c < 32; ++c) cur[c] = nxt[c];
+                }
+                lds_store_neg_row(Pn + (32 + t) * PS, a);
+            } else if (tid >= UOFF && tid < UOFF + NTR) {
+                const int cc = tid - UOFF;
+                T u[32], cur[32], nxt[32];
+                lds_row<0>(UbT + cc * PS, u);
+                lds_row<1>(L11T, cur);
+#pragma unroll
+                for (int jj = 0; jj < 31; ++jj) {   // u[jj] = 0 past the partition's end
+                    if (jj < 30) lds_row_from<T>(L11T + (jj + 1) * LY::LTS, nxt, jj + 2);
+                    const T uj = u[jj];
+#pragma unroll
+                    for (int m = jj + 1; m < 32; ++m) u[m] -= cur[m] * uj;
+#pragma unroll
+                    for (int m = 0; m < 32; ++m) cur[m] = nxt[m];
+                }
+                lds_store_row(UbT + cc * PS, u);
+            }
+        } else if (tid < NTR) {
             const int t = tid;
             T a[32];
 #pragma unroll
@@ -376,8 +454,8 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
             }
 #pragma unroll
             for (int c = 0; c < 32; ++c) Pn[(32 + t) * PS + c] = -a[c];
-        } else if (tid >= 256 && tid < 256 + NTR) {
-            const int cc = tid - 256;
+        } else if (tid >= UOFF && tid < UOFF + NTR) {
+            const int cc = tid - UOFF;
             T u[32];
 #pragma unroll
             for (int m = 0; m < 32; ++m) u[m] = UbT[cc * PS + m];
@@ -385,7 +463,7 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
             for (int jj = 0; jj < 31; ++jj) {   // u[jj] = 0 past the partition's end
                 const T uj = u[jj];
 #pragma unroll
-                for (int m = jj + 1; m < 32; ++m) u[m] -= L11T[jj * 33 + m] * uj;
+                for (int m = jj + 1; m < 32; ++m) u[m] -= L11T[jj * LY::LTS + m] * uj;
             }
 #pragma unroll
             for (int m = 0; m < 32; ++m) UbT[cc * PS + m] = u[m];
@@ -396,7 +474,7 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
         //    other buffer, so no barrier is needed after this phase.
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
-            const int s = wave + 8 * j;
+            const int s = wave + NW * j;
             if (s < NS) {
                 const int rho = rot<KT>(s / KT, sh), gam = rot<KT>(s % KT, sh);
                 if (rho >= 2 && gam >= 2) {
@@ -415,13 +493,21 @@ __global__ __launch_bounds__(512) void band_factor_resident_kernel(T* __restrict
     if (npan > 0) store_panel(npan - 1, sm + ((npan - 1) & 1) * LY::buf_elems, threadIdx.x);
 }
 
+// Wavefronts per workgroup: the window's tiles are spread over them, so more
+// wavefronts leave each more registers for the panel phases (and more
+// latency hiding), as long as the accumulators still fit.
+template <class T, int KT>
+constexpr int resident_waves() { return 8; }
+
 template <class T, int KT>
 void launch_factor_resident(T* B0, T* B1, int k, const FactorJob* jobs, int njobs, hipStream_t st)
 {
+    constexpr int NW = resident_waves<T, KT>();
     const size_t lds = ResLds<T, KT>::bytes;
-    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_factor_resident_kernel<T, KT>,
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_factor_resident_kernel<T, KT, NW>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((band_factor_resident_kernel<T, KT>), dim3(njobs), dim3(512), lds, st, B0, B1, k, jobs);
+    hipLaunchKernelGGL((band_factor_resident_kernel<T, KT, NW>), dim3(njobs), dim3(64 * NW), lds, st, B0, B1, k,
+                       jobs);
     SAP_LAUNCH_CHECK();
 }
 
