@@ -188,6 +188,10 @@ public:
     double* hscalar_ = nullptr;
     std::vector<double*> vecs_;
     size_t vec_next_ = 0;
+    // b - A x of the last BiCGStab(L) solve when it converged (on a passed
+    // true-residual check; x its returned solution), else null; a pool
+    // vector valid until the next vec().
+    const double* last_res_ = nullptr;
 
     Solver(int P, const sap_options& o) : opt(o), P_req(P)
     {
@@ -288,6 +292,7 @@ public:
         double* x_min = vec();
         double* Pxx = vec();
         double* APxx = vec();
+        double* rsave = vec();
         for (int i = 0; i <= L; ++i) { launch_fill(n, 0.0, rr[i], stream); launch_fill(n, 0.0, uu[i], stream); }
         launch_fill(n, 0.0, u, stream);
         launch_fill(n, 0.0, x_min, stream);
@@ -309,13 +314,16 @@ public:
         double r_norm_act = r_norm;
         bool pxx_valid = false;    // Pxx == M^{-1} xx for the current xx
 
+        last_res_ = nullptr;
         auto check_true = [&]() -> bool {
             prec(xx, Pxx);
             spmv(Pxx, APxx);
             axpby(b, APxx, rr[0], 1.0, -1.0);
             pxx_valid = true;
             r_norm_act = nrm2(rr[0]);
-            return monitor.finished(r_norm_act);
+            const bool done = monitor.finished(r_norm_act);
+            if (done && monitor.converged()) last_res_ = rr[0];
+            return done;
         };
 
         while (true) {
@@ -334,15 +342,26 @@ public:
                 alpha = rho0 / gam;
                 for (int i = 0; i <= j; ++i) axpy(uu[i + 1], rr[i], -alpha);
                 r_norm_act = r_norm = nrm2(rr[0]);
-                prec(rr[j], Pv);
-                spmv(Pv, rr[j + 1]);
+                // rr(j+1) <- A P^{-1} rr(j) of the recursive rr(j).  The
+                // reference forms it before the true-residual check; here it
+                // is formed after a check that does not end the solve (the
+                // check overwrites rr(0), so a recursive rr(0) is saved
+                // first), which gives the same iterates and saves one
+                // preconditioner application and one SpMV on the converging
+                // step.
                 if (std::fabs(alpha) * nrm2(uu[0]) < eps * nrm2(xx)) monitor.incrementStag();
                 else monitor.resetStag();
                 axpy(uu[0], xx, alpha);
                 pxx_valid = false;
                 if (monitor.needCheckConvergence(r_norm)) {
+                    const double* src = rr[j];
+                    if (j == 0) { copy(rr[0], rsave); src = rsave; }
                     if (check_true()) break;
+                    prec(src, Pv);
+                } else {
+                    prec(rr[j], Pv);
                 }
+                spmv(Pv, rr[j + 1]);
                 if (r_norm_act < r_norm_min) { r_norm_min = r_norm_act; copy(xx, x_min); }
                 if (monitor.finished()) break;
             }
@@ -516,6 +535,7 @@ public:
     int krylov(double* x, const double* b, double rtol, double atol, bool x_is_zero, float* its, double* rnorm,
                double* rhs_norm)
     {
+        last_res_ = nullptr;
         if (use_minres()) {
             mmon.reset(opt.maxNumIterations, rtol, atol);
             mmon.init(nrm2(b));
@@ -657,7 +677,7 @@ public:
         user_ctx = ctx;
         stats.numApplies = stats.numSpmv = stats.numRefine = 0;
         // b, x, three refinement vectors, and the Krylov method's vectors
-        // (BiCGStab(2): 20 including the non-converged tail; MINRES: 8)
+        // (BiCGStab(2): 19 including the non-converged tail; MINRES: 8)
         reserve_vectors(32);
         double* bv = vec();
         double* xv = vec();
@@ -677,11 +697,30 @@ public:
             const size_t mark = vec_next_;
             bn = nrm2(bv);
             const double tol = opt.absTol + opt.relTol * bn;
+            // r = b - A x.  A*0 = 0 exactly, so a zero initial guess needs no
+            // SpMV; after an inner solve that converged, r is that solve's
+            // own b - A d (the residual of x + d up to FP64 rounding of the
+            // update), and convergence is only declared on b - A x formed
+            // afresh.  After an inner solve that did not converge, b - A x
+            // is formed afresh.
+            bool have_r = nrm2(xv) == 0.0;
+            if (have_r) copy(bv, r);
+            bool recurred = false;
             code = -1;
             for (int outer = 0;; ++outer) {
-                spmv(xv, Ax);
-                axpby(bv, Ax, r, 1.0, -1.0);
+                if (!have_r) {
+                    spmv(xv, Ax);
+                    axpby(bv, Ax, r, 1.0, -1.0);
+                    recurred = false;
+                }
                 rn = nrm2(r);
+                if (!std::isnan(rn) && rn <= tol && recurred) {
+                    spmv(xv, Ax);
+                    axpby(bv, Ax, r, 1.0, -1.0);
+                    recurred = false;
+                    rn = nrm2(r);
+                }
+                have_r = false;
                 if (std::isnan(rn)) { code = -2; break; }
                 if (rn <= tol) { code = 1; break; }
                 if (outer >= opt.refine) { code = -1; break; }
@@ -693,6 +732,10 @@ public:
                 stats.numRefine++;
                 axpy(dv, xv, 1.0);
                 if (c == -2) { code = -2; break; }
+                if (last_res_) {
+                    copy(last_res_, r);
+                    have_r = recurred = true;
+                }
             }
             message = code == 1 ? "Converged"
                       : code == -2 ? "Residual norm is NaN" : "Maximum number of refinement steps was reached";
