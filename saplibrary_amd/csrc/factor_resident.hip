@@ -191,14 +191,6 @@ __device__ inline double pick(double a, double b, int m)
     const unsigned hi = ((unsigned)(x >> 32) & (unsigned)m) | ((unsigned)(y >> 32) & ~(unsigned)m);
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
-// (x + KT - sh) mod KT for 0 <= x, sh < KT
-template <int KT>
-__device__ inline int rot(int x, int sh)
-{
-    const int t = x - sh;
-    return t < 0 ? t + KT : t;
-}
-
 template <class T, int KT, int NW>
 __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __restrict__ B0, T* __restrict__ B1, int k,
                                                                    const FactorJob* __restrict__ jobs)
@@ -210,6 +202,7 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
     constexpr int NT = 64 * NW;         // threads
     constexpr int UOFF = 64 * (NW / 2); // first thread of the U12 solves in B2
     static_assert(NTR <= UOFF && UOFF + NTR <= NT, "B2 thread ranges");
+    static_assert(SPW <= 32, "slot class masks are 32-bit");
     const FactorJob job = jobs[blockIdx.x];
     T* __restrict__ base = job.buf ? B1 : B0;
     const int64_t w = 2 * k + 1;
@@ -326,6 +319,28 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
         const int nb = min(32, rem);
         const int sh = (int)((2 * p) % KT);
         const int obp = rel0 + dir * (int)(j0 * w + k);
+        // Slot bookkeeping, lane-parallel: lane j holds the rotated tile
+        // coordinates (rho, gam) of this wavefront's slot j (s = wave + NW j,
+        // clamped to NS - 1 past the last slot), and three ballots give the
+        // slot classes: panel column (gam < 2), U row (rho < 2 <= gam) and
+        // trailing (both >= 2).  Each unrolled slot step then costs a mask bit
+        // test and one readlane instead of its own scalar index arithmetic.
+        unsigned pm, um, tm;
+        int rg;
+        {
+            const int sl = min(wave + NW * lane, NS - 1);
+            int rl = sl / KT - sh, gl = sl % KT - sh;
+            rl += KT & neg_mask(rl);
+            gl += KT & neg_mask(gl);
+            const bool okl = lane < SPW && wave + NW * lane < NS;
+            pm = (unsigned)__builtin_amdgcn_ballot_w64(okl && gl < 2);
+            um = (unsigned)__builtin_amdgcn_ballot_w64(okl && gl >= 2 && rl < 2);
+            tm = (unsigned)__builtin_amdgcn_ballot_w64(okl && gl >= 2 && rl >= 2);
+            rg = rl | (gl << 8);
+        }
+        // rho and gam of slot j (j a compile-time constant)
+        auto slot_rho = [&](int j) { return __builtin_amdgcn_readlane(rg, j) & 0xff; };
+        auto slot_gam = [&](int j) { return __builtin_amdgcn_readlane(rg, j) >> 8; };
 
         // Every load and store of the previous panel has long completed (B1,
         // B2 and C ran since); saying so with an explicit wait clears the
@@ -342,28 +357,23 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
         //    waits for every load issued before it.)
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
-            const int s = wave + NW * j;
-            if (s < NS) {
-                const int rho = rot<KT>(s / KT, sh), gam = rot<KT>(s % KT, sh);
-                if (gam < 2) {
+            if ((pm >> j) & 1) {
+                const int rho = slot_rho(j), gam = slot_gam(j);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) Pn[(16 * rho + lr) * PS + 16 * gam + MF::row(lane, q)] = acc[j][q];
-                } else if (rho < 2) {
+                for (int q = 0; q < 4; ++q) Pn[(16 * rho + lr) * PS + 16 * gam + MF::row(lane, q)] = acc[j][q];
+            } else if ((um >> j) & 1) {
+                const int rho = slot_rho(j), gam = slot_gam(j);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) UbT[(16 * (gam - 2) + MF::row(lane, q)) * PS + 16 * rho + lr] = acc[j][q];
-                }
+                for (int q = 0; q < 4; ++q) UbT[(16 * (gam - 2) + MF::row(lane, q)) * PS + 16 * rho + lr] = acc[j][q];
             }
         }
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
-            const int s = wave + NW * j;
-            if (s < NS) {
-                const int rho = rot<KT>(s / KT, sh), gam = rot<KT>(s % KT, sh);
-                if (gam < 2 || rho < 2) {
-                    const int rn = rho < 2 ? rho + KT - 2 : rho - 2;
-                    const int cn = gam < 2 ? gam + KT - 2 : gam - 2;
-                    load_tile(acc[j], obp, rem, 32 + 16 * rn, 32 + 16 * cn, lr, lane);
-                }
+            if (((pm | um) >> j) & 1) {
+                const int rho = slot_rho(j), gam = slot_gam(j);
+                const int rn = rho < 2 ? rho + KT - 2 : rho - 2;
+                const int cn = gam < 2 ? gam + KT - 2 : gam - 2;
+                load_tile(acc[j], obp, rem, 32 + 16 * rn, 32 + 16 * cn, lr, lane);
             }
         }
         store_panel(p - 1, sm + zoff + (1 - cur) * LY::buf_elems, tid);
@@ -472,19 +482,46 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
 
         // C. trailing update on the matrix cores.  The next panel writes the
         //    other buffer, so no barrier is needed after this phase.
+        if constexpr (sizeof(T) == 4) {
+            // FP32: each slot's operands are two 16-byte rows (lane group lg
+            // takes k = 8 lg .. 8 lg + 7), fetched one slot ahead of the slot
+            // being multiplied.  The fetches are unconditional: for a slot
+            // that is not updated (panel rows or columns; s >= NS reads slot
+            // NS - 1) the addresses still fall inside this panel buffer (rows
+            // 16 rho + lr < 16 KT of Pn, or rows of UbT at most 32 before it,
+            // which are rows of Pn), and the values go unused.
+            auto fetch = [&](int j, float4 (&o)[4]) {
+                const int rho = slot_rho(j), gam = slot_gam(j);
+                const float4* ap = reinterpret_cast<const float4*>(Pn + (16 * rho + lr) * PS + 8 * lg);
+                const float4* bp = reinterpret_cast<const float4*>(UbT + (16 * (gam - 2) + lr) * PS + 8 * lg);
+                o[0] = ap[0]; o[1] = ap[1]; o[2] = bp[0]; o[3] = bp[1];
+            };
+            float4 op[4], nx[4];
+            fetch(0, op);
 #pragma unroll
-        for (int j = 0; j < SPW; ++j) {
-            const int s = wave + NW * j;
-            if (s < NS) {
-                const int rho = rot<KT>(s / KT, sh), gam = rot<KT>(s % KT, sh);
-                if (rho >= 2 && gam >= 2) {
+            for (int j = 0; j < SPW; ++j) {
+                if (j + 1 < SPW) fetch(j + 1, nx);
+                if ((tm >> j) & 1) {
+                    const float av[8] = {op[0].x, op[0].y, op[0].z, op[0].w, op[1].x, op[1].y, op[1].z, op[1].w};
+                    const float bv[8] = {op[2].x, op[2].y, op[2].z, op[2].w, op[3].x, op[3].y, op[3].z, op[3].w};
+#pragma unroll
+                    // transposed tiles: acc^T += U12^T (-L21)^T
+                    for (int q = 0; q < 8; ++q) acc[j] = MF::mma(bv[q], av[q], acc[j]);
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) op[i] = nx[i];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < SPW; ++j) {
+                if ((tm >> j) & 1) {
+                    const int rho = slot_rho(j), gam = slot_gam(j);
                     const T* ap = Pn + (16 * rho + lr) * PS + 8 * lg;
                     const T* bp = UbT + (16 * (gam - 2) + lr) * PS + 8 * lg;
                     T av[8], bv[8];
 #pragma unroll
                     for (int q = 0; q < 8; ++q) { av[q] = ap[q]; bv[q] = bp[q]; }
 #pragma unroll
-                    // transposed tiles: acc^T += U12^T (-L21)^T
                     for (int q = 0; q < 8; ++q) acc[j] = MF::mma(bv[q], av[q], acc[j]);
                 }
             }
