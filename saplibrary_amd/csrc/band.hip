@@ -2,6 +2,8 @@
 // row-major FP64 band -> column-major preconditioner band conversion.
 #include "sap_internal.hpp"
 
+#include <algorithm>
+
 namespace sap_amd {
 
 namespace {
@@ -149,6 +151,62 @@ __global__ __launch_bounds__(256) void band_to_colmajor_kernel(int64_t n, int k,
     }
 }
 
+// Column tiles: one workgroup builds tc whole output columns (all 2k+1
+// diagonals) in LDS and writes them as one contiguous run.  Input row r
+// holds A(r, c) at offset o = c - r + k, so the tile's columns take one
+// contiguous segment of at most tc values from each of the tc + 2k rows
+// r = c0 - k + rr, o in [max(0, 2k - rr), min(2k+1, 2k + tc - rr)): every
+// input element is read once, in whole-segment loads.  In LDS, column cc
+// has stride 2k+2 (even), so the scatter of one row's segment (consecutive
+// o: stride 2k+1 words, odd) hits distinct banks.
+// ---------------------------------------------------------------------------
+constexpr int B2C_THREADS = 1024;
+constexpr int B2C_ROWS = 8;   // rows per wavefront whose loads are in flight together
+
+template <class T>
+__global__ __launch_bounds__(B2C_THREADS) void band_to_colmajor_cols_kernel(int64_t n, int k, int tc,
+                                                                        const double* __restrict__ in,
+                                                                        T* __restrict__ out)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* const lds = reinterpret_cast<T*>(smem_raw);
+    const int w = 2 * k + 1, ls = w + 1;
+    const int64_t c0 = (int64_t)blockIdx.x * tc;
+    const int ncol = (int)min((int64_t)tc, n - c0);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int NWV = B2C_THREADS / 64;
+    const int nrow = ncol + 2 * k;
+    for (int rb = wave; rb < nrow; rb += NWV * B2C_ROWS) {
+        double v[B2C_ROWS];
+#pragma unroll
+        for (int u = 0; u < B2C_ROWS; ++u) {
+            const int rr = rb + u * NWV;
+            const int64_t r = c0 - k + rr;
+            const int o = max(0, 2 * k - rr) + lane;
+            const bool ok = rr < nrow && o < min(w, 2 * k + ncol - rr) && r >= 0 && r < n;
+            v[u] = ok ? in[r * w + o] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < B2C_ROWS; ++u) {
+            const int rr = rb + u * NWV;
+            const int o = max(0, 2 * k - rr) + lane;
+            if (rr < nrow && o < min(w, 2 * k + ncol - rr)) lds[(rr - 2 * k + o) * ls + 2 * k - o] = (T)v[u];
+        }
+    }
+    __syncthreads();
+    const int64_t tot = (int64_t)ncol * w;
+    T* const dst = out + c0 * w;
+    // element e of the run is column e / w, diagonal e % w (stepped, no division per element)
+    int cc = (int)threadIdx.x / w, d = (int)threadIdx.x - cc * w;
+    const int qs = B2C_THREADS / w, rs = B2C_THREADS - qs * w;
+    for (int64_t e = threadIdx.x; e < tot; e += B2C_THREADS) {
+        dst[e] = lds[cc * ls + d];
+        cc += qs;
+        d += rs;
+        if (d >= w) { d -= w; ++cc; }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // CSR entries into the zero-filled column-major band, cast to the band type
 // (the COO scatter copyFromCOOMatrixToBandedMatrix,
@@ -281,8 +339,20 @@ template <class T>
 void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st)
 {
     const int w = 2 * k + 1;
-    dim3 grid((unsigned)((n + TC - 1) / TC), (unsigned)((w + TD - 1) / TD));
-    hipLaunchKernelGGL(band_to_colmajor_kernel<T>, grid, dim3(256), 0, st, n, k, band_rm, B);
+    // column tiles of up to 64 columns within 150 KiB of LDS; bands too wide
+    // for 8 columns use the diagonal-box kernel
+    constexpr size_t budget = 150 * 1024;
+    const int tc = (int)std::min<size_t>(64, budget / ((size_t)(w + 1) * sizeof(T)));
+    if (tc >= 8) {
+        const size_t lds = (size_t)tc * (w + 1) * sizeof(T);
+        SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_to_colmajor_cols_kernel<T>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(band_to_colmajor_cols_kernel<T>, dim3((unsigned)((n + tc - 1) / tc)), dim3(B2C_THREADS),
+                           lds, st, n, k, tc, band_rm, B);
+    } else {
+        dim3 grid((unsigned)((n + TC - 1) / TC), (unsigned)((w + TD - 1) / TD));
+        hipLaunchKernelGGL(band_to_colmajor_kernel<T>, grid, dim3(256), 0, st, n, k, band_rm, B);
+    }
     SAP_LAUNCH_CHECK();
 }
 
