@@ -277,7 +277,7 @@ public:
 
     // ------------------------------------------- BiCGStab(L), sap/bicgstab2.h:31-354
     template <int L>
-    void bicgstabl(double* x, const double* b, BiCGStabLMonitor& monitor, bool x_is_zero)
+    void bicgstabl(double* x, const double* b, BiCGStabLMonitor& monitor, bool x_is_zero, bool correction)
     {
         const double eps = 1e-20;
         double rho0 = 1, alpha = 0, omega = 1, rho1;
@@ -326,6 +326,15 @@ public:
             return done;
         };
 
+        // A refinement correction solve (x = 0) whose first step already
+        // meets the tolerance returns x = alpha M^{-1} uu(0), using the
+        // application made for uu(1): its residual b - alpha A M^{-1} uu(0)
+        // is the recursive rr(0) (up to FP64 rounding of the products), so
+        // the check needs no further application or SpMV.  (The reference
+        // returns M^{-1}(alpha uu(0)), which differs only by the rounding of
+        // the preconditioner; the refinement forms b - A x afresh before it
+        // declares convergence.)
+        bool first_step = correction && x_is_zero;
         while (true) {
             rho0 = -omega * rho0;
             monitor.increment(0.25f);
@@ -353,11 +362,23 @@ public:
                 else monitor.resetStag();
                 axpy(uu[0], xx, alpha);
                 pxx_valid = false;
+                const bool first = first_step;
+                first_step = false;
                 if (monitor.needCheckConvergence(r_norm)) {
-                    const double* src = rr[j];
-                    if (j == 0) { copy(rr[0], rsave); src = rsave; }
-                    if (check_true()) break;
-                    prec(src, Pv);
+                    if (first) {
+                        axpby(Pv, Pv, Pxx, alpha, 0.0);   // Pv = M^{-1} uu(0) until rr(1) is formed
+                        pxx_valid = true;
+                        r_norm_act = r_norm;
+                        const bool done = monitor.finished(r_norm_act);
+                        if (done && monitor.converged()) last_res_ = rr[0];
+                        if (done) break;
+                        prec(rr[0], Pv);
+                    } else {
+                        const double* src = rr[j];
+                        if (j == 0) { copy(rr[0], rsave); src = rsave; }
+                        if (check_true()) break;
+                        prec(src, Pv);
+                    }
                 } else {
                     prec(rr[j], Pv);
                 }
@@ -532,8 +553,8 @@ public:
     bool use_minres() const { return opt.solverType == SAP_MINRES; }
 
     // One Krylov solve of A x = b (the reference's solve path).
-    int krylov(double* x, const double* b, double rtol, double atol, bool x_is_zero, float* its, double* rnorm,
-               double* rhs_norm)
+    int krylov(double* x, const double* b, double rtol, double atol, bool x_is_zero, bool correction, float* its,
+               double* rnorm, double* rhs_norm)
     {
         last_res_ = nullptr;
         if (use_minres()) {
@@ -548,8 +569,8 @@ public:
         }
         bmon.reset(opt.maxNumIterations, 8, rtol, atol);   // maxmSteps = 8 (sap/solver.h:314-319)
         bmon.init(nrm2(b));
-        if (opt.solverType == SAP_BiCGStab1) bicgstabl<1>(x, b, bmon, x_is_zero);
-        else bicgstabl<2>(x, b, bmon, x_is_zero);
+        if (opt.solverType == SAP_BiCGStab1) bicgstabl<1>(x, b, bmon, x_is_zero, correction);
+        else bicgstabl<2>(x, b, bmon, x_is_zero, correction);
         *its += bmon.iterations;
         *rnorm = bmon.rNorm;
         *rhs_norm = bmon.rhsNorm;
@@ -728,7 +749,7 @@ public:
                 launch_fill(n, 0.0, dv, stream);
                 vec_next_ = mark;
                 double inner_r, inner_b;
-                const int c = krylov(dv, r, rt, 0.0, true, &its, &inner_r, &inner_b);
+                const int c = krylov(dv, r, rt, 0.0, true, true, &its, &inner_r, &inner_b);
                 stats.numRefine++;
                 axpy(dv, xv, 1.0);
                 if (c == -2) { code = -2; break; }
@@ -742,7 +763,7 @@ public:
             bmon.code = mmon.code = code;
             bmon.message = mmon.message = message;
         } else {
-            code = krylov(xv, bv, opt.relTol, opt.absTol, false, &its, &rn, &bn);
+            code = krylov(xv, bv, opt.relTol, opt.absTol, false, false, &its, &rn, &bn);
         }
         copy(xv, d_x);            // thrust::copy(x_vector -> x) (sap/solver.h:534)
         SAP_HIP_CHECK(hipStreamSynchronize(stream));
