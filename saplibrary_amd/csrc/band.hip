@@ -119,7 +119,7 @@ constexpr int TD = 32;
 
 template <class T>
 __global__ __launch_bounds__(256) void band_to_colmajor_kernel(int64_t n, int k, const double* __restrict__ in,
-                                                               T* __restrict__ out)
+                                                               T* __restrict__ out, T* __restrict__ out2)
 {
     __shared__ double tile[TC + TD][TD + 1];
     const int w = 2 * k + 1;
@@ -148,6 +148,7 @@ __global__ __launch_bounds__(256) void band_to_colmajor_kernel(int64_t n, int k,
         double v = 0.0;
         if (r >= 0 && r < n) v = tile[cc + dd][(TD - 1) - dd];
         out[c * w + d] = (T)v;
+        if (out2) out2[c * w + d] = (T)v;
     }
 }
 
@@ -158,7 +159,8 @@ __global__ __launch_bounds__(256) void band_to_colmajor_kernel(int64_t n, int k,
 // r = c0 - k + rr, o in [max(0, 2k - rr), min(2k+1, 2k + tc - rr)): every
 // input element is read once, in whole-segment loads.  In LDS, column cc
 // has stride 2k+2 (even), so the scatter of one row's segment (consecutive
-// o: stride 2k+1 words, odd) hits distinct banks.
+// o: stride 2k+1 words, odd) hits distinct banks.  out2, when given,
+// receives a second copy (the band the UL factorizations overwrite).
 // ---------------------------------------------------------------------------
 constexpr int B2C_THREADS = 1024;
 constexpr int B2C_ROWS = 8;   // rows per wavefront whose loads are in flight together
@@ -166,7 +168,7 @@ constexpr int B2C_ROWS = 8;   // rows per wavefront whose loads are in flight to
 template <class T>
 __global__ __launch_bounds__(B2C_THREADS) void band_to_colmajor_cols_kernel(int64_t n, int k, int tc,
                                                                         const double* __restrict__ in,
-                                                                        T* __restrict__ out)
+                                                                        T* __restrict__ out, T* __restrict__ out2)
 {
     extern __shared__ unsigned char smem_raw[];
     T* const lds = reinterpret_cast<T*>(smem_raw);
@@ -196,11 +198,14 @@ __global__ __launch_bounds__(B2C_THREADS) void band_to_colmajor_cols_kernel(int6
     __syncthreads();
     const int64_t tot = (int64_t)ncol * w;
     T* const dst = out + c0 * w;
+    T* const dst2 = out2 ? out2 + c0 * w : nullptr;
     // element e of the run is column e / w, diagonal e % w (stepped, no division per element)
     int cc = (int)threadIdx.x / w, d = (int)threadIdx.x - cc * w;
     const int qs = B2C_THREADS / w, rs = B2C_THREADS - qs * w;
     for (int64_t e = threadIdx.x; e < tot; e += B2C_THREADS) {
-        dst[e] = lds[cc * ls + d];
+        const T v = lds[cc * ls + d];
+        dst[e] = v;
+        if (dst2) dst2[e] = v;
         cc += qs;
         d += rs;
         if (d >= w) { d -= w; ++cc; }
@@ -336,7 +341,7 @@ void launch_band_spmv(int64_t n, int k, const double* band, const double* x, dou
 }
 
 template <class T>
-void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st)
+void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st, T* B2)
 {
     const int w = 2 * k + 1;
     // column tiles of up to 64 columns within 150 KiB of LDS; bands too wide
@@ -348,16 +353,16 @@ void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipS
         SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_to_colmajor_cols_kernel<T>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(band_to_colmajor_cols_kernel<T>, dim3((unsigned)((n + tc - 1) / tc)), dim3(B2C_THREADS),
-                           lds, st, n, k, tc, band_rm, B);
+                           lds, st, n, k, tc, band_rm, B, B2);
     } else {
         dim3 grid((unsigned)((n + TC - 1) / TC), (unsigned)((w + TD - 1) / TD));
-        hipLaunchKernelGGL(band_to_colmajor_kernel<T>, grid, dim3(256), 0, st, n, k, band_rm, B);
+        hipLaunchKernelGGL(band_to_colmajor_kernel<T>, grid, dim3(256), 0, st, n, k, band_rm, B, B2);
     }
     SAP_LAUNCH_CHECK();
 }
 
-template void launch_band_to_colmajor<float>(int64_t, int, const double*, float*, hipStream_t);
-template void launch_band_to_colmajor<double>(int64_t, int, const double*, double*, hipStream_t);
+template void launch_band_to_colmajor<float>(int64_t, int, const double*, float*, hipStream_t, float*);
+template void launch_band_to_colmajor<double>(int64_t, int, const double*, double*, hipStream_t, double*);
 
 template <class Tin, class T>
 void launch_csr_to_band(int64_t n, int k, const int* rp, const int* ci, const Tin* vals, T* B, hipStream_t st)

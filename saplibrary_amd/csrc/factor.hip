@@ -729,13 +729,16 @@ __global__ __launch_bounds__(256) void schur_inverse_kernel(const T* __restrict_
 }
 
 template <class T>
-__global__ void count_zero_kernel(const T* __restrict__ B, int64_t count, int* out)
+__global__ void count_zero_kernel(const T* __restrict__ A, int64_t na, const T* __restrict__ B, int64_t nb,
+                                  unsigned long long* out)
 {
-    int local = 0;
+    // zeros in A[0, na) and B[0, nb), one grid-stride pass over both ranges
+    unsigned local = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) local += (B[i] == (T)0);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += stride)
+        local += (i < na ? A[i] : B[i - na]) == (T)0;
     for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
-    if ((threadIdx.x & 63) == 0 && local) atomicAdd(out, local);
+    if ((threadIdx.x & 63) == 0 && local) atomicAdd(out, (unsigned long long)local);
 }
 
 template <class T>
@@ -852,22 +855,22 @@ void launch_reduced_factor(const T* WV, int k, int nb, T* S, T* Sinv, int safe, 
 }
 
 template <class T>
-int64_t count_band_zeros(const T* B, int64_t count, int* d_tmp, hipStream_t st)
+int64_t count_band_zeros(const T* A, int64_t na, const T* B, int64_t nb, unsigned long long* d_cnt, hipStream_t st)
 {
-    SAP_HIP_CHECK(hipMemsetAsync(d_tmp, 0, sizeof(int), st));
-    int64_t g = (count + 255) / 256;
+    SAP_HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), st));
+    int64_t g = (na + nb + 255) / 256;
     if (g > 8192) g = 8192;
     if (g < 1) g = 1;
-    hipLaunchKernelGGL(count_zero_kernel<T>, dim3((unsigned)g), dim3(256), 0, st, B, count, d_tmp);
+    hipLaunchKernelGGL(count_zero_kernel<T>, dim3((unsigned)g), dim3(256), 0, st, A, na, B, nb, d_cnt);
     SAP_LAUNCH_CHECK();
-    int h = 0;
-    SAP_HIP_CHECK(hipMemcpyAsync(&h, d_tmp, sizeof(int), hipMemcpyDeviceToHost, st));
+    unsigned long long h = 0;
+    SAP_HIP_CHECK(hipMemcpyAsync(&h, d_cnt, sizeof(h), hipMemcpyDeviceToHost, st));
     SAP_HIP_CHECK(hipStreamSynchronize(st));
-    return h;
+    return (int64_t)h;
 }
 
 #define SAP_INST(T)                                                                                          \
-    template int64_t count_band_zeros<T>(const T*, int64_t, int*, hipStream_t);                               \
+    template int64_t count_band_zeros<T>(const T*, int64_t, const T*, int64_t, unsigned long long*, hipStream_t); \
     template void launch_band_factor<T>(T*, T*, int, const FactorJob*, int, int64_t, hipStream_t);                     \
     template void launch_zero_pivot_check<T>(const T*, int, int64_t, int64_t, int*, hipStream_t);             \
     template void launch_extract_coupling<T>(const T*, int, PartInfo, T*, T*, hipStream_t);                   \

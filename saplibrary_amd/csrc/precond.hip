@@ -30,11 +30,12 @@ void Precond<T>::factor(const std::vector<FactorJob>& jobs)
 }
 
 template <class T>
-bool Precond<T>::zero_pivots(const T* band, int64_t row0, int64_t rows)
+bool Precond<T>::zero_pivots(const T* band, int64_t row0, int64_t rows, const T* band1, int64_t row1, int64_t rows1)
 {
     flag_.resize(1);
     SAP_HIP_CHECK(hipMemsetAsync(flag_.get(), 0, sizeof(int), stream));
     launch_zero_pivot_check<T>(band, k, row0, rows, flag_.get(), stream);
+    if (band1) launch_zero_pivot_check<T>(band1, k, row1, rows1, flag_.get(), stream);
     int h = 0;
     SAP_HIP_CHECK(hipMemcpyAsync(&h, flag_.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
     SAP_HIP_CHECK(hipStreamSynchronize(stream));
@@ -106,14 +107,16 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
     src_count_ = n_ * (2 * (int64_t)k_ + 1);
     upd_map_.release();
     upd_scale_.release();
-    Events ev;
-    SAP_HIP_CHECK(hipEventRecord(ev.e[0], stream));
+    tm_.clear();
+    tm_.mark(0, stream);
     {
         ProfScope ps(prof, "band_to_colmajor");
-        launch_band_to_colmajor<T>(n, k, d_band, B_.get(), stream);
+        // the UL factorizations' copy of the band is written in the same pass
+        launch_band_to_colmajor<T>(n, k, d_band, B_.get(), stream, Bul_.get());
+        bul_filled_ = Bul_.get() != nullptr;
     }
-    SAP_HIP_CHECK(hipEventRecord(ev.e[1], stream));
-    time_toBanded = elapsed_ms(ev.e[0], ev.e[1]);
+    tm_.mark(1, stream);
+    tm_.span(&time_toBanded, 0, 1);
     return factor_band();
 }
 
@@ -123,6 +126,7 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
 {
     Events ev;
     var_mode_ = false;
+    tm_.clear();
     if (!rc.reorder) {
         reorder_ = scale_ = false;
         row_perm_.release(); col_reord_.release(); row_scale_.release(); col_scale_.release();
@@ -144,14 +148,14 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
         src_ci_ = d_ci;
         upd_map_.release();
         upd_scale_.release();
-        SAP_HIP_CHECK(hipEventRecord(ev.e[0], stream));
+        tm_.mark(0, stream);
         {
             ProfScope ps(prof, "csr_to_band");
             SAP_HIP_CHECK(hipMemsetAsync(B_.get(), 0, B_.size() * sizeof(T), stream));
             launch_csr_to_band<double, T>(n, k, d_rp, d_ci, d_val, B_.get(), stream);
         }
-        SAP_HIP_CHECK(hipEventRecord(ev.e[1], stream));
-        time_toBanded = elapsed_ms(ev.e[0], ev.e[1]);
+        tm_.mark(1, stream);
+        tm_.span(&time_toBanded, 0, 1);
         return factor_band();
     }
 
@@ -272,12 +276,13 @@ bool Precond<T>::update(const double* d_entries, int64_t count, int* rc)
         std::chrono::steady_clock::now().time_since_epoch()).count();
     time_transfer = 0;
     time_offDiags = time_bandLU = time_bandUL = time_assembly = time_fullLU = 0;
-    Events ev;
-    SAP_HIP_CHECK(hipEventRecord(ev.e[0], stream));
+    tm_.clear();
+    tm_.mark(0, stream);
     {
         ProfScope ps(prof, "update_band");
         if (src_ == Source::band) {
-            launch_band_to_colmajor<T>(n, k, d_entries, B_.get(), stream);
+            launch_band_to_colmajor<T>(n, k, d_entries, B_.get(), stream, Bul_.get());
+            bul_filled_ = Bul_.get() != nullptr;
         } else {
             SAP_HIP_CHECK(hipMemsetAsync(B_.get(), 0, B_.size() * sizeof(T), stream));
             if (src_ == Source::csr)
@@ -286,8 +291,8 @@ bool Precond<T>::update(const double* d_entries, int64_t count, int* rc)
                 launch_update_band<T>(count, d_entries, upd_map_.get(), upd_scale_.get(), B_.get(), stream);
         }
     }
-    SAP_HIP_CHECK(hipEventRecord(ev.e[1], stream));
-    time_toBanded = elapsed_ms(ev.e[0], ev.e[1]);
+    tm_.mark(1, stream);
+    tm_.span(&time_toBanded, 0, 1);
     time_cpu_assemble = std::chrono::duration<double, std::milli>(
         std::chrono::steady_clock::now().time_since_epoch()).count() - t0;
     *rc = factor_band();
@@ -297,19 +302,31 @@ bool Precond<T>::update(const double* d_entries, int64_t count, int* rc)
 template <class T>
 int Precond<T>::factor_band()
 {
+    const int rc = factor_band_steps();
+    bul_filled_ = false;
+    tm_.resolve();
+    return rc;
+}
+
+template <class T>
+int Precond<T>::factor_band_steps()
+{
     const PartInfo pi{n, P};
     const int64_t w = 2 * k + 1;
-    Events ev;
-    hipEvent_t e1 = ev.e[1], e2 = ev.e[2], e3 = ev.e[3], e4 = ev.e[4], e5 = ev.e[5];
-
-    offB_.release(); offC_.release(); WV_.release(); S_.release(); Sinv_.release(); corr_.release();
+    // timing events: 2 before the coupling blocks, 3 before and 4 after the
+    // factorization, 5 and 6 around the spike tips, 7 after the reduced blocks
+    cnt_.resize(1);
     g_.resize(n);
     scr0_.resize(n);
-    scr1_.release();
 
+    if (!spike_active()) {
+        // buffers of the SaP-C steps are not needed (and not reported)
+        offB_.release(); offC_.release(); WV_.release(); S_.release(); Sinv_.release(); corr_.release();
+        scr1_.release();
+    }
     if (k == 0 || cfg.precondType == SAP_None) {
         if (cfg.precondType != SAP_None)
-            actual_nnz = n - count_band_zeros<T>(B_.get(), n, flag_.get(), stream);
+            actual_nnz = n - count_band_zeros<T>(B_.get(), n, (const T*)nullptr, 0, cnt_.get(), stream);
         return 0;
     }
 
@@ -325,12 +342,13 @@ int Precond<T>::factor_band()
             pi.bounds(p, s, len);
             jobs.push_back({s, len, 1, boost, boost_last, 0});
         }
-        SAP_HIP_CHECK(hipEventRecord(e2, stream));
+        tm_.mark(3, stream);
         factor(jobs);
-        SAP_HIP_CHECK(hipEventRecord(e3, stream));
-        time_bandLU = elapsed_ms(e2, e3);
+        tm_.mark(4, stream);
+        tm_.span(&time_bandLU, 3, 4);
         if (!cfg.safeFactorization && zero_pivots(B_.get(), 0, n)) return SAP_ERR_ZERO_PIVOT;
-        actual_nnz = (int64_t)w * n - count_band_zeros<T>(B_.get(), (int64_t)w * n, flag_.get(), stream);
+        actual_nnz = (int64_t)w * n - count_band_zeros<T>(B_.get(), (int64_t)w * n, (const T*)nullptr, 0,
+                                                          cnt_.get(), stream);
         return 0;
     }
 
@@ -339,12 +357,12 @@ int Precond<T>::factor_band()
     offB_.resize(kk * (P - 1));
     offC_.resize(kk * (P - 1));
     WV_.resize(2 * kk * (P - 1));
-    SAP_HIP_CHECK(hipEventRecord(e1, stream));
+    tm_.mark(2, stream);
     {
         ProfScope ps(prof, "extract_coupling");
         launch_extract_coupling<T>(B_.get(), k, pi, offB_.get(), offC_.get(), stream);
     }
-    SAP_HIP_CHECK(hipEventRecord(e2, stream));
+    tm_.mark(3, stream);
 
     // (3) factorization: LU of partitions 0..P-2 (LU_UL) or 0..P-1 (LU_only);
     //     for LU_UL also UL of partitions 1..P-1 on a copy of the band, in the
@@ -353,7 +371,8 @@ int Precond<T>::factor_band()
     int64_t s_last, l_last;
     pi.bounds(P - 1, s_last, l_last);
     if (lu_ul()) {
-        SAP_HIP_CHECK(hipMemcpyAsync(Bul_.get(), B_.get(), sizeof(T) * w * n, hipMemcpyDeviceToDevice, stream));
+        if (!bul_filled_)
+            SAP_HIP_CHECK(hipMemcpyAsync(Bul_.get(), B_.get(), sizeof(T) * w * n, hipMemcpyDeviceToDevice, stream));
         for (int p = 0; p < P - 1; ++p) {
             int64_t s, len;
             pi.bounds(p, s, len);
@@ -366,6 +385,7 @@ int Precond<T>::factor_band()
         }
         scr1_.resize(n);
     } else {
+        scr1_.release();
         for (int p = 0; p < P; ++p) {
             int64_t s, len;
             pi.bounds(p, s, len);
@@ -373,27 +393,27 @@ int Precond<T>::factor_band()
         }
     }
     factor(jobs);
-    SAP_HIP_CHECK(hipEventRecord(e3, stream));
-    time_offDiags = elapsed_ms(e1, e2);
-    time_bandLU = elapsed_ms(e2, e3);
+    tm_.mark(4, stream);
+    tm_.span(&time_offDiags, 2, 3);
+    tm_.span(&time_bandLU, 3, 4);
     if (!cfg.safeFactorization) {
         if (lu_ul()) {
             int64_t s1, l1;
             pi.bounds(1, s1, l1);
-            if (zero_pivots(B_.get(), 0, s_last) || zero_pivots(Bul_.get(), s1, n - s1)) return SAP_ERR_ZERO_PIVOT;
+            if (zero_pivots(B_.get(), 0, s_last, Bul_.get(), s1, n - s1)) return SAP_ERR_ZERO_PIVOT;
         } else if (zero_pivots(B_.get(), 0, n)) {
             return SAP_ERR_ZERO_PIVOT;
         }
     }
 
     // (4) spike tips
-    SAP_HIP_CHECK(hipEventRecord(e3, stream));
+    tm_.mark(5, stream);
     {
         ProfScope ps(prof, "spike_tips");
         launch_spike_tips<T>(B_.get(), Bul_.get(), k, pi, lu_ul() ? 1 : 0, offB_.get(), offC_.get(), WV_.get(),
                              stream);
     }
-    SAP_HIP_CHECK(hipEventRecord(e4, stream));
+    tm_.mark(6, stream);
 
     // (5) reduced blocks: S_i = I - W V_i, LU, inverse
     S_.resize(kk * (P - 1));
@@ -403,14 +423,15 @@ int Precond<T>::factor_band()
         ProfScope ps(prof, "reduced_factor");
         launch_reduced_factor<T>(WV_.get(), k, P - 1, S_.get(), Sinv_.get(), cfg.safeFactorization, stream);
     }
-    SAP_HIP_CHECK(hipEventRecord(e5, stream));
-    time_assembly = elapsed_ms(e3, e4);
-    time_fullLU = elapsed_ms(e4, e5);
+    tm_.mark(7, stream);
+    tm_.span(&time_assembly, 5, 6);
+    tm_.span(&time_fullLU, 6, 7);
     // non-zeros of the factored band (sap/precond.h:1352): the LU part and,
     // for LU_UL, the UL-factored last partition
-    int64_t zeros = count_band_zeros<T>(B_.get(), (int64_t)w * (lu_ul() ? s_last : n), flag_.get(), stream);
-    if (lu_ul())
-        zeros += count_band_zeros<T>(Bul_.get() + (size_t)w * s_last, (int64_t)w * l_last, flag_.get(), stream);
+    const int64_t zeros =
+        lu_ul() ? count_band_zeros<T>(B_.get(), (int64_t)w * s_last, Bul_.get() + (size_t)w * s_last,
+                                      (int64_t)w * l_last, cnt_.get(), stream)
+                : count_band_zeros<T>(B_.get(), (int64_t)w * n, (const T*)nullptr, 0, cnt_.get(), stream);
     actual_nnz = (int64_t)w * n - zeros;
     return 0;
 }

@@ -27,6 +27,49 @@ struct ReorderConfig {
     bool variable = false;         // variableBandwidth (LU_only, every pivot boosted)
 };
 
+// Setup timings: events recorded on the stream while a setup runs, turned
+// into milliseconds once at its end (reading an event waits for it, which
+// would stall the host between the setup's launches).
+class SetupTimer {
+public:
+    SetupTimer() = default;
+    ~SetupTimer()
+    {
+        for (hipEvent_t& e : ev_)
+            if (e) (void)hipEventDestroy(e);
+    }
+    SetupTimer(const SetupTimer&) = delete;
+    SetupTimer& operator=(const SetupTimer&) = delete;
+    static constexpr int N = 8;
+    // record event i (each index at most once per setup)
+    void mark(int i, hipStream_t st)
+    {
+        if (!ev_[i]) SAP_HIP_CHECK(hipEventCreate(&ev_[i]));
+        SAP_HIP_CHECK(hipEventRecord(ev_[i], st));
+    }
+    // *dst = time from event a to event b, filled in by resolve()
+    void span(double* dst, int a, int b) { pend_.push_back({dst, a, b}); }
+    void resolve()
+    {
+        for (const Span& p : pend_) {
+            float ms = 0.f;
+            SAP_HIP_CHECK(hipEventSynchronize(ev_[p.b]));
+            SAP_HIP_CHECK(hipEventElapsedTime(&ms, ev_[p.a], ev_[p.b]));
+            *p.dst = (double)ms;
+        }
+        pend_.clear();
+    }
+    void clear() { pend_.clear(); }
+
+private:
+    struct Span {
+        double* dst;
+        int a, b;
+    };
+    hipEvent_t ev_[N] = {};
+    std::vector<Span> pend_;
+};
+
 template <class T>
 class Precond {
 public:
@@ -85,6 +128,11 @@ private:
     DevBuf<T> B_, Bul_, offB_, offC_, WV_, S_, Sinv_, g_, corr_, scr0_, scr1_;
     DevBuf<FactorJob> jobs_;
     DevBuf<int> flag_;
+    DevBuf<unsigned long long> cnt_;
+    SetupTimer tm_;
+    // the conversion that preceded factor_band() also wrote Bul_ (the copy
+    // the UL factorizations overwrite), so no device copy is needed
+    bool bul_filled_ = false;
     // reordering: row permutation (DB then RCM), column reordering, scalings
     bool reorder_ = false, scale_ = false;
     // variable-bandwidth setup: the preconditioner of the variable-bandwidth
@@ -102,9 +150,13 @@ private:
     DevBuf<T> upd_scale_;                   // DB scale factor per input entry
     void begin_setup(int64_t n_, int k_);
     int factor_band();
+    int factor_band_steps();
     void solve_band(const double* v, double* z);
     void factor(const std::vector<FactorJob>& jobs);
-    bool zero_pivots(const T* band, int64_t row0, int64_t rows);
+    // true when a pivot of rows [row0, row0 + rows) of band, or of rows
+    // [row1, row1 + rows1) of band1 (if given), is below 1e-7 in magnitude
+    bool zero_pivots(const T* band, int64_t row0, int64_t rows, const T* band1 = nullptr, int64_t row1 = 0,
+                     int64_t rows1 = 0);
     double elapsed_ms(hipEvent_t a, hipEvent_t b);
 };
 

@@ -132,7 +132,7 @@ void launch_gen_banded(int64_t n, int k, double dom, uint64_t seed, double* band
 void launch_gen_vector(int64_t n, uint64_t seed, double* x, hipStream_t st);
 void launch_band_spmv(int64_t n, int k, const double* band, const double* x, double* y, hipStream_t st);
 template <class T>
-void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st);
+void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st, T* B2 = nullptr);
 template <class Tin, class T>   // B zero-filled beforehand
 void launch_csr_to_band(int64_t n, int k, const int* rp, const int* ci, const Tin* vals, T* B, hipStream_t st);
 // B[map[e]] = (T)entries[e] * scale[e] for map[e] >= 0 (scale may be null)
@@ -194,7 +194,8 @@ void launch_sweep_full(const T* B, const T* Bul, int k, PartInfo pi, int last_is
                        double* outd, T* outt, T* scr, hipStream_t st);
 template <class T>
 void launch_diag_divide(const T* B, int64_t n, const double* v, double* out, hipStream_t st);
+// zeros in A[0, na) and B[0, nb) (B may be null when nb = 0)
 template <class T>
-int64_t count_band_zeros(const T* B, int64_t count, int* d_tmp, hipStream_t st);
+int64_t count_band_zeros(const T* A, int64_t na, const T* B, int64_t nb, unsigned long long* d_cnt, hipStream_t st);
 
 }  // namespace sap_amd
