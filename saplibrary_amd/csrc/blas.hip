@@ -26,24 +26,45 @@ __device__ inline double block_reduce_sum(double v)
     return s;
 }
 
-__global__ __launch_bounds__(RED_THREADS) void dot_partial_kernel(int64_t n, const double* __restrict__ a,
-                                                                  const double* __restrict__ b,
-                                                                  double* __restrict__ partials)
+// Up to three dot products in one pass (q = 0 .. nd-1: a_q . b_q); each
+// keeps the summation order of a single one, so the values do not depend on
+// what it is computed with.
+struct DotArgs {
+    const double* a[3];
+    const double* b[3];
+};
+
+template <int ND>
+__global__ __launch_bounds__(RED_THREADS) void dot_partial_kernel(int64_t n, DotArgs d, double* __restrict__ partials)
 {
-    double s = 0.0;
+    double s[ND];
+#pragma unroll
+    for (int q = 0; q < ND; ++q) s[q] = 0.0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) s = fma(a[i], b[i], s);
-    s = block_reduce_sum(s);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+#pragma unroll
+        for (int q = 0; q < ND; ++q) s[q] = fma(d.a[q][i], d.b[q][i], s[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        const double t = block_reduce_sum(s[q]);
+        if (threadIdx.x == 0) partials[q * RED_BLOCKS + blockIdx.x] = t;
+        __syncthreads();   // the reduction's LDS is reused by the next q
+    }
 }
 
-__global__ __launch_bounds__(RED_THREADS) void reduce_final_kernel(int m, const double* __restrict__ partials,
+// out[q] = sum of the partials of dot q; out may be host memory mapped into
+// the device's address space (read by the host after a stream synchronize)
+__global__ __launch_bounds__(RED_THREADS) void reduce_final_kernel(int m, int nd, const double* __restrict__ partials,
                                                                    double* __restrict__ out)
 {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) s += partials[i];
-    s = block_reduce_sum(s);
-    if (threadIdx.x == 0) *out = s;
+    for (int q = 0; q < nd; ++q) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < m; i += blockDim.x) s += partials[q * m + i];
+        s = block_reduce_sum(s);
+        if (threadIdx.x == 0) out[q] = s;
+        __syncthreads();
+    }
 }
 
 __global__ void axpby_kernel(int64_t n, double alpha, const double* __restrict__ x, double beta,
@@ -76,14 +97,25 @@ inline unsigned grid_for(int64_t n)
 
 }  // namespace
 
-int dot_partials_size() { return RED_BLOCKS; }
+int dot_partials_size() { return 3 * RED_BLOCKS; }
 
-void launch_dot(int64_t n, const double* a, const double* b, double* d_partials, double* d_out, hipStream_t st)
+void launch_dots(int64_t n, int nd, const double* const* a, const double* const* b, double* d_partials,
+                 double* out, hipStream_t st)
 {
-    hipLaunchKernelGGL(dot_partial_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, a, b, d_partials);
+    if (nd < 1 || nd > 3) throw Error(SAP_ERR_BADARG, "launch_dots: 1 to 3 products");
+    DotArgs d{};
+    for (int q = 0; q < nd; ++q) { d.a[q] = a[q]; d.b[q] = b[q]; }
+    if (nd == 1) hipLaunchKernelGGL(dot_partial_kernel<1>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, d, d_partials);
+    else if (nd == 2) hipLaunchKernelGGL(dot_partial_kernel<2>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, d, d_partials);
+    else hipLaunchKernelGGL(dot_partial_kernel<3>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, n, d, d_partials);
     SAP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(RED_THREADS), 0, st, RED_BLOCKS, d_partials, d_out);
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(RED_THREADS), 0, st, RED_BLOCKS, nd, d_partials, out);
     SAP_LAUNCH_CHECK();
+}
+
+void launch_dot(int64_t n, const double* a, const double* b, double* d_partials, double* out, hipStream_t st)
+{
+    launch_dots(n, 1, &a, &b, d_partials, out, st);
 }
 
 void launch_axpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* z, hipStream_t st)

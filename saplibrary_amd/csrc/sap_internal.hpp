@@ -150,6 +150,9 @@ void launch_permute_scale(int64_t n, const double* v, const int* perm, const T* 
 
 // blas.hip (FP64 vectors, results of reductions left in device memory)
 void launch_dot(int64_t n, const double* a, const double* b, double* d_partials, double* d_out, hipStream_t st);
+// out[q] = a[q] . b[q] for q < nd (1..3), one pass; out may be mapped host memory
+void launch_dots(int64_t n, int nd, const double* const* a, const double* const* b, double* d_partials,
+                 double* out, hipStream_t st);
 void launch_axpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* z, hipStream_t st);
 void launch_copy(int64_t n, const double* x, double* y, hipStream_t st);
 void launch_fill(int64_t n, double v, double* x, hipStream_t st);

@@ -184,8 +184,9 @@ public:
     int code = 0;
     std::string message;
 
-    DevBuf<double> work_, partials_, dscalar_;
+    DevBuf<double> work_, partials_;
     double* hscalar_ = nullptr;
+    double* dscalar_map_ = nullptr;   // hscalar_ in the device's address space
     std::vector<double*> vecs_;
     size_t vec_next_ = 0;
     // b - A x of the last BiCGStab(L) solve when it converged (on a passed
@@ -197,7 +198,10 @@ public:
     {
         std::memset(&stats, 0, sizeof(stats));
         stats.rhsNorm = stats.residualNorm = stats.relResidualNorm = std::numeric_limits<double>::max();
-        SAP_HIP_CHECK(hipHostMalloc((void**)&hscalar_, sizeof(double) * 8, hipHostMallocDefault));
+        // reduction results land here directly (mapped, coherent pinned memory)
+        SAP_HIP_CHECK(hipHostMalloc((void**)&hscalar_, sizeof(double) * 8,
+                                    hipHostMallocMapped | hipHostMallocCoherent));
+        SAP_HIP_CHECK(hipHostGetDevicePointer((void**)&dscalar_map_, hscalar_, 0));
         prof.enabled = opt.profile != 0;
         make_precond();
     }
@@ -241,18 +245,24 @@ public:
         for (int i = 0; i < count; ++i) vecs_.push_back(work_.get() + (size_t)i * n);
         vec_next_ = 0;
         partials_.resize(dot_partials_size());
-        dscalar_.resize(8);
     }
     double* vec() { return vecs_.at(vec_next_++); }
 
     double dot(const double* a, const double* b)
     {
-        launch_dot(n, a, b, partials_.get(), dscalar_.get(), stream);
-        SAP_HIP_CHECK(hipMemcpyAsync(hscalar_, dscalar_.get(), sizeof(double), hipMemcpyDeviceToHost, stream));
+        launch_dot(n, a, b, partials_.get(), dscalar_map_, stream);
         SAP_HIP_CHECK(hipStreamSynchronize(stream));
         return hscalar_[0];
     }
     double nrm2(const double* a) { return std::sqrt(dot(a, a)); }
+    // norms of up to three vectors with one pass and one synchronization
+    // (each value as nrm2 gives it)
+    void nrm2s(int m, const double* const* v, double* out)
+    {
+        launch_dots(n, m, v, v, partials_.get(), dscalar_map_, stream);
+        SAP_HIP_CHECK(hipStreamSynchronize(stream));
+        for (int q = 0; q < m; ++q) out[q] = std::sqrt(hscalar_[q]);
+    }
     void axpby(const double* x, const double* y, double* z, double alpha, double beta)
     {
         launch_axpby(n, alpha, x, beta, y, z, stream);
@@ -279,6 +289,7 @@ public:
     template <int L>
     void bicgstabl(double* x, const double* b, BiCGStabLMonitor& monitor, bool x_is_zero, bool correction)
     {
+        // for a zero guess the initial residual is b, whose norm the monitor holds
         const double eps = 1e-20;
         double rho0 = 1, alpha = 0, omega = 1, rho1;
         double* r0 = vec();
@@ -309,7 +320,7 @@ public:
         copy(u, uu[0]);
         copy(x, xx);
         copy(r, rr[0]);
-        double r_norm_min = nrm2(r);
+        double r_norm_min = x_is_zero ? monitor.rhsNorm : nrm2(r);
         double r_norm = r_norm_min;
         double r_norm_act = r_norm;
         bool pxx_valid = false;    // Pxx == M^{-1} xx for the current xx
@@ -350,7 +361,12 @@ public:
                 if (gam == 0) { monitor.stop(-11, "gamma is zero"); break; }
                 alpha = rho0 / gam;
                 for (int i = 0; i <= j; ++i) axpy(uu[i + 1], rr[i], -alpha);
-                r_norm_act = r_norm = nrm2(rr[0]);
+                double nv[3];
+                {
+                    const double* vs[3] = {rr[0], uu[0], xx};
+                    nrm2s(3, vs, nv);   // |rr(0)|, and |uu(0)|, |xx| for the stagnation test below
+                }
+                r_norm_act = r_norm = nv[0];
                 // rr(j+1) <- A P^{-1} rr(j) of the recursive rr(j).  The
                 // reference forms it before the true-residual check; here it
                 // is formed after a check that does not end the solve (the
@@ -358,7 +374,7 @@ public:
                 // first), which gives the same iterates and saves one
                 // preconditioner application and one SpMV on the converging
                 // step.
-                if (std::fabs(alpha) * nrm2(uu[0]) < eps * nrm2(xx)) monitor.incrementStag();
+                if (std::fabs(alpha) * nv[1] < eps * nv[2]) monitor.incrementStag();
                 else monitor.resetStag();
                 axpy(uu[0], xx, alpha);
                 pxx_valid = false;
@@ -409,8 +425,13 @@ public:
                 gamma_pp[j] = gamma[j + 1];
                 for (int i = j + 1; i < L; ++i) gamma_pp[j] += tao[j][i] * gamma[i + 1];
             }
-            if (std::fabs(gamma[1]) * nrm2(rr[0]) < eps * nrm2(xx)) monitor.incrementStag();
-            else monitor.resetStag();
+            {
+                double nv[2];
+                const double* vs[2] = {rr[0], xx};
+                nrm2s(2, vs, nv);
+                if (std::fabs(gamma[1]) * nv[0] < eps * nv[1]) monitor.incrementStag();
+                else monitor.resetStag();
+            }
             axpy(rr[0], xx, gamma[1]);
             axpy(rr[L], rr[0], -gamma_prime[L]);
             axpy(uu[L], uu[0], -gamma[L]);
@@ -427,8 +448,13 @@ public:
             bool brk = false;
             for (int j = 1; j < L; ++j) {
                 axpy(uu[j], uu[0], -gamma[j]);
-                if (std::fabs(gamma_pp[j]) * nrm2(rr[j]) < eps * nrm2(xx)) monitor.incrementStag();
-                else monitor.resetStag();
+                {
+                    double nv[2];
+                    const double* vs[2] = {rr[j], xx};
+                    nrm2s(2, vs, nv);
+                    if (std::fabs(gamma_pp[j]) * nv[0] < eps * nv[1]) monitor.incrementStag();
+                    else monitor.resetStag();
+                }
                 axpy(rr[j], xx, gamma_pp[j]);
                 axpy(rr[j], rr[0], -gamma_prime[j]);
                 pxx_valid = false;
@@ -553,8 +579,9 @@ public:
     bool use_minres() const { return opt.solverType == SAP_MINRES; }
 
     // One Krylov solve of A x = b (the reference's solve path).
+    // b_norm: |b| when the caller has it (as nrm2 gives it), else negative
     int krylov(double* x, const double* b, double rtol, double atol, bool x_is_zero, bool correction, float* its,
-               double* rnorm, double* rhs_norm)
+               double* rnorm, double* rhs_norm, double b_norm = -1.0)
     {
         last_res_ = nullptr;
         if (use_minres()) {
@@ -568,7 +595,7 @@ public:
             return mmon.code;
         }
         bmon.reset(opt.maxNumIterations, 8, rtol, atol);   // maxmSteps = 8 (sap/solver.h:314-319)
-        bmon.init(nrm2(b));
+        bmon.init(b_norm >= 0 ? b_norm : nrm2(b));
         if (opt.solverType == SAP_BiCGStab1) bicgstabl<1>(x, b, bmon, x_is_zero, correction);
         else bicgstabl<2>(x, b, bmon, x_is_zero, correction);
         *its += bmon.iterations;
@@ -716,7 +743,12 @@ public:
             double* Ax = vec();
             double* dv = vec();
             const size_t mark = vec_next_;
-            bn = nrm2(bv);
+            double bx[2];
+            {
+                const double* vs[2] = {bv, xv};
+                nrm2s(2, vs, bx);
+            }
+            bn = bx[0];
             const double tol = opt.absTol + opt.relTol * bn;
             // r = b - A x.  A*0 = 0 exactly, so a zero initial guess needs no
             // SpMV; after an inner solve that converged, r is that solve's
@@ -724,8 +756,9 @@ public:
             // update), and convergence is only declared on b - A x formed
             // afresh.  After an inner solve that did not converge, b - A x
             // is formed afresh.
-            bool have_r = nrm2(xv) == 0.0;
+            bool have_r = bx[1] == 0.0;
             if (have_r) copy(bv, r);
+            double r_known = bn;   // |r| while have_r (the norm nrm2 gives for it)
             bool recurred = false;
             code = -1;
             for (int outer = 0;; ++outer) {
@@ -733,8 +766,10 @@ public:
                     spmv(xv, Ax);
                     axpby(bv, Ax, r, 1.0, -1.0);
                     recurred = false;
+                    rn = nrm2(r);
+                } else {
+                    rn = r_known;
                 }
-                rn = nrm2(r);
                 if (!std::isnan(rn) && rn <= tol && recurred) {
                     spmv(xv, Ax);
                     axpby(bv, Ax, r, 1.0, -1.0);
@@ -749,13 +784,14 @@ public:
                 launch_fill(n, 0.0, dv, stream);
                 vec_next_ = mark;
                 double inner_r, inner_b;
-                const int c = krylov(dv, r, rt, 0.0, true, true, &its, &inner_r, &inner_b);
+                const int c = krylov(dv, r, rt, 0.0, true, true, &its, &inner_r, &inner_b, rn);
                 stats.numRefine++;
                 axpy(dv, xv, 1.0);
                 if (c == -2) { code = -2; break; }
                 if (last_res_) {
                     copy(last_res_, r);
                     have_r = recurred = true;
+                    r_known = inner_r;   // the monitor's norm of that residual
                 }
             }
             message = code == 1 ? "Converged"
