@@ -7,6 +7,7 @@
 // the same storage read backwards: with X = (s+len)*(2k+1) - 1, the reversed
 // block's element (r, c) lives at X - (c*(2k+1) + k + r - c).  So one kernel
 // serves both directions.
+#include "mfma.hpp"
 #include "sap_internal.hpp"
 
 namespace sap_amd {
@@ -472,6 +473,184 @@ __global__ __launch_bounds__(256) void spike_tips_kernel(const T* __restrict__ B
     }
 }
 
+// Spike tips on the matrix cores: the sweeps of corner_solve (unit L, D,
+// unit U' in the factorization's order) blocked by 32 columns.  Per block,
+// wavefront 0 solves the 32 x 32 unit-triangular diagonal block (one
+// right-hand side per lane, the factor column block broadcast from LDS),
+// then every wavefront applies the block to the remaining rows with
+// 16 x 16 MFMA tiles, Y_rest -= F_blk Y_blk.  The factor's 32 columns are
+// staged in LDS transposed (column jj, rows t contiguous), loaded from HBM
+// one block ahead.  Same operator as corner_solve; the sums are grouped by
+// block, so the values agree to rounding.
+template <class T>
+struct StkCfg;
+template <>
+struct StkCfg<float> {
+    static constexpr int NR = 64;   // right-hand sides per workgroup
+};
+template <>
+struct StkCfg<double> {
+    static constexpr int NR = 32;
+};
+constexpr int STK_THREADS = 512;
+constexpr int STK_IT = 13;          // rows per staged column per lane: k <= 32 * 13
+
+template <class T>
+struct StkLayout {
+    static constexpr int NR = StkCfg<T>::NR;
+    static constexpr int YS = NR + 16;   // Y row stride: the MFMA B operand reads four rows without bank conflicts
+    int kpad, lts;
+    __host__ __device__ explicit StkLayout(int k)
+    {
+        kpad = ((k + 15) / 16) * 16 + 32;   // rows a 32-row block or 16-row tile may touch past k (zero)
+        // column stride of the staged factor block: the A operand reads four columns
+        lts = sizeof(T) == 4 ? ((kpad + 47) / 64) * 64 + 16 : ((kpad + 15) / 32) * 32 + 16;
+    }
+    __host__ __device__ size_t y_elems() const { return (size_t)kpad * YS; }
+    __host__ __device__ size_t bytes() const { return sizeof(T) * (y_elems() + (size_t)32 * lts); }
+};
+
+template <class T>
+__global__ __launch_bounds__(STK_THREADS) void spike_tips_mfma_kernel(const T* __restrict__ B,
+                                                                      const T* __restrict__ Bul, int k, PartInfo pi,
+                                                                      const T* __restrict__ offB,
+                                                                      const T* __restrict__ offC,
+                                                                      T* __restrict__ WV)
+{
+    using MF = Mfma16<T>;
+    using acc_t = typename MF::acc_t;
+    constexpr int NR = StkCfg<T>::NR, YS = StkLayout<T>::YS, NCT = NR / 16;
+    const StkLayout<T> L(k);
+    extern __shared__ unsigned char smem_raw[];
+    T* const Y = reinterpret_cast<T*>(smem_raw);   // Y[t * YS + r], t < kpad
+    T* const LT = Y + L.y_elems();                  // LT[jj * lts + t] = F(t, jb + jj)
+    const int lts = L.lts;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = blockIdx.x, which = blockIdx.y, r0 = blockIdx.z * NR;
+    const int nrhs = min(NR, k - r0);
+    const int64_t w = 2 * k + 1;
+    const size_t kk = (size_t)k * k;
+    int64_t s0, l0, s1, l1;
+    pi.bounds(i, s0, l0);
+    pi.bounds(i + 1, s1, l1);
+    const T* const F = which == 0 ? B : Bul;
+    const int64_t o = which == 0 ? s0 + l0 - k : s1;   // first row / column of the corner block
+    auto el = [&](int t, int j) -> T { return F[(o + j) * w + k + t - j]; };
+
+    // Y = the coupling block's columns r0 .. r0 + nrhs - 1 (zero elsewhere), LT = 0
+    const T* src = (which == 0 ? offB : offC) + i * kk;
+    for (int e = tid; e < L.kpad * YS; e += STK_THREADS) {
+        const int t = e / YS, r = e - t * YS;
+        Y[e] = (t < k && r < nrhs) ? src[(size_t)(r0 + r) * k + t] : (T)0;
+    }
+    for (int e = tid; e < 32 * lts; e += STK_THREADS) LT[e] = (T)0;
+
+    // block sequence: pass 0 then pass 1, each over ceil(k/32) blocks; V:
+    // lower (forward) then upper (backward); W (UL factors): upper then lower
+    const int nblk = (k + 31) / 32;
+    auto block_jb = [&](int q) -> int {   // q-th block of the sequence
+        const bool second = q >= nblk;
+        const int b = second ? q - nblk : q;
+        const bool lower = (which == 0) != second;
+        return lower ? 32 * b : 32 * (nblk - 1 - b);
+    };
+    auto block_lower = [&](int q) -> bool { return (which == 0) != (q >= nblk); };
+    // staging: lanes 0-31 / 32-63 of wavefront v take columns 2v, 2v+1 (+16),
+    // rows lane%32 + 32 it
+    T pf[2][STK_IT];
+    auto prefetch = [&](int q) {
+        const int jb = block_jb(q);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int jj = 2 * wave + (lane >> 5) + 16 * h;
+            const int j = jb + jj;
+#pragma unroll
+            for (int it = 0; it < STK_IT; ++it) {
+                const int t = (lane & 31) + 32 * it;
+                pf[h][it] = (j < k && t < k) ? el(t, j) : (T)0;
+            }
+        }
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int jj = 2 * wave + (lane >> 5) + 16 * h;
+#pragma unroll
+            for (int it = 0; it < STK_IT; ++it) {
+                const int t = (lane & 31) + 32 * it;
+                if (t < k) LT[jj * lts + t] = pf[h][it];
+            }
+        }
+    };
+    const int nseq = 2 * nblk;
+    prefetch(0);
+    __syncthreads();
+    for (int q = 0; q < nseq; ++q) {
+        const int jb = block_jb(q);
+        const bool lower = block_lower(q);
+        stage();
+        __syncthreads();
+        if (q + 1 < nseq) prefetch(q + 1);
+        if (wave == 0 && lane < NR) {
+            // diagonal block, one right-hand side per lane
+            T y[32];
+#pragma unroll
+            for (int tt = 0; tt < 32; ++tt) y[tt] = Y[(jb + tt) * YS + lane];
+            if (lower) {
+#pragma unroll
+                for (int jj = 0; jj < 31; ++jj) {
+                    const T yj = y[jj];
+#pragma unroll
+                    for (int tt = jj + 1; tt < 32; ++tt) y[tt] -= LT[jj * lts + jb + tt] * yj;
+                }
+            } else {
+#pragma unroll
+                for (int jj = 31; jj > 0; --jj) {
+                    const T yj = y[jj];
+#pragma unroll
+                    for (int tt = 0; tt < jj; ++tt) y[tt] -= LT[jj * lts + jb + tt] * yj;
+                }
+            }
+#pragma unroll
+            for (int tt = 0; tt < 32; ++tt) Y[(jb + tt) * YS + lane] = y[tt];
+        }
+        __syncthreads();
+        // the rest of the rows: below the block (lower) or above it (upper)
+        const int t_lo = lower ? jb + 32 : 0;
+        const int nrow = lower ? k - t_lo : jb;
+        const int nrt = nrow > 0 ? (nrow + 15) / 16 : 0;
+        const int g = lane >> 4, c = lane & 15;
+        for (int f = wave; f < nrt * NCT; f += STK_THREADS / 64) {
+            const int t0 = t_lo + 16 * (f / NCT), rc0 = 16 * (f % NCT);
+            acc_t acc;
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) acc[qq] = Y[(t0 + MF::row(lane, qq)) * YS + rc0 + c];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const T a = -LT[(4 * m + g) * lts + t0 + c];
+                const T b = Y[(jb + 4 * m + g) * YS + rc0 + c];
+                acc = MF::mma(a, b, acc);
+            }
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq) Y[(t0 + MF::row(lane, qq)) * YS + rc0 + c] = acc[qq];
+        }
+        __syncthreads();
+        if (q == nblk - 1) {
+            // between the passes: Y /= D
+            for (int e = tid; e < k * NR; e += STK_THREADS) {
+                const int t = e / NR, r = e - t * NR;
+                Y[t * YS + r] /= el(t, t);
+            }
+            __syncthreads();
+        }
+    }
+    T* dst = WV + (2 * i + which) * kk;
+    for (int e = tid; e < k * nrhs; e += STK_THREADS) {
+        const int a = e % k, r = e / k;
+        dst[(size_t)(r0 + r) * k + a] = Y[a * YS + r];
+    }
+}
+
 // W tips for LU_only: the full partition sweep with k right-hand sides
 // (calculateSpikes_const, sap/precond.h:5910-5948), RC columns per
 // workgroup, working vector in global scratch (len x RC, row-major).
@@ -807,6 +986,16 @@ void launch_spike_tips(const T* B, const T* Bul, int k, PartInfo pi, int fact_lu
                        T* WV, hipStream_t st)
 {
     if (pi.P < 2) return;
+    if (fact_lu_ul && k <= 32 * STK_IT && StkLayout<T>(k).bytes() <= LDS_LIMIT) {
+        const size_t lds = StkLayout<T>(k).bytes();
+        SAP_HIP_CHECK(hipFuncSetAttribute((const void*)spike_tips_mfma_kernel<T>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        constexpr int NR = StkCfg<T>::NR;
+        hipLaunchKernelGGL(spike_tips_mfma_kernel<T>, dim3(pi.P - 1, 2, (k + NR - 1) / NR), dim3(STK_THREADS), lds,
+                           st, B, Bul, k, pi, offB, offC, WV);
+        SAP_LAUNCH_CHECK();
+        return;
+    }
     const size_t lds = sizeof(T) * (size_t)k * (RC + 1);
     if (lds > LDS_LIMIT) throw Error(SAP_ERR_UNSUPPORTED, "half-bandwidth too large for the spike kernel");
     SAP_HIP_CHECK(hipFuncSetAttribute((const void*)spike_tips_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,

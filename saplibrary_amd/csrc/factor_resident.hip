@@ -4,6 +4,7 @@
 // Band layout (column-major, as the reference's m_B, sap/device/
 // data_transfer.cuh:440-447): A(r, c) at B[c*(2k+1) + k + r - c]; a UL
 // factorization is the LU factorization of the same storage read backwards.
+#include "mfma.hpp"
 #include "sap_internal.hpp"
 
 #include <climits>
@@ -44,30 +45,6 @@ namespace {
 // B2 to complete.  Factors are stored as L (unit), D, U' like the other
 // factorization kernels; the multipliers use the pivot's reciprocal.
 // ---------------------------------------------------------------------------
-template <class T>
-struct Mfma16;
-
-template <>
-struct Mfma16<float> {
-    typedef float acc_t __attribute__((ext_vector_type(4)));
-    __device__ static inline acc_t mma(float a, float b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    // accumulator register q of lane l holds D[row(l, q)][l % 16]
-    __device__ static inline int row(int l, int q) { return 4 * (l >> 4) + q; }
-};
-
-template <>
-struct Mfma16<double> {
-    typedef double acc_t __attribute__((ext_vector_type(4)));
-    __device__ static inline acc_t mma(double a, double b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    __device__ static inline int row(int l, int q) { return (l >> 4) + 4 * q; }
-};
-
 __device__ inline float lane_bcast(float v, int src)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
