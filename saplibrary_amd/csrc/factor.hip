@@ -510,51 +510,26 @@ struct StkLayout {
     __host__ __device__ size_t bytes() const { return sizeof(T) * (y_elems() + (size_t)32 * lts); }
 };
 
-template <class T>
-__global__ __launch_bounds__(STK_THREADS) void spike_tips_mfma_kernel(const T* __restrict__ B,
-                                                                      const T* __restrict__ Bul, int k, PartInfo pi,
-                                                                      const T* __restrict__ offB,
-                                                                      const T* __restrict__ offC,
-                                                                      T* __restrict__ WV)
+// The blocked sweeps on Y (kpad x YS in LDS, rows >= k zero; LT: 32 x lts,
+// zero): lower_first: Y = U'^{-1} D^{-1} L^{-1} Y, else L^{-1} D^{-1} U'^{-1} Y,
+// with el(t, j) the factor entry of row t, column j (t, j < k).  Called by
+// all STK_THREADS threads after a barrier; ends with one.
+template <class T, class El>
+__device__ void stk_solve(T* Y, T* LT, const StkLayout<T>& L, int k, bool lower_first, El el)
 {
     using MF = Mfma16<T>;
     using acc_t = typename MF::acc_t;
     constexpr int NR = StkCfg<T>::NR, YS = StkLayout<T>::YS, NCT = NR / 16;
-    const StkLayout<T> L(k);
-    extern __shared__ unsigned char smem_raw[];
-    T* const Y = reinterpret_cast<T*>(smem_raw);   // Y[t * YS + r], t < kpad
-    T* const LT = Y + L.y_elems();                  // LT[jj * lts + t] = F(t, jb + jj)
     const int lts = L.lts;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int i = blockIdx.x, which = blockIdx.y, r0 = blockIdx.z * NR;
-    const int nrhs = min(NR, k - r0);
-    const int64_t w = 2 * k + 1;
-    const size_t kk = (size_t)k * k;
-    int64_t s0, l0, s1, l1;
-    pi.bounds(i, s0, l0);
-    pi.bounds(i + 1, s1, l1);
-    const T* const F = which == 0 ? B : Bul;
-    const int64_t o = which == 0 ? s0 + l0 - k : s1;   // first row / column of the corner block
-    auto el = [&](int t, int j) -> T { return F[(o + j) * w + k + t - j]; };
-
-    // Y = the coupling block's columns r0 .. r0 + nrhs - 1 (zero elsewhere), LT = 0
-    const T* src = (which == 0 ? offB : offC) + i * kk;
-    for (int e = tid; e < L.kpad * YS; e += STK_THREADS) {
-        const int t = e / YS, r = e - t * YS;
-        Y[e] = (t < k && r < nrhs) ? src[(size_t)(r0 + r) * k + t] : (T)0;
-    }
-    for (int e = tid; e < 32 * lts; e += STK_THREADS) LT[e] = (T)0;
-
-    // block sequence: pass 0 then pass 1, each over ceil(k/32) blocks; V:
-    // lower (forward) then upper (backward); W (UL factors): upper then lower
+    // block sequence: the first pass then the second, each over ceil(k/32)
+    // blocks (lower passes forward, upper passes backward)
     const int nblk = (k + 31) / 32;
-    auto block_jb = [&](int q) -> int {   // q-th block of the sequence
-        const bool second = q >= nblk;
-        const int b = second ? q - nblk : q;
-        const bool lower = (which == 0) != second;
-        return lower ? 32 * b : 32 * (nblk - 1 - b);
+    auto block_lower = [&](int q) -> bool { return lower_first != (q >= nblk); };
+    auto block_jb = [&](int q) -> int {
+        const int b = q >= nblk ? q - nblk : q;
+        return block_lower(q) ? 32 * b : 32 * (nblk - 1 - b);
     };
-    auto block_lower = [&](int q) -> bool { return (which == 0) != (q >= nblk); };
     // staging: lanes 0-31 / 32-63 of wavefront v take columns 2v, 2v+1 (+16),
     // rows lane%32 + 32 it
     T pf[2][STK_IT];
@@ -562,8 +537,7 @@ __global__ __launch_bounds__(STK_THREADS) void spike_tips_mfma_kernel(const T* _
         const int jb = block_jb(q);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int jj = 2 * wave + (lane >> 5) + 16 * h;
-            const int j = jb + jj;
+            const int j = jb + 2 * wave + (lane >> 5) + 16 * h;
 #pragma unroll
             for (int it = 0; it < STK_IT; ++it) {
                 const int t = (lane & 31) + 32 * it;
@@ -584,7 +558,6 @@ __global__ __launch_bounds__(STK_THREADS) void spike_tips_mfma_kernel(const T* _
     };
     const int nseq = 2 * nblk;
     prefetch(0);
-    __syncthreads();
     for (int q = 0; q < nseq; ++q) {
         const int jb = block_jb(q);
         const bool lower = block_lower(q);
@@ -644,7 +617,71 @@ __global__ __launch_bounds__(STK_THREADS) void spike_tips_mfma_kernel(const T* _
             __syncthreads();
         }
     }
+}
+
+template <class T>
+__global__ __launch_bounds__(STK_THREADS) void spike_tips_mfma_kernel(const T* __restrict__ B,
+                                                                      const T* __restrict__ Bul, int k, PartInfo pi,
+                                                                      const T* __restrict__ offB,
+                                                                      const T* __restrict__ offC,
+                                                                      T* __restrict__ WV)
+{
+    constexpr int NR = StkCfg<T>::NR, YS = StkLayout<T>::YS;
+    const StkLayout<T> L(k);
+    extern __shared__ unsigned char smem_raw[];
+    T* const Y = reinterpret_cast<T*>(smem_raw);   // Y[t * YS + r], t < kpad
+    T* const LT = Y + L.y_elems();                  // LT[jj * lts + t] = F(t, jb + jj)
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x, which = blockIdx.y, r0 = blockIdx.z * NR;
+    const int nrhs = min(NR, k - r0);
+    const int64_t w = 2 * k + 1;
+    const size_t kk = (size_t)k * k;
+    int64_t s0, l0, s1, l1;
+    pi.bounds(i, s0, l0);
+    pi.bounds(i + 1, s1, l1);
+    const T* const F = which == 0 ? B : Bul;
+    const int64_t o = which == 0 ? s0 + l0 - k : s1;   // first row / column of the corner block
+    // Y = the coupling block's columns r0 .. r0 + nrhs - 1 (zero elsewhere), LT = 0
+    const T* src = (which == 0 ? offB : offC) + i * kk;
+    for (int e = tid; e < L.kpad * YS; e += STK_THREADS) {
+        const int t = e / YS, r = e - t * YS;
+        Y[e] = (t < k && r < nrhs) ? src[(size_t)(r0 + r) * k + t] : (T)0;
+    }
+    for (int e = tid; e < 32 * L.lts; e += STK_THREADS) LT[e] = (T)0;
+    __syncthreads();
+    // V: the LU's corner, lower sweep first; W: the UL's corner, upper first
+    stk_solve<T>(Y, LT, L, k, which == 0, [&](int t, int j) -> T { return F[(o + j) * w + k + t - j]; });
     T* dst = WV + (2 * i + which) * kk;
+    for (int e = tid; e < k * nrhs; e += STK_THREADS) {
+        const int a = e % k, r = e / k;
+        dst[(size_t)(r0 + r) * k + a] = Y[a * YS + r];
+    }
+}
+
+// S_i^{-1} from the L D U' factors of S_i (column-major, k x k): the blocked
+// sweeps on identity columns, NR per workgroup.
+template <class T>
+__global__ __launch_bounds__(STK_THREADS) void schur_inverse_mfma_kernel(const T* __restrict__ S, int k,
+                                                                         T* __restrict__ Sinv)
+{
+    constexpr int NR = StkCfg<T>::NR, YS = StkLayout<T>::YS;
+    const StkLayout<T> L(k);
+    extern __shared__ unsigned char smem_raw[];
+    T* const Y = reinterpret_cast<T*>(smem_raw);
+    T* const LT = Y + L.y_elems();
+    const int tid = threadIdx.x;
+    const int i = blockIdx.x, r0 = blockIdx.y * NR;
+    const int nrhs = min(NR, k - r0);
+    const size_t kk = (size_t)k * k;
+    const T* A = S + (size_t)i * kk;
+    for (int e = tid; e < L.kpad * YS; e += STK_THREADS) {
+        const int t = e / YS, r = e - t * YS;
+        Y[e] = (t < k && r < nrhs && t == r0 + r) ? (T)1 : (T)0;
+    }
+    for (int e = tid; e < 32 * L.lts; e += STK_THREADS) LT[e] = (T)0;
+    __syncthreads();
+    stk_solve<T>(Y, LT, L, k, true, [&](int t, int j) -> T { return A[(size_t)j * k + t]; });
+    T* dst = Sinv + (size_t)i * kk;
     for (int e = tid; e < k * nrhs; e += STK_THREADS) {
         const int a = e % k, r = e / k;
         dst[(size_t)(r0 + r) * k + a] = Y[a * YS + r];
@@ -809,7 +846,8 @@ __device__ inline T wave_bcast(const T (&y)[4], int m)
 }
 
 template <class T>
-__global__ __launch_bounds__(1024) void schur_lu_inv_kernel(T* __restrict__ S, T* __restrict__ Sinv, int k, int safe)
+__global__ __launch_bounds__(1024) void schur_lu_inv_kernel(T* __restrict__ S, T* __restrict__ Sinv, int k, int safe,
+                                                            int with_inverse)
 {
     extern __shared__ unsigned char smem_raw[];
     T* M = reinterpret_cast<T*>(smem_raw);   // S_i, column-major, leading dimension k
@@ -848,6 +886,7 @@ __global__ __launch_bounds__(1024) void schur_lu_inv_kernel(T* __restrict__ S, T
     }
     __syncthreads();
     for (size_t e = tid; e < kk; e += blockDim.x) A[e] = M[e];
+    if (!with_inverse) return;
     T* X = Sinv + (size_t)blockIdx.x * kk;
     for (int r = wave; r < k; r += nw) {
         T y[4];
@@ -1028,10 +1067,22 @@ void launch_reduced_factor(const T* WV, int k, int nb, T* S, T* Sinv, int safe, 
     SAP_LAUNCH_CHECK();
     const size_t lds_lu = sizeof(T) * (size_t)k * k;
     if (lds_lu <= LDS_LIMIT && k <= 256) {
+        // LU in LDS; the inverse by the blocked matrix-core sweeps when they fit
+        const bool mfma_inv = k <= 32 * STK_IT && StkLayout<T>(k).bytes() <= LDS_LIMIT;
         SAP_HIP_CHECK(hipFuncSetAttribute((const void*)schur_lu_inv_kernel<T>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_lu));
-        hipLaunchKernelGGL(schur_lu_inv_kernel<T>, dim3(nb), dim3(1024), lds_lu, st, S, Sinv, k, safe);
+        hipLaunchKernelGGL(schur_lu_inv_kernel<T>, dim3(nb), dim3(1024), lds_lu, st, S, Sinv, k, safe,
+                           mfma_inv ? 0 : 1);
         SAP_LAUNCH_CHECK();
+        if (mfma_inv) {
+            const size_t lds = StkLayout<T>(k).bytes();
+            constexpr int NR = StkCfg<T>::NR;
+            SAP_HIP_CHECK(hipFuncSetAttribute((const void*)schur_inverse_mfma_kernel<T>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(schur_inverse_mfma_kernel<T>, dim3(nb, (k + NR - 1) / NR), dim3(STK_THREADS), lds, st,
+                               S, k, Sinv);
+            SAP_LAUNCH_CHECK();
+        }
         return;
     }
     hipLaunchKernelGGL(schur_lu_kernel<T>, dim3(nb), dim3(256), 0, st, S, k, safe);
