@@ -25,11 +25,13 @@ namespace {
 // columns and U rows, and the slots they free take the tiles that enter, so
 // every band entry is read from HBM once and every factor entry written once.
 // Per panel:
-//   A.  slots holding the panel columns and the U rows are copied to LDS and
-//       start loading the entering tiles (first used one panel later); the
-//       previous panel's factors are stored from the other LDS buffer;
-//   B1. wavefront 0 factors the 32 x 32 diagonal block in registers, one row
-//       per lane, pivot rows broadcast with readlane, boosting at pivot time;
+//   A0. the slots of the 32 x 32 diagonal block are copied to LDS;
+//   B1. wavefront 0 factors that block in registers, one row per lane, pivot
+//       rows broadcast with readlane, boosting at pivot time, while
+//   A.  the other wavefronts (and wavefront 0 after B1) copy the other panel
+//       column and U row slots to LDS, start loading the entering tiles
+//       (first used one panel later) and store the previous panel's factors
+//       from the other LDS buffer;
 //   B2. 16(KT-2) threads solve L21 = A21 U11^-1 row by row, another 16(KT-2)
 //       threads U12 = L11^-1 A12 column by column (unit L11);
 //   C.  trailing tiles += (-L21) U12 on the matrix cores
@@ -226,8 +228,11 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
     // stores are unconditional buffer stores: entries outside the band get
     // an offset past the end of the buffer and are dropped, so every panel
     // issues the same number of stores (pp < 0: none land).
-    auto store_panel = [&](int64_t pp, const T* bq, int tid) {
-        constexpr int N1 = (32 * HR + NT - 1) / NT, N2 = (32 * NTR + NT - 1) / NT;
+    // Stored by wavefronts 1 .. NW-1 (st = thread index - 64): wavefront 0
+    // factors the next diagonal block meanwhile.
+    constexpr int NTS = NT - 64;
+    auto store_panel = [&](int64_t pp, const T* bq, int st) {
+        constexpr int N1 = (32 * HR + NTS - 1) / NTS, N2 = (32 * NTR + NTS - 1) / NTS;
         const int64_t pj0 = 32 * max(pp, (int64_t)0);
         const int obq = rel0 + dir * (int)(pj0 * w + k);
         const int prem = pp < 0 ? 0 : (int)min(len - pj0, (int64_t)(1 << 30));
@@ -239,7 +244,7 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
         int o1[N1], o2[N2];
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
-            const int e = tid + NT * i;
+            const int e = st + NTS * i;
             const int c = e / HR, r = e - c * HR;
             const int ok = neg_mask((e - 32 * HR) & (c - pnb) & (r - prem) & (r - c - k - 1) & (c - r - k - 1));
             const int lrow = ~neg_mask(r - 32);   // L21 rows, kept negated in LDS
@@ -251,7 +256,7 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
         }
 #pragma unroll
         for (int i = 0; i < N2; ++i) {
-            const int e = tid + NT * i;
+            const int e = st + NTS * i;
             const int cc = e >> 5, m = e & 31;
             const int ok = neg_mask((e - 32 * NTR) & (m - pnb) & (32 + cc - prem) & (32 + cc - m - k - 1));
             v2[i] = qUb[(cc * PS + m) & ok] * qrp[m];
@@ -302,7 +307,7 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
         // slot classes: panel column (gam < 2), U row (rho < 2 <= gam) and
         // trailing (both >= 2).  Each unrolled slot step then costs a mask bit
         // test and one readlane instead of its own scalar index arithmetic.
-        unsigned pm, um, tm;
+        unsigned pm, um, tm, dm;
         int rg;
         {
             const int sl = min(wave + NW * lane, NS - 1);
@@ -313,50 +318,35 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
             pm = (unsigned)__builtin_amdgcn_ballot_w64(okl && gl < 2);
             um = (unsigned)__builtin_amdgcn_ballot_w64(okl && gl >= 2 && rl < 2);
             tm = (unsigned)__builtin_amdgcn_ballot_w64(okl && gl >= 2 && rl >= 2);
+            dm = (unsigned)__builtin_amdgcn_ballot_w64(okl && gl < 2 && rl < 2);   // diagonal block
             rg = rl | (gl << 8);
         }
         // rho and gam of slot j (j a compile-time constant)
         auto slot_rho = [&](int j) { return __builtin_amdgcn_readlane(rg, j) & 0xff; };
         auto slot_gam = [&](int j) { return __builtin_amdgcn_readlane(rg, j) >> 8; };
 
-        // Every load and store of the previous panel has long completed (B1,
-        // B2 and C ran since); saying so with an explicit wait clears the
+        // Every load and store of the previous panel has long completed (B2
+        // and C ran since); saying so with an explicit wait clears the
         // compiler's record of loads pending across the loop back-edge, which
         // would otherwise make each slot below wait for the loads just issued
         // by the slots before it (vmcnt bits [3:0] and [15:14] = 0, expcnt and
         // lgkmcnt at their maxima).
         __builtin_amdgcn_s_waitcnt((0x7 << 4) | (0xF << 8));
-        // A. leaving tiles to LDS, then their slots load the entering tiles,
-        //    then the previous panel's factors go out: every global access of
-        //    the panel is issued here, so loads and stores drain during B1 and
-        //    B2 before the matrix-core phase needs the registers.  (All copies
-        //    to LDS come first: reading a slot that may still be loading
-        //    waits for every load issued before it.)
+        // A0. the diagonal block's four tiles to LDS: B1 needs nothing else,
+        //     so wavefront 0 factors it while the others finish phase A.
 #pragma unroll
         for (int j = 0; j < SPW; ++j) {
-            if ((pm >> j) & 1) {
+            if ((dm >> j) & 1) {
                 const int rho = slot_rho(j), gam = slot_gam(j);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) Pn[(16 * rho + lr) * PS + 16 * gam + MF::row(lane, q)] = acc[j][q];
-            } else if ((um >> j) & 1) {
-                const int rho = slot_rho(j), gam = slot_gam(j);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) UbT[(16 * (gam - 2) + MF::row(lane, q)) * PS + 16 * rho + lr] = acc[j][q];
             }
         }
-#pragma unroll
-        for (int j = 0; j < SPW; ++j) {
-            if (((pm | um) >> j) & 1) {
-                const int rho = slot_rho(j), gam = slot_gam(j);
-                const int rn = rho < 2 ? rho + KT - 2 : rho - 2;
-                const int cn = gam < 2 ? gam + KT - 2 : gam - 2;
-                load_tile(acc[j], obp, rem, 32 + 16 * rn, 32 + 16 * cn, lr, lane);
-            }
-        }
-        store_panel(p - 1, sm + zoff + (1 - cur) * LY::buf_elems, tid);
         lds_barrier();
 
-        // B1. LU of the diagonal block, row r in lane r (lanes 32..63 mirror).
+        // B1. LU of the diagonal block (wavefront 0), row r in lane r (lanes
+        //     32..63 mirror).  It writes rows 0..31 of Pn, the pivots and L11^T
+        //     only, which the rest of phase A does not touch.
         //     Straight-line code (no branch per column), so the scheduler can
         //     run ahead: a lane that must not change gets multiplier 0, and
         //     columns past the partition's end (last panel) get reciprocal 0,
@@ -388,6 +378,34 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
                 }
             }
         }
+        // A. the other leaving tiles to LDS, then their slots load the entering
+        //    tiles, then the previous panel's factors go out: every global
+        //    access of the panel is issued here, so loads and stores drain
+        //    during B2 and C before the next panel needs them.  (All copies to
+        //    LDS come first: reading a slot that may still be loading waits for
+        //    every load issued before it.)
+#pragma unroll
+        for (int j = 0; j < SPW; ++j) {
+            if (((pm & ~dm) >> j) & 1) {
+                const int rho = slot_rho(j), gam = slot_gam(j);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) Pn[(16 * rho + lr) * PS + 16 * gam + MF::row(lane, q)] = acc[j][q];
+            } else if ((um >> j) & 1) {
+                const int rho = slot_rho(j), gam = slot_gam(j);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) UbT[(16 * (gam - 2) + MF::row(lane, q)) * PS + 16 * rho + lr] = acc[j][q];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < SPW; ++j) {
+            if (((pm | um) >> j) & 1) {
+                const int rho = slot_rho(j), gam = slot_gam(j);
+                const int rn = rho < 2 ? rho + KT - 2 : rho - 2;
+                const int cn = gam < 2 ? gam + KT - 2 : gam - 2;
+                load_tile(acc[j], obp, rem, 32 + 16 * rn, 32 + 16 * cn, lr, lane);
+            }
+        }
+        if (wave > 0) store_panel(p - 1, sm + zoff + (1 - cur) * LY::buf_elems, tid - 64);
         lds_barrier();
 
         // B2. L21 rows (stored negated: the trailing update adds) and U12 columns
@@ -504,7 +522,7 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
             }
         }
     }
-    if (npan > 0) store_panel(npan - 1, sm + ((npan - 1) & 1) * LY::buf_elems, threadIdx.x);
+    if (npan > 0 && wave > 0) store_panel(npan - 1, sm + ((npan - 1) & 1) * LY::buf_elems, (int)threadIdx.x - 64);
 }
 
 // Wavefronts per workgroup: the window's tiles are spread over them, so more
