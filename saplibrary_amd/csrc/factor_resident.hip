@@ -106,15 +106,15 @@ __device__ inline void buffer_store<double>(double v, __amdgpu_buffer_rsrc_t r, 
     __builtin_amdgcn_raw_buffer_store_b64(x, r, off, 0, 0);
 }
 
-// 32 values of a 16-byte aligned LDS row (float: 16-byte loads), from
+// N values of a 16-byte aligned LDS row (float: 16-byte loads), from
 // element 4 * (FROM / 4) on; the rest of v is left as it is
-template <class T>
-__device__ inline void lds_row_from(const T* p, T (&v)[32], int from)
+template <class T, int N>
+__device__ inline void lds_row_from(const T* p, T (&v)[N], int from)
 {
     if constexpr (sizeof(T) == 4) {
         const float4* q = reinterpret_cast<const float4*>(p);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < N / 4; ++i) {
             if (4 * i + 3 >= from) {
                 const float4 x = q[i];
                 v[4 * i] = x.x; v[4 * i + 1] = x.y; v[4 * i + 2] = x.z; v[4 * i + 3] = x.w;
@@ -122,26 +122,28 @@ __device__ inline void lds_row_from(const T* p, T (&v)[32], int from)
         }
     } else {
 #pragma unroll
-        for (int c = 0; c < 32; ++c)
+        for (int c = 0; c < N; ++c)
             if (c >= from) v[c] = p[c];
     }
 }
-template <int FROM, class T>
-__device__ inline void lds_row(const T* p, T (&v)[32])
+template <int FROM, class T, int N>
+__device__ inline void lds_row(const T* p, T (&v)[N])
 {
-    lds_row_from<T>(p, v, FROM);
+    lds_row_from<T, N>(p, v, FROM);
 }
-__device__ inline void lds_store_row(float* p, const float (&v)[32])
+template <int N>
+__device__ inline void lds_store_row(float* p, const float (&v)[N])
 {
     float4* q = reinterpret_cast<float4*>(p);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) q[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+    for (int i = 0; i < N / 4; ++i) q[i] = make_float4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
 }
-__device__ inline void lds_store_neg_row(float* p, const float (&v)[32])
+template <int N>
+__device__ inline void lds_store_neg_row(float* p, const float (&v)[N])
 {
     float4* q = reinterpret_cast<float4*>(p);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) q[i] = make_float4(-v[4 * i], -v[4 * i + 1], -v[4 * i + 2], -v[4 * i + 3]);
+    for (int i = 0; i < N / 4; ++i) q[i] = make_float4(-v[4 * i], -v[4 * i + 1], -v[4 * i + 2], -v[4 * i + 3]);
 }
 
 __device__ inline float pivot_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
@@ -410,40 +412,100 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
 
         // B2. L21 rows (stored negated: the trailing update adds) and U12 columns
         if constexpr (sizeof(T) == 4) {
-            // FP32: the rows of U11 (L11^T) come in 16-byte loads one step
-            // ahead of their use, so the LDS latency hides behind the
-            // previous step's multiply-adds
+            // FP32, by halves: columns 0..15 of L21 = A21 U11^-1 (rows 0..15 of
+            // U12 = L11^-1 A12) by substitution; their contribution to the
+            // other half (A21_b -= L21_a U11_ab, A12_b -= L11_ba U12_a) on the
+            // matrix cores; then the other half by substitution.  The U11
+            // (L11^T) rows come in 16-byte loads one step ahead of their use.
             if (tid < NTR) {
                 const int t = tid;
-                T a[32], cur[32], nxt[32];
+                T a[16], cur[16], nxt[16];
                 lds_row<0>(Pn + (32 + t) * PS, a);
                 lds_row<1>(Pn, cur);
 #pragma unroll
-                for (int jj = 0; jj < 32; ++jj) {   // rpiv = 0 past the partition's end
-                    if (jj < 31) lds_row_from<T>(Pn + (jj + 1) * PS, nxt, jj + 2);
+                for (int jj = 0; jj < 16; ++jj) {   // rpiv = 0 past the partition's end
+                    if (jj < 15) lds_row_from<T>(Pn + (jj + 1) * PS, nxt, jj + 2);
                     const T l = a[jj] * rpiv[jj];
                     a[jj] = l;
 #pragma unroll
-                    for (int c = jj + 1; c < 32; ++c) a[c] -= l * cur[c];
+                    for (int c = jj + 1; c < 16; ++c) a[c] -= l * cur[c];
 #pragma unroll
-                    for (int c = 0; c < 32; ++c) cur[c] = nxt[c];
+                    for (int c = 0; c < 16; ++c) cur[c] = nxt[c];
                 }
                 lds_store_neg_row(Pn + (32 + t) * PS, a);
             } else if (tid >= UOFF && tid < UOFF + NTR) {
                 const int cc = tid - UOFF;
-                T u[32], cur[32], nxt[32];
+                T u[16], cur[16], nxt[16];
                 lds_row<0>(UbT + cc * PS, u);
                 lds_row<1>(L11T, cur);
 #pragma unroll
-                for (int jj = 0; jj < 31; ++jj) {   // u[jj] = 0 past the partition's end
-                    if (jj < 30) lds_row_from<T>(L11T + (jj + 1) * LY::LTS, nxt, jj + 2);
+                for (int jj = 0; jj < 15; ++jj) {   // u[jj] = 0 past the partition's end
+                    if (jj < 14) lds_row_from<T>(L11T + (jj + 1) * LY::LTS, nxt, jj + 2);
                     const T uj = u[jj];
 #pragma unroll
-                    for (int m = jj + 1; m < 32; ++m) u[m] -= cur[m] * uj;
+                    for (int m = jj + 1; m < 16; ++m) u[m] -= cur[m] * uj;
 #pragma unroll
-                    for (int m = 0; m < 32; ++m) cur[m] = nxt[m];
+                    for (int m = 0; m < 16; ++m) cur[m] = nxt[m];
                 }
                 lds_store_row(UbT + cc * PS, u);
+            }
+            lds_barrier();
+            {
+                // 16 x 16 tiles: KT-2 of L21's rows (A = the stored -L21_a,
+                // B = U11 rows 0..15, columns 16..31) and KT-2 of U12's
+                // columns (A = -U12_a^T, B = L11^T rows 0..15, columns 16..31)
+                const int g = lane >> 4, c16 = lane & 15;
+                for (int f = wave; f < 2 * (KT - 2); f += NW) {
+                    const bool lside = f < KT - 2;
+                    const int r0 = lside ? 32 + 16 * f : 16 * (f - (KT - 2));
+                    T* const rows = lside ? Pn : UbT;
+                    const T* const bm = lside ? Pn : L11T;
+                    const int bs = lside ? PS : LY::LTS;
+                    acc_t acc;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] = rows[(r0 + MF::row(lane, q)) * PS + 16 + c16];
+#pragma unroll
+                    for (int st = 0; st < 4; ++st) {
+                        const T av = rows[(r0 + c16) * PS + 4 * st + g];
+                        const T bv = bm[(4 * st + g) * bs + 16 + c16];
+                        acc = MF::mma(lside ? av : -av, bv, acc);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) rows[(r0 + MF::row(lane, q)) * PS + 16 + c16] = acc[q];
+                }
+            }
+            lds_barrier();
+            if (tid < NTR) {
+                const int t = tid;
+                T a[16], cur[16], nxt[16];   // columns 16..31
+                lds_row<0>(Pn + (32 + t) * PS + 16, a);
+                lds_row<1>(Pn + 16 * PS + 16, cur);
+#pragma unroll
+                for (int jj = 0; jj < 16; ++jj) {
+                    if (jj < 15) lds_row_from<T>(Pn + (17 + jj) * PS + 16, nxt, jj + 2);
+                    const T l = a[jj] * rpiv[16 + jj];
+                    a[jj] = l;
+#pragma unroll
+                    for (int c = jj + 1; c < 16; ++c) a[c] -= l * cur[c];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) cur[c] = nxt[c];
+                }
+                lds_store_neg_row(Pn + (32 + t) * PS + 16, a);
+            } else if (tid >= UOFF && tid < UOFF + NTR) {
+                const int cc = tid - UOFF;
+                T u[16], cur[16], nxt[16];   // rows 16..31
+                lds_row<0>(UbT + cc * PS + 16, u);
+                lds_row<1>(L11T + 16 * LY::LTS + 16, cur);
+#pragma unroll
+                for (int jj = 0; jj < 15; ++jj) {
+                    if (jj < 14) lds_row_from<T>(L11T + (17 + jj) * LY::LTS + 16, nxt, jj + 2);
+                    const T uj = u[jj];
+#pragma unroll
+                    for (int m = jj + 1; m < 16; ++m) u[m] -= cur[m] * uj;
+#pragma unroll
+                    for (int m = 0; m < 16; ++m) cur[m] = nxt[m];
+                }
+                lds_store_row(UbT + cc * PS + 16, u);
             }
         } else if (tid < NTR) {
             const int t = tid;
