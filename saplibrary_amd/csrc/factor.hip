@@ -1095,22 +1095,31 @@ void launch_reduced_factor(const T* WV, int k, int nb, T* S, T* Sinv, int safe, 
 }
 
 template <class T>
-int64_t count_band_zeros(const T* A, int64_t na, const T* B, int64_t nb, unsigned long long* d_cnt, hipStream_t st)
+void count_band_zeros_async(const T* A, int64_t na, const T* B, int64_t nb, unsigned long long* d_cnt,
+                            unsigned long long* h_out, hipStream_t st, int max_groups)
 {
     SAP_HIP_CHECK(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), st));
     int64_t g = (na + nb + 255) / 256;
-    if (g > 8192) g = 8192;
+    if (g > max_groups) g = max_groups;
     if (g < 1) g = 1;
     hipLaunchKernelGGL(count_zero_kernel<T>, dim3((unsigned)g), dim3(256), 0, st, A, na, B, nb, d_cnt);
     SAP_LAUNCH_CHECK();
+    SAP_HIP_CHECK(hipMemcpyAsync(h_out, d_cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+}
+
+template <class T>
+int64_t count_band_zeros(const T* A, int64_t na, const T* B, int64_t nb, unsigned long long* d_cnt, hipStream_t st)
+{
     unsigned long long h = 0;
-    SAP_HIP_CHECK(hipMemcpyAsync(&h, d_cnt, sizeof(h), hipMemcpyDeviceToHost, st));
+    count_band_zeros_async<T>(A, na, B, nb, d_cnt, &h, st, 8192);
     SAP_HIP_CHECK(hipStreamSynchronize(st));
     return (int64_t)h;
 }
 
 #define SAP_INST(T)                                                                                          \
     template int64_t count_band_zeros<T>(const T*, int64_t, const T*, int64_t, unsigned long long*, hipStream_t); \
+    template void count_band_zeros_async<T>(const T*, int64_t, const T*, int64_t, unsigned long long*,        \
+                                            unsigned long long*, hipStream_t, int);                           \
     template void launch_band_factor<T>(T*, T*, int, const FactorJob*, int, int64_t, hipStream_t);                     \
     template void launch_zero_pivot_check<T>(const T*, int, int64_t, int64_t, int*, hipStream_t);             \
     template void launch_extract_coupling<T>(const T*, int, PartInfo, T*, T*, hipStream_t);                   \

@@ -304,6 +304,7 @@ int Precond<T>::factor_band()
 {
     const int rc = factor_band_steps();
     bul_filled_ = false;
+    if (side_.s) SAP_HIP_CHECK(hipStreamSynchronize(side_.s));   // also on the early returns
     tm_.resolve();
     return rc;
 }
@@ -406,6 +407,20 @@ int Precond<T>::factor_band_steps()
         }
     }
 
+    // non-zeros of the factored band (sap/precond.h:1352): the LU part and,
+    // for LU_UL, the UL-factored last partition.  The count reads the factors
+    // only, so it runs on a second stream beside the spike tips and the
+    // reduced blocks.
+    {
+        hipStream_t side = side_.after(stream);
+        if (lu_ul())
+            count_band_zeros_async<T>(B_.get(), (int64_t)w * s_last, Bul_.get() + (size_t)w * s_last,
+                                      (int64_t)w * l_last, cnt_.get(), side_.host, side, 1024);
+        else
+            count_band_zeros_async<T>(B_.get(), (int64_t)w * n, (const T*)nullptr, 0, cnt_.get(), side_.host, side,
+                                      1024);
+    }
+
     // (4) spike tips
     tm_.mark(5, stream);
     {
@@ -426,13 +441,8 @@ int Precond<T>::factor_band_steps()
     tm_.mark(7, stream);
     tm_.span(&time_assembly, 5, 6);
     tm_.span(&time_fullLU, 6, 7);
-    // non-zeros of the factored band (sap/precond.h:1352): the LU part and,
-    // for LU_UL, the UL-factored last partition
-    const int64_t zeros =
-        lu_ul() ? count_band_zeros<T>(B_.get(), (int64_t)w * s_last, Bul_.get() + (size_t)w * s_last,
-                                      (int64_t)w * l_last, cnt_.get(), stream)
-                : count_band_zeros<T>(B_.get(), (int64_t)w * n, (const T*)nullptr, 0, cnt_.get(), stream);
-    actual_nnz = (int64_t)w * n - zeros;
+    SAP_HIP_CHECK(hipStreamSynchronize(side_.s));
+    actual_nnz = (int64_t)w * n - (int64_t)*side_.host;
     return 0;
 }
 

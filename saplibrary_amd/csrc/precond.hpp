@@ -70,6 +70,36 @@ private:
     std::vector<Span> pend_;
 };
 
+// A second stream (default priority) for work that runs beside the main
+// stream's, with the event that orders it after the main stream, and a
+// pinned host word for a result read back from it.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr;
+    unsigned long long* host = nullptr;
+    SideStream() = default;
+    SideStream(const SideStream&) = delete;
+    SideStream& operator=(const SideStream&) = delete;
+    ~SideStream()
+    {
+        if (host) (void)hipHostFree(host);
+        if (fork) (void)hipEventDestroy(fork);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    // the side stream, ordered after everything queued on main so far
+    hipStream_t after(hipStream_t main)
+    {
+        if (!s) {
+            SAP_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            SAP_HIP_CHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+            SAP_HIP_CHECK(hipHostMalloc((void**)&host, sizeof(unsigned long long), hipHostMallocDefault));
+        }
+        SAP_HIP_CHECK(hipEventRecord(fork, main));
+        SAP_HIP_CHECK(hipStreamWaitEvent(s, fork, 0));
+        return s;
+    }
+};
+
 template <class T>
 class Precond {
 public:
@@ -129,6 +159,7 @@ private:
     DevBuf<FactorJob> jobs_;
     DevBuf<int> flag_;
     DevBuf<unsigned long long> cnt_;
+    SideStream side_;
     SetupTimer tm_;
     // the conversion that preceded factor_band() also wrote Bul_ (the copy
     // the UL factorizations overwrite), so no device copy is needed

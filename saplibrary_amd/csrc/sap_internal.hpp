@@ -200,5 +200,11 @@ void launch_diag_divide(const T* B, int64_t n, const double* v, double* out, hip
 // zeros in A[0, na) and B[0, nb) (B may be null when nb = 0)
 template <class T>
 int64_t count_band_zeros(const T* A, int64_t na, const T* B, int64_t nb, unsigned long long* d_cnt, hipStream_t st);
+// the same queued on st, the count copied to *h_out (pinned) when st reaches
+// it; at most max_groups workgroups (fewer leave room for kernels running
+// beside it)
+template <class T>
+void count_band_zeros_async(const T* A, int64_t na, const T* B, int64_t nb, unsigned long long* d_cnt,
+                            unsigned long long* h_out, hipStream_t st, int max_groups);
 
 }  // namespace sap_amd
