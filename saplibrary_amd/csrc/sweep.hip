@@ -285,15 +285,17 @@ __device__ __forceinline__ void lower_sweep_fast(const BandView<T>& V, int64_t l
 // dependency chain.  In step b:
 //   wave 0     diagonal-block chain and sub-diagonal block of block b (as in
 //              lower_sweep_fast), multipliers from buffer b;
-//   waves 1-3  far rows of block b-1 from buffer b-1; wave 1 also moves the
-//              staged right-hand side rows into the pending ring;
-//   waves 4-7  producers, loading block b + NBUF - 2 into the buffer of
-//              block b-2.
-// Each producer (and wave 1 for the staging ring) waits with an explicit
+//   waves 1-3  far rows of block b-1 from buffer b-1;
+//   waves 4-6  producers, loading block b + NBUF - 2 into the buffer of
+//              block b-2;
+//   wave 7     moves the staged right-hand side rows into the pending ring
+//              and streams the next ones.
+// Each producer (and wave 7 for the staging ring) waits with an explicit
 // vmcnt count for its own copies before the barrier that publishes them.
 // ---------------------------------------------------------------------------
-constexpr int SW_PROD = 4;           // producer wavefronts
-constexpr int SW_NI_MAX = 8;         // load instructions per producer per block (k <= 192 + BS)
+constexpr int SW_PROD = 3;           // producer wavefronts (4 .. 3 + SW_PROD)
+constexpr int SW_STREAMER = 4 + SW_PROD;   // the right-hand side stream's wavefront
+constexpr int SW_NI_MAX = 10;        // load instructions per producer per block (k <= 192 + BS)
 constexpr int SW_THREADS = 512;
 constexpr int SW_DIN = 8;            // right-hand side steps in flight
 constexpr int SW_NIN = SW_DIN + 1;   // staging slots of 64 dwords
@@ -385,7 +387,7 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
     int64_t poff[SW_NI_MAX];
     int pcol[SW_NI_MAX];
     const int64_t first0 = seg_lo(0) & ~(int64_t)(E - 1);
-    if (wave >= 4) {
+    if (wave >= 4 && wave < SW_STREAMER) {
         const int lane = threadIdx.x & 63;
 #pragma unroll
         for (int i = 0; i < SW_NI_MAX; ++i) {
@@ -417,7 +419,7 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
         const int64_t last = min(nblk - 1, b + nbuf - 3);
         return (int)min(max((int64_t)0, last - b) * G.ni, (int64_t)63);
     };
-    // right-hand side stream (wave 1): step s carries rows [s*BS + H, s*BS + H + BS)
+    // right-hand side stream (wave 7): step s carries rows [s*BS + H, s*BS + H + BS)
     const int64_t nstream = len > H ? (len - H + BS - 1) / BS : 0;
     auto stream = [&](int64_t st, int lane) {
         const int tr = (lane / WPR) % BS, word = lane % WPR;
@@ -427,10 +429,10 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
         __builtin_amdgcn_global_load_lds(sp, instage + (st % SW_NIN) * 64, 4, 0, 0);
     };
 
-    if (wave >= 4) {
+    if (wave >= 4 && wave < SW_STREAMER) {
         for (int64_t b = 0; b < min(nblk, (int64_t)nbuf - 2); ++b) produce(b, (int)b, threadIdx.x & 63);
         wait_vmcnt(produced_after(0));
-    } else if (wave == 1) {
+    } else if (wave == SW_STREAMER) {
         for (int64_t st = 0; st < min(nstream, (int64_t)SW_DIN); ++st) stream(st, threadIdx.x & 63);
     }
     lds_barrier();
@@ -529,18 +531,19 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
             } else if (t < BS) {
                 acc2[BS + t] = (T)0;
             }
-            if (wave == 1 && b < nstream) {
-                if (b + SW_DIN < nstream) stream(b + SW_DIN, lane);
-                wait_vmcnt((int)(min(b + SW_DIN, nstream - 1) - b));
-                const int64_t rr = c0 + H + t;
-                if (t < BS && rr < len) {
-                    const RawT raw = reinterpret_cast<const RawT*>(instage + (b % SW_NIN) * 64)[t];
-                    pend[rr & ring_mask] = fin(rr, raw);
-                }
-            }
-        } else {
+        } else if (wave < SW_STREAMER) {
             if (b + nbuf - 2 < nblk) produce(b + nbuf - 2, slot >= 2 ? slot - 2 : slot + nbuf - 2, lane);
             wait_vmcnt(produced_after(b + 1));
+        } else if (b < nstream) {
+            // right-hand side rows c0 + H .. c0 + H + BS - 1 into the pending
+            // ring (rows no other wavefront touches in this step)
+            if (b + SW_DIN < nstream) stream(b + SW_DIN, lane);
+            wait_vmcnt((int)(min(b + SW_DIN, nstream - 1) - b));
+            const int64_t rr = c0 + H + lane;
+            if (lane < BS && rr < len) {
+                const RawT raw = reinterpret_cast<const RawT*>(instage + (b % SW_NIN) * 64)[lane];
+                pend[rr & ring_mask] = fin(rr, raw);
+            }
         }
         lds_barrier();
     }
