@@ -286,15 +286,19 @@ __device__ __forceinline__ void lower_sweep_fast(const BandView<T>& V, int64_t l
 //   wave 0     diagonal-block chain and sub-diagonal block of block b (as in
 //              lower_sweep_fast), multipliers from buffer b;
 //   waves 1-3  far rows of block b-1 from buffer b-1;
-//   waves 4-6  producers, loading block b + NBUF - 2 into the buffer of
-//              block b-2;
-//   wave 7     moves the staged right-hand side rows into the pending ring
-//              and streams the next ones.
-// Each producer (and wave 7 for the staging ring) waits with an explicit
+//   wave 4     moves the staged right-hand side rows into the pending ring
+//              and streams the next ones;
+//   waves 5-7  producers, loading block b + NBUF - 2 into the buffer of
+//              block b-2.
+// Each producer (and wave 4 for the staging ring) waits with an explicit
 // vmcnt count for its own copies before the barrier that publishes them.
 // ---------------------------------------------------------------------------
-constexpr int SW_PROD = 3;           // producer wavefronts (4 .. 3 + SW_PROD)
-constexpr int SW_STREAMER = 4 + SW_PROD;   // the right-hand side stream's wavefront
+// Wavefront roles: 0 the chain, 1-3 far rows, SW_STREAMER the right-hand
+// side stream, SW_PROD0 .. SW_PROD0 + SW_PROD - 1 producers.  Wavefronts w
+// and w + 4 share a SIMD, so the chain's partner is the light streamer.
+constexpr int SW_PROD = 3;           // producer wavefronts
+constexpr int SW_STREAMER = 4;
+constexpr int SW_PROD0 = 5;
 constexpr int SW_NI_MAX = 10;        // load instructions per producer per block (k <= 192 + BS)
 constexpr int SW_THREADS = 512;
 constexpr int SW_DIN = 8;            // right-hand side steps in flight
@@ -387,11 +391,11 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
     int64_t poff[SW_NI_MAX];
     int pcol[SW_NI_MAX];
     const int64_t first0 = seg_lo(0) & ~(int64_t)(E - 1);
-    if (wave >= 4 && wave < SW_STREAMER) {
+    if (wave >= SW_PROD0) {
         const int lane = threadIdx.x & 63;
 #pragma unroll
         for (int i = 0; i < SW_NI_MAX; ++i) {
-            const int rnd = i * SW_PROD + (wave - 4);       // 64-chunk round of this block
+            const int rnd = i * SW_PROD + (wave - SW_PROD0);   // 64-chunk round of this block
             const int g = rnd * 64 + lane;
             const int j = g / G.nch, ch = g - j * G.nch;
             const bool real = i < G.ni && g < BS * G.nch;
@@ -407,7 +411,7 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
 #pragma unroll
         for (int i = 0; i < SW_NI_MAX; ++i) {
             if (i < G.ni) {
-                const int rnd = i * SW_PROD + (wave - 4);
+                const int rnd = i * SW_PROD + (wave - SW_PROD0);
                 // columns past the end re-read the block's first chunk
                 const int64_t off = (pcol[i] < rem ? poff[i] : first0) + shift;
                 __builtin_amdgcn_global_load_lds(V.base + off, dst + (size_t)rnd * 64 * E, 16, 0, 0);
@@ -419,7 +423,7 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
         const int64_t last = min(nblk - 1, b + nbuf - 3);
         return (int)min(max((int64_t)0, last - b) * G.ni, (int64_t)63);
     };
-    // right-hand side stream (wave 7): step s carries rows [s*BS + H, s*BS + H + BS)
+    // right-hand side stream (wave 4): step s carries rows [s*BS + H, s*BS + H + BS)
     const int64_t nstream = len > H ? (len - H + BS - 1) / BS : 0;
     auto stream = [&](int64_t st, int lane) {
         const int tr = (lane / WPR) % BS, word = lane % WPR;
@@ -429,7 +433,7 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
         __builtin_amdgcn_global_load_lds(sp, instage + (st % SW_NIN) * 64, 4, 0, 0);
     };
 
-    if (wave >= 4 && wave < SW_STREAMER) {
+    if (wave >= SW_PROD0) {
         for (int64_t b = 0; b < min(nblk, (int64_t)nbuf - 2); ++b) produce(b, (int)b, threadIdx.x & 63);
         wait_vmcnt(produced_after(0));
     } else if (wave == SW_STREAMER) {
@@ -531,7 +535,7 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
             } else if (t < BS) {
                 acc2[BS + t] = (T)0;
             }
-        } else if (wave < SW_STREAMER) {
+        } else if (wave >= SW_PROD0) {
             if (b + nbuf - 2 < nblk) produce(b + nbuf - 2, slot >= 2 ? slot - 2 : slot + nbuf - 2, lane);
             wait_vmcnt(produced_after(b + 1));
         } else if (b < nstream) {
