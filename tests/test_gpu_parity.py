@@ -104,6 +104,21 @@ def test_factors_match_oracle(sap, oracle, n, k, P, fact, precond, prec_double):
             np.testing.assert_allclose(s[i].T, Sc, rtol=0, atol=tol * max(1.0, np.abs(Sc).max()))
 
 
+@pytest.mark.parametrize("n,k,P,fact,precond", FACTOR_CASES)
+@pytest.mark.parametrize("prec_double", [True, False])
+def test_actual_nnz_counts_the_factored_band(sap, oracle, n, k, P, fact, precond, prec_double):
+    """Stats.actual_nnz is (2k+1) n minus the zeros of the factored band
+    (sap/precond.h:1352, thrust::count over m_B), the band factors() returns
+    (the UL-factored last partition under LU_UL, as in m_B)."""
+    band = oracle.gen_banded(n, k, 0.8, 23)
+    gpu, _ = make_pair(sap, oracle, P, fact, precond, prec_double)
+    gpu.setup(sap.BandedMatrix(dev(band)))
+    F = gpu.factors()
+    assert gpu.getStats().actual_nnz == F.size - int(np.count_nonzero(F == 0.0))
+    # out-of-matrix corners are exact zeros, so the count is below the full band
+    assert gpu.getStats().actual_nnz <= F.size - k * (k + 1)
+
+
 # ------------------------------------------------------------------- apply
 APPLY_CASES = FACTOR_CASES + [(20000, 100, 16, "LU_UL", "Spike"), (5000, 0, 4, "LU_UL", "Spike"),
                               (3000, 10, 3, "LU_UL", "None"),
