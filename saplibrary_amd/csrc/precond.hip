@@ -20,9 +20,19 @@ template <class T>
 void Precond<T>::factor(const std::vector<FactorJob>& jobs)
 {
     if (jobs.empty()) return;
-    jobs_.resize(jobs.size());
-    SAP_HIP_CHECK(hipMemcpyAsync(jobs_.get(), jobs.data(), jobs.size() * sizeof(FactorJob), hipMemcpyHostToDevice,
-                                 stream));
+    // the job list is uploaded only when it changed (repeated setups of one
+    // size reuse it)
+    const bool same = jobs.size() == jobs_host_.size() &&
+                      std::equal(jobs.begin(), jobs.end(), jobs_host_.begin(), [](const FactorJob& a, const FactorJob& b) {
+                          return a.start == b.start && a.len == b.len && a.dir == b.dir && a.boost == b.boost &&
+                                 a.boost_last == b.boost_last && a.buf == b.buf;
+                      });
+    if (!same) {
+        jobs_.resize(jobs.size());
+        SAP_HIP_CHECK(hipMemcpyAsync(jobs_.get(), jobs.data(), jobs.size() * sizeof(FactorJob), hipMemcpyHostToDevice,
+                                     stream));
+        jobs_host_ = jobs;
+    }
     ProfScope ps(prof, "band_factor");
     int64_t max_len = 0;
     for (const FactorJob& j : jobs) max_len = std::max(max_len, (int64_t)j.len);
@@ -335,6 +345,9 @@ int Precond<T>::factor_band_steps()
     // pivot (sap/device/factor_band_var.cuh)
     const int boost = (k >= 70) || cfg.safeFactorization || var_mode_;
     const int boost_last = (k >= 70) || var_mode_;
+    // With every pivot boosted (to +-1e-4 when |d| <= 1e-7, NaN included)
+    // no stored pivot can be below 1e-7, so the check is skipped.
+    const bool check_pivots = !cfg.safeFactorization && !(boost && boost_last);
 
     if (cfg.precondType == SAP_Block || P == 1) {
         std::vector<FactorJob> jobs;
@@ -347,7 +360,7 @@ int Precond<T>::factor_band_steps()
         factor(jobs);
         tm_.mark(4, stream);
         tm_.span(&time_bandLU, 3, 4);
-        if (!cfg.safeFactorization && zero_pivots(B_.get(), 0, n)) return SAP_ERR_ZERO_PIVOT;
+        if (check_pivots && zero_pivots(B_.get(), 0, n)) return SAP_ERR_ZERO_PIVOT;
         actual_nnz = (int64_t)w * n - count_band_zeros<T>(B_.get(), (int64_t)w * n, (const T*)nullptr, 0,
                                                           cnt_.get(), stream);
         return 0;
@@ -397,7 +410,7 @@ int Precond<T>::factor_band_steps()
     tm_.mark(4, stream);
     tm_.span(&time_offDiags, 2, 3);
     tm_.span(&time_bandLU, 3, 4);
-    if (!cfg.safeFactorization) {
+    if (check_pivots) {
         if (lu_ul()) {
             int64_t s1, l1;
             pi.bounds(1, s1, l1);

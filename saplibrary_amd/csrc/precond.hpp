@@ -157,6 +157,7 @@ public:
 private:
     DevBuf<T> B_, Bul_, offB_, offC_, WV_, S_, Sinv_, g_, corr_, scr0_, scr1_;
     DevBuf<FactorJob> jobs_;
+    std::vector<FactorJob> jobs_host_;   // what jobs_ holds
     DevBuf<int> flag_;
     DevBuf<unsigned long long> cnt_;
     SideStream side_;
