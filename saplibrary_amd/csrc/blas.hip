@@ -87,6 +87,26 @@ __global__ void fill_kernel(int64_t n, double v, double* __restrict__ x)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = v;
 }
 
+struct SpreadArgs {
+    const double* src;
+    double* dst[4];
+    double* zero[4];
+    int nd, nz;
+};
+
+// dst[q] = src for q < nd and zero[q] = 0 for q < nz, in one pass
+__global__ void spread_kernel(int64_t n, SpreadArgs a)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (a.nd > 0) {
+            const double v = a.src[i];
+            for (int q = 0; q < a.nd; ++q) a.dst[q][i] = v;
+        }
+        for (int q = 0; q < a.nz; ++q) a.zero[q][i] = 0.0;
+    }
+}
+
 inline unsigned grid_for(int64_t n)
 {
     int64_t g = (n + 255) / 256;
@@ -127,6 +147,22 @@ void launch_axpby(int64_t n, double alpha, const double* x, double beta, const d
 void launch_copy(int64_t n, const double* x, double* y, hipStream_t st)
 {
     hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
+    SAP_LAUNCH_CHECK();
+}
+
+void launch_spread(int64_t n, const double* src, double* const* dst, int nd, double* const* zero, int nz,
+                   hipStream_t st)
+{
+    if (nd < 0 || nd > 4 || nz < 0 || nz > 4 || (nd > 0 && !src))
+        throw Error(SAP_ERR_BADARG, "launch_spread: at most four copies and four zero fills");
+    if (nd + nz == 0) return;
+    SpreadArgs a{};
+    a.src = src;
+    for (int q = 0; q < nd; ++q) a.dst[q] = dst[q];
+    for (int q = 0; q < nz; ++q) a.zero[q] = zero[q];
+    a.nd = nd;
+    a.nz = nz;
+    hipLaunchKernelGGL(spread_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, a);
     SAP_LAUNCH_CHECK();
 }
 

@@ -156,6 +156,9 @@ void launch_dots(int64_t n, int nd, const double* const* a, const double* const*
 void launch_axpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* z, hipStream_t st);
 void launch_copy(int64_t n, const double* x, double* y, hipStream_t st);
 void launch_fill(int64_t n, double v, double* x, hipStream_t st);
+// dst[q] = src (q < nd <= 4) and zero[q] = 0 (q < nz <= 4) in one launch
+void launch_spread(int64_t n, const double* src, double* const* dst, int nd, double* const* zero, int nz,
+                   hipStream_t st);
 int  dot_partials_size();
 
 // factor.hip

@@ -293,8 +293,6 @@ public:
         const double eps = 1e-20;
         double rho0 = 1, alpha = 0, omega = 1, rho1;
         double* r0 = vec();
-        double* r = vec();
-        double* u = vec();
         double* xx = vec();
         double* Pv = vec();
         double* rr[L + 1];
@@ -304,23 +302,28 @@ public:
         double* Pxx = vec();
         double* APxx = vec();
         double* rsave = vec();
-        for (int i = 0; i <= L; ++i) { launch_fill(n, 0.0, rr[i], stream); launch_fill(n, 0.0, uu[i], stream); }
-        launch_fill(n, 0.0, u, stream);
-        launch_fill(n, 0.0, x_min, stream);
         double tao[L + 1][L + 1] = {};
         double gamma[L + 2] = {}, gamma_prime[L + 2] = {}, gamma_pp[L + 2] = {}, sigma[L + 2] = {};
 
-        // r0 <- b - A x  (A*0 = 0 exactly, so the SpMV is skipped for a zero guess)
-        if (x_is_zero) copy(b, r0);
-        else {
+        // r0 = rr(0) <- b - A x, uu(0) <- 0, xx <- x.  A*0 = 0 exactly, so a
+        // zero guess needs no SpMV (and x is not read).  rr(1..L) and
+        // uu(1..L) are written before they are read, so they are not
+        // cleared; x_min (zero until a first minimum is recorded) is only
+        // read by the non-converged ending, which clears it then.
+        if (x_is_zero) {
+            double* cp[2] = {r0, rr[0]};
+            double* zs[2] = {uu[0], xx};
+            launch_spread(n, b, cp, 2, zs, 2, stream);
+        } else {
             spmv(x, r0);
             axpby(b, r0, r0, 1.0, -1.0);
+            double* cp[2] = {rr[0], nullptr};
+            double* zs[1] = {uu[0]};
+            launch_spread(n, r0, cp, 1, zs, 1, stream);
+            copy(x, xx);
         }
-        copy(r0, r);
-        copy(u, uu[0]);
-        copy(x, xx);
-        copy(r, rr[0]);
-        double r_norm_min = x_is_zero ? monitor.rhsNorm : nrm2(r);
+        bool have_xmin = false;
+        double r_norm_min = x_is_zero ? monitor.rhsNorm : nrm2(r0);
         double r_norm = r_norm_min;
         double r_norm_act = r_norm;
         bool pxx_valid = false;    // Pxx == M^{-1} xx for the current xx
@@ -399,7 +402,7 @@ public:
                     prec(rr[j], Pv);
                 }
                 spmv(Pv, rr[j + 1]);
-                if (r_norm_act < r_norm_min) { r_norm_min = r_norm_act; copy(xx, x_min); }
+                if (r_norm_act < r_norm_min) { r_norm_min = r_norm_act; copy(xx, x_min); have_xmin = true; }
                 if (monitor.finished()) break;
             }
             if (monitor.finished()) break;
@@ -441,7 +444,7 @@ public:
             if (monitor.needCheckConvergence(r_norm)) {
                 if (check_true()) break;
             }
-            if (r_norm_act < r_norm_min) { r_norm_min = r_norm_act; copy(xx, x_min); }
+            if (r_norm_act < r_norm_min) { r_norm_min = r_norm_act; copy(xx, x_min); have_xmin = true; }
             if (monitor.finished()) break;
             monitor.increment(0.25f);
 
@@ -462,13 +465,10 @@ public:
                 if (monitor.needCheckConvergence(r_norm)) {
                     if (check_true()) { brk = true; break; }
                 }
-                if (r_norm_act < r_norm_min) { r_norm_min = r_norm_act; copy(xx, x_min); }
+                if (r_norm_act < r_norm_min) { r_norm_min = r_norm_act; copy(xx, x_min); have_xmin = true; }
                 if (monitor.finished()) { brk = true; break; }
             }
             if (brk || monitor.finished()) break;
-            copy(uu[0], u);
-            copy(xx, x);
-            copy(rr[0], r);
             monitor.increment(0.25f);
         }
 
@@ -485,6 +485,7 @@ public:
             prec(xx, Pxx);
             spmv(Pxx, APxx);
             axpby(b, APxx, rc, 1.0, -1.0);
+            if (!have_xmin) launch_fill(n, 0.0, x_min, stream);
             prec(x_min, Pxmin);
             spmv(Pxmin, APxmin);
             axpby(b, APxmin, rcm, 1.0, -1.0);
@@ -579,12 +580,14 @@ public:
     bool use_minres() const { return opt.solverType == SAP_MINRES; }
 
     // One Krylov solve of A x = b (the reference's solve path).
-    // b_norm: |b| when the caller has it (as nrm2 gives it), else negative
+    // b_norm: |b| when the caller has it (as nrm2 gives it), else negative;
+    // x_is_zero: start from x = 0 without reading x
     int krylov(double* x, const double* b, double rtol, double atol, bool x_is_zero, bool correction, float* its,
                double* rnorm, double* rhs_norm, double b_norm = -1.0)
     {
         last_res_ = nullptr;
         if (use_minres()) {
+            if (x_is_zero) launch_fill(n, 0.0, x, stream);
             mmon.reset(opt.maxNumIterations, rtol, atol);
             mmon.init(nrm2(b));
             minres(x, b, mmon);
@@ -781,7 +784,6 @@ public:
                 if (rn <= tol) { code = 1; break; }
                 if (outer >= opt.refine) { code = -1; break; }
                 const double rt = std::max(opt.refineTol, 0.5 * tol / rn);
-                launch_fill(n, 0.0, dv, stream);
                 vec_next_ = mark;
                 double inner_r, inner_b;
                 const int c = krylov(dv, r, rt, 0.0, true, true, &its, &inner_r, &inner_b, rn);
