@@ -364,6 +364,37 @@ void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipS
 template void launch_band_to_colmajor<float>(int64_t, int, const double*, float*, hipStream_t, float*);
 template void launch_band_to_colmajor<double>(int64_t, int, const double*, double*, hipStream_t, double*);
 
+// The first k and last k columns of partitions p0 .. p1-1 (each column one
+// contiguous run of 2k+1 band entries), copied from in to out.
+template <class T>
+__global__ void copy_partition_edges_kernel(const T* __restrict__ in, T* __restrict__ out, int k, PartInfo pi, int p0,
+                                            int64_t total)
+{
+    const int64_t w = 2 * (int64_t)k + 1, per = 2 * (int64_t)k * w;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int p = p0 + (int)(i / per);
+        const int64_t e = i % per, c = e / w;
+        int64_t s, len;
+        pi.bounds(p, s, len);
+        const int64_t col = c < k ? s + c : s + len - 2 * k + c;
+        out[col * w + e % w] = in[col * w + e % w];
+    }
+}
+
+template <class T>
+void launch_copy_partition_edges(const T* in, T* out, int k, PartInfo pi, int p0, int p1, hipStream_t st)
+{
+    if (k <= 0 || p1 <= p0) return;
+    const int64_t total = (int64_t)(p1 - p0) * 2 * k * (2 * (int64_t)k + 1);
+    int64_t g = (total + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(copy_partition_edges_kernel<T>, dim3((unsigned)g), dim3(256), 0, st, in, out, k, pi, p0, total);
+    SAP_LAUNCH_CHECK();
+}
+template void launch_copy_partition_edges<float>(const float*, float*, int, PartInfo, int, int, hipStream_t);
+template void launch_copy_partition_edges<double>(const double*, double*, int, PartInfo, int, int, hipStream_t);
+
 template <class Tin, class T>
 void launch_csr_to_band(int64_t n, int k, const int* rp, const int* ci, const Tin* vals, T* B, hipStream_t st)
 {

@@ -990,10 +990,12 @@ void launch_factor_mfma(T* B0, T* B1, int k, const FactorJob* jobs, int njobs, h
 }
 
 template <class T>
-void launch_band_factor(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, int64_t max_len, hipStream_t st)
+void launch_band_factor(T* B0, T* B1, const T* Bin, int k, const FactorJob* d_jobs, int njobs, int64_t max_len,
+                        hipStream_t st)
 {
     if (njobs <= 0) return;
-    if (launch_band_factor_resident<T>(B0, B1, k, d_jobs, njobs, max_len, st)) return;
+    if (launch_band_factor_resident<T>(B0, B1, Bin, k, d_jobs, njobs, max_len, st)) return;
+    if (Bin) throw Error(SAP_ERR_HIP, "separate-input factorization needs the register-resident kernel");
     const bool fits = MfmaLds<T>(k).bytes <= LDS_LIMIT;
     if (fits && k + MNB <= 256) { launch_factor_mfma<T, 1>(B0, B1, k, d_jobs, njobs, st); return; }
     if (factor_lds_bytes<T>(k, 32) <= LDS_LIMIT)      launch_factor_nb<T, 32>(B0, B1, k, d_jobs, njobs, st);
@@ -1120,7 +1122,7 @@ int64_t count_band_zeros(const T* A, int64_t na, const T* B, int64_t nb, unsigne
     template int64_t count_band_zeros<T>(const T*, int64_t, const T*, int64_t, unsigned long long*, hipStream_t); \
     template void count_band_zeros_async<T>(const T*, int64_t, const T*, int64_t, unsigned long long*,        \
                                             unsigned long long*, hipStream_t, int);                           \
-    template void launch_band_factor<T>(T*, T*, int, const FactorJob*, int, int64_t, hipStream_t);                     \
+    template void launch_band_factor<T>(T*, T*, const T*, int, const FactorJob*, int, int64_t, hipStream_t);                     \
     template void launch_zero_pivot_check<T>(const T*, int, int64_t, int64_t, int*, hipStream_t);             \
     template void launch_extract_coupling<T>(const T*, int, PartInfo, T*, T*, hipStream_t);                   \
     template void launch_spike_tips<T>(const T*, const T*, int, PartInfo, int, const T*, const T*, T*,        \

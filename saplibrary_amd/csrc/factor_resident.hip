@@ -157,7 +157,8 @@ __device__ inline double pivot_rcp(double v) { return 1.0 / v; }
 __device__ inline int neg_mask(int x) { return __builtin_amdgcn_sbfe(x, 31, 1); }
 
 template <class T, int KT, int NW>
-__global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __restrict__ B0, T* __restrict__ B1, int k,
+__global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __restrict__ B0, T* __restrict__ B1,
+                                                                   const T* __restrict__ Bin, int k,
                                                                    const FactorJob* __restrict__ jobs)
 {
     using MF = Mfma16<T>;
@@ -190,8 +191,13 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
     // (A register only ever written by loads needs no wait for an earlier
     // load into it.)  Element (j0 + dr, j0 + dc) is at element offset
     // ob + dir * (dc * 2k + dr) with ob = rel0 + dir * (j0 * (2k + 1) + k).
+    // The band is read from Bin when given (the LU and UL factorizations of
+    // one partition then share one unfactored copy) and the factors are
+    // written to the job's band; without Bin the factorization is in place.
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(base + job.start * w), 0, (int)(len * w * (int64_t)sizeof(T)), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsrc_in = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)((Bin ? Bin : base) + job.start * w), 0, (int)(len * w * (int64_t)sizeof(T)), 0x00020000);
     const int rel0 = dir > 0 ? 0 : (int)(len * w - 1);
     // Tiles are held transposed: accumulator register q of lane l holds
     // A(dr0 + l % 16, dc0 + row(l, q)), so for each register the 16 lanes of
@@ -204,7 +210,7 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
             const int dc = dc0 + MF::row(lane, q);
             const int m = inb(dr, dc, rem);
             const int off = (((ob + dir * (dc * wm1 + dr)) * (int)sizeof(T)) & m) | ~m;
-            t[q] = buffer_load<T>(rsrc, off);
+            t[q] = buffer_load<T>(rsrc_in, off);
         }
     };
     // factors of panel pp from panel buffer bq: its 32 columns (rows 0 .. HR-1:
@@ -600,14 +606,14 @@ template <class T, int KT>
 constexpr int resident_waves() { return 8; }
 
 template <class T, int KT>
-void launch_factor_resident(T* B0, T* B1, int k, const FactorJob* jobs, int njobs, hipStream_t st)
+void launch_factor_resident(T* B0, T* B1, const T* Bin, int k, const FactorJob* jobs, int njobs, hipStream_t st)
 {
     constexpr int NW = resident_waves<T, KT>();
     const size_t lds = ResLds<T, KT>::bytes;
     SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_factor_resident_kernel<T, KT, NW>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((band_factor_resident_kernel<T, KT, NW>), dim3(njobs), dim3(64 * NW), lds, st, B0, B1, k,
-                       jobs);
+    hipLaunchKernelGGL((band_factor_resident_kernel<T, KT, NW>), dim3(njobs), dim3(64 * NW), lds, st, B0, B1, Bin,
+                       k, jobs);
     SAP_LAUNCH_CHECK();
 }
 
@@ -617,28 +623,40 @@ template <class T>
 constexpr int resident_max_kt() { return sizeof(T) == 4 ? 16 : 9; }
 
 template <class T, int KT>
-bool dispatch_resident(int kt, T* B0, T* B1, int k, const FactorJob* jobs, int njobs, hipStream_t st)
+bool dispatch_resident(int kt, T* B0, T* B1, const T* Bin, int k, const FactorJob* jobs, int njobs,
+                       hipStream_t st)
 {
     if constexpr (KT > resident_max_kt<T>()) {
         return false;
     } else {
-        if (kt == KT) { launch_factor_resident<T, KT>(B0, B1, k, jobs, njobs, st); return true; }
-        return dispatch_resident<T, KT + 1>(kt, B0, B1, k, jobs, njobs, st);
+        if (kt == KT) { launch_factor_resident<T, KT>(B0, B1, Bin, k, jobs, njobs, st); return true; }
+        return dispatch_resident<T, KT + 1>(kt, B0, B1, Bin, k, jobs, njobs, st);
     }
 }
 
 }  // namespace
 
 template <class T>
-bool launch_band_factor_resident(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, int64_t max_len,
-                                 hipStream_t st)
+bool band_factor_resident_fits(int k, int64_t max_len)
 {
     // buffer offsets are 32-bit: a partition's storage must stay below 2 GiB
     if (max_len * (2 * (int64_t)k + 1) * (int64_t)sizeof(T) >= ((int64_t)1 << 31)) return false;
-    return dispatch_resident<T, 3>((k + 32 + 15) / 16, B0, B1, k, d_jobs, njobs, st);
+    return (k + 32 + 15) / 16 <= resident_max_kt<T>();
 }
 
-template bool launch_band_factor_resident<float>(float*, float*, int, const FactorJob*, int, int64_t, hipStream_t);
-template bool launch_band_factor_resident<double>(double*, double*, int, const FactorJob*, int, int64_t, hipStream_t);
+template <class T>
+bool launch_band_factor_resident(T* B0, T* B1, const T* Bin, int k, const FactorJob* d_jobs, int njobs,
+                                 int64_t max_len, hipStream_t st)
+{
+    if (!band_factor_resident_fits<T>(k, max_len)) return false;
+    return dispatch_resident<T, 3>((k + 32 + 15) / 16, B0, B1, Bin, k, d_jobs, njobs, st);
+}
+
+template bool band_factor_resident_fits<float>(int, int64_t);
+template bool band_factor_resident_fits<double>(int, int64_t);
+template bool launch_band_factor_resident<float>(float*, float*, const float*, int, const FactorJob*, int, int64_t,
+                                                 hipStream_t);
+template bool launch_band_factor_resident<double>(double*, double*, const double*, int, const FactorJob*, int,
+                                                  int64_t, hipStream_t);
 
 }  // namespace sap_amd

@@ -36,7 +36,8 @@ void Precond<T>::factor(const std::vector<FactorJob>& jobs)
     ProfScope ps(prof, "band_factor");
     int64_t max_len = 0;
     for (const FactorJob& j : jobs) max_len = std::max(max_len, (int64_t)j.len);
-    launch_band_factor<T>(B_.get(), Bul_.get(), k, jobs_.get(), (int)jobs.size(), max_len, stream);
+    launch_band_factor<T>(B_.get(), Bul_.get(), sep_in_ ? Bin_.get() : nullptr, k, jobs_.get(), (int)jobs.size(),
+                          max_len, stream);
 }
 
 template <class T>
@@ -101,6 +102,13 @@ void Precond<T>::begin_setup(int64_t n_, int k_)
     const bool need_ul = lu_ul() && cfg.precondType == SAP_Spike && P > 1 && k > 0;
     if (need_ul) Bul_.resize((size_t)w * n + 8);
     else Bul_.release();
+    // the first partition is the longest (PartInfo gives the n mod P extra
+    // rows to the first partitions)
+    int64_t s0 = 0, len0 = n;
+    PartInfo{n, P}.bounds(0, s0, len0);
+    sep_in_ = need_ul && band_factor_resident_fits<T>(k, len0);
+    if (sep_in_) Bin_.resize((size_t)w * n + 8);
+    else Bin_.release();
 }
 
 template <class T>
@@ -121,9 +129,10 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
     tm_.mark(0, stream);
     {
         ProfScope ps(prof, "band_to_colmajor");
-        // the UL factorizations' copy of the band is written in the same pass
-        launch_band_to_colmajor<T>(n, k, d_band, B_.get(), stream, Bul_.get());
-        bul_filled_ = Bul_.get() != nullptr;
+        // the UL factorizations' copy of the band, when they factor in
+        // place, is written in the same pass
+        launch_band_to_colmajor<T>(n, k, d_band, band_in(), stream, sep_in_ ? nullptr : Bul_.get());
+        bul_filled_ = !sep_in_ && Bul_.get() != nullptr;
     }
     tm_.mark(1, stream);
     tm_.span(&time_toBanded, 0, 1);
@@ -161,8 +170,8 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
         tm_.mark(0, stream);
         {
             ProfScope ps(prof, "csr_to_band");
-            SAP_HIP_CHECK(hipMemsetAsync(B_.get(), 0, B_.size() * sizeof(T), stream));
-            launch_csr_to_band<double, T>(n, k, d_rp, d_ci, d_val, B_.get(), stream);
+            SAP_HIP_CHECK(hipMemsetAsync(band_in(), 0, band_in_bytes(), stream));
+            launch_csr_to_band<double, T>(n, k, d_rp, d_ci, d_val, band_in(), stream);
         }
         tm_.mark(1, stream);
         tm_.span(&time_toBanded, 0, 1);
@@ -265,8 +274,8 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
     SAP_HIP_CHECK(hipEventRecord(ev.e[0], stream));
     {
         ProfScope ps(prof, "csr_to_band");
-        SAP_HIP_CHECK(hipMemsetAsync(B_.get(), 0, B_.size() * sizeof(T), stream));
-        launch_csr_to_band<T, T>(n, k, rp_d.get(), ci_d.get(), val_d.get(), B_.get(), stream);
+        SAP_HIP_CHECK(hipMemsetAsync(band_in(), 0, band_in_bytes(), stream));
+        launch_csr_to_band<T, T>(n, k, rp_d.get(), ci_d.get(), val_d.get(), band_in(), stream);
     }
     SAP_HIP_CHECK(hipEventRecord(ev.e[1], stream));
     time_toBanded = elapsed_ms(ev.e[0], ev.e[1]);   // waits for the fill before the uploads are freed
@@ -291,14 +300,14 @@ bool Precond<T>::update(const double* d_entries, int64_t count, int* rc)
     {
         ProfScope ps(prof, "update_band");
         if (src_ == Source::band) {
-            launch_band_to_colmajor<T>(n, k, d_entries, B_.get(), stream, Bul_.get());
-            bul_filled_ = Bul_.get() != nullptr;
+            launch_band_to_colmajor<T>(n, k, d_entries, band_in(), stream, sep_in_ ? nullptr : Bul_.get());
+            bul_filled_ = !sep_in_ && Bul_.get() != nullptr;
         } else {
-            SAP_HIP_CHECK(hipMemsetAsync(B_.get(), 0, B_.size() * sizeof(T), stream));
+            SAP_HIP_CHECK(hipMemsetAsync(band_in(), 0, band_in_bytes(), stream));
             if (src_ == Source::csr)
-                launch_csr_to_band<double, T>(n, k, src_rp_, src_ci_, d_entries, B_.get(), stream);
+                launch_csr_to_band<double, T>(n, k, src_rp_, src_ci_, d_entries, band_in(), stream);
             else
-                launch_update_band<T>(count, d_entries, upd_map_.get(), upd_scale_.get(), B_.get(), stream);
+                launch_update_band<T>(count, d_entries, upd_map_.get(), upd_scale_.get(), band_in(), stream);
         }
     }
     tm_.mark(1, stream);
@@ -374,19 +383,34 @@ int Precond<T>::factor_band_steps()
     tm_.mark(2, stream);
     {
         ProfScope ps(prof, "extract_coupling");
-        launch_extract_coupling<T>(B_.get(), k, pi, offB_.get(), offC_.get(), stream);
+        launch_extract_coupling<T>(band_in(), k, pi, offB_.get(), offC_.get(), stream);
     }
     tm_.mark(3, stream);
 
     // (3) factorization: LU of partitions 0..P-2 (LU_UL) or 0..P-1 (LU_only);
     //     for LU_UL also UL of partitions 1..P-1 on a copy of the band, in the
-    //     same launch.
+    //     same launch (or both from the one copy in Bin_).
     std::vector<FactorJob> jobs;
     int64_t s_last, l_last;
     pi.bounds(P - 1, s_last, l_last);
     if (lu_ul()) {
-        if (!bul_filled_)
+        if (sep_in_) {
+            // Band entries no factorization writes keep the matrix, as with
+            // in-place factorization: the last partition in B_, the first in
+            // Bul_, and in the others the entries coupling a partition to
+            // its neighbours (rows outside it, in its first and last k
+            // columns; the factorization then overwrites the rest of those
+            // columns).
+            int64_t s0, l0;
+            pi.bounds(0, s0, l0);
+            SAP_HIP_CHECK(hipMemcpyAsync(B_.get() + (size_t)w * s_last, Bin_.get() + (size_t)w * s_last,
+                                         sizeof(T) * w * l_last, hipMemcpyDeviceToDevice, stream));
+            SAP_HIP_CHECK(hipMemcpyAsync(Bul_.get(), Bin_.get(), sizeof(T) * w * l0, hipMemcpyDeviceToDevice, stream));
+            launch_copy_partition_edges<T>(Bin_.get(), B_.get(), k, pi, 0, P - 1, stream);
+            launch_copy_partition_edges<T>(Bin_.get(), Bul_.get(), k, pi, 1, P, stream);
+        } else if (!bul_filled_) {
             SAP_HIP_CHECK(hipMemcpyAsync(Bul_.get(), B_.get(), sizeof(T) * w * n, hipMemcpyDeviceToDevice, stream));
+        }
         for (int p = 0; p < P - 1; ++p) {
             int64_t s, len;
             pi.bounds(p, s, len);

@@ -165,6 +165,13 @@ private:
     // the conversion that preceded factor_band() also wrote Bul_ (the copy
     // the UL factorizations overwrite), so no device copy is needed
     bool bul_filled_ = false;
+    // LU_UL with the register-resident factorization: the matrix is written
+    // once, to Bin_, which the LU (into B_) and UL (into Bul_)
+    // factorizations both read; otherwise both factor in place
+    DevBuf<T> Bin_;
+    bool sep_in_ = false;
+    T* band_in() const { return sep_in_ ? Bin_.get() : B_.get(); }
+    size_t band_in_bytes() const { return (sep_in_ ? Bin_.size() : B_.size()) * sizeof(T); }
     // reordering: row permutation (DB then RCM), column reordering, scalings
     bool reorder_ = false, scale_ = false;
     // variable-bandwidth setup: the preconditioner of the variable-bandwidth

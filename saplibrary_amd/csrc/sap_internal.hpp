@@ -133,6 +133,10 @@ void launch_gen_vector(int64_t n, uint64_t seed, double* x, hipStream_t st);
 void launch_band_spmv(int64_t n, int k, const double* band, const double* x, double* y, hipStream_t st);
 template <class T>
 void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st, T* B2 = nullptr);
+// columns [s, s + k) and [s + len - k, s + len) of partitions p0 .. p1-1
+// (each partition has at least 2k rows), in -> out
+template <class T>
+void launch_copy_partition_edges(const T* in, T* out, int k, PartInfo pi, int p0, int p1, hipStream_t st);
 template <class Tin, class T>   // B zero-filled beforehand
 void launch_csr_to_band(int64_t n, int k, const int* rp, const int* ci, const Tin* vals, T* B, hipStream_t st);
 // B[map[e]] = (T)entries[e] * scale[e] for map[e] >= 0 (scale may be null)
@@ -170,14 +174,20 @@ struct FactorJob {          // one banded LU (dir = +1) or UL (dir = -1) of a pa
     int     boost_last;     // boost also the partition's final pivot
     int     buf;            // 0: main factor band, 1: UL band
 };
-template <class T>   // max_len: longest partition among the jobs
-void launch_band_factor(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, int64_t max_len, hipStream_t st);
+// max_len: longest partition among the jobs.  Bin: the unfactored band the
+// jobs read (the factors go to B0 / B1), or null for in-place factorization;
+// a non-null Bin needs band_factor_resident_fits(k, max_len).
+template <class T>
+void launch_band_factor(T* B0, T* B1, const T* Bin, int k, const FactorJob* d_jobs, int njobs, int64_t max_len,
+                        hipStream_t st);
 // factor_resident.hip: register-resident MFMA factorization; false if k is
 // too wide for it or a partition too long (the caller then uses the
 // LDS-blocked kernels)
 template <class T>
-bool launch_band_factor_resident(T* B0, T* B1, int k, const FactorJob* d_jobs, int njobs, int64_t max_len,
-                                 hipStream_t st);
+bool band_factor_resident_fits(int k, int64_t max_len);
+template <class T>
+bool launch_band_factor_resident(T* B0, T* B1, const T* Bin, int k, const FactorJob* d_jobs, int njobs,
+                                 int64_t max_len, hipStream_t st);
 template <class T>
 void launch_zero_pivot_check(const T* B, int k, int64_t row0, int64_t rows, int* d_flag, hipStream_t st);
 template <class T>
