@@ -921,6 +921,113 @@ __global__ __launch_bounds__(1024) void schur_lu_inv_kernel(T* __restrict__ S, T
     }
 }
 
+// The same L D U' of S_i (the same operations on each entry, in the same
+// order) with the matrix in registers, for k <= 32 R: thread (tr, tc) =
+// (tid % 32, tid / 32) holds rows tr + 32 i and columns tc + 32 j (i, j < R).
+// Step j: the half-wavefront holding column j divides it by the pivot (read
+// from the pivot's lane) and publishes the multipliers, the threads holding
+// row j publish it, and after one barrier every thread applies the rank-1
+// update to its R x R entries from two LDS vectors; rows and columns that
+// are done see a zero multiplier or a zero row entry, which leaves them
+// unchanged.  The published vectors alternate between two buffers, so one
+// barrier per step suffices.
+template <class T>
+__device__ inline T lane_value(T v, int src)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+    } else {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), src);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    }
+}
+
+template <class T, int R>
+__global__ __launch_bounds__(1024) void schur_lu_reg_kernel(T* __restrict__ S, int k, int safe)
+{
+    __shared__ __align__(16) T lbuf[2][32 * 8];
+    __shared__ __align__(16) T ubuf[2][32 * 8];
+    __shared__ T dpiv[32 * R];
+    const int tid = threadIdx.x, tr = tid & 31, tc = tid >> 5;
+    T* A = S + (size_t)blockIdx.x * k * k;
+    T a[R][R];
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int r = tr + 32 * i, c = tc + 32 * j;
+            a[i][j] = (r < k && c < k) ? A[(size_t)c * k + r] : (T)0;
+        }
+#pragma unroll
+    for (int jb = 0; jb < R; ++jb) {
+        for (int jj = 0; jj < 32 && 32 * jb + jj < k; ++jj) {
+            const int j = 32 * jb + jj, buf = j & 1;
+            if (tc == jj) {   // column j (one half-wavefront)
+                T piv = lane_value(a[jb][jb], ((jj & 1) << 5) | jj);
+                if (safe) piv = boost_pivot(piv);
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    const int r = tr + 32 * i;
+                    T l = (T)0;
+                    if (r > j && r < k) {
+                        l = a[i][jb] / piv;
+                        a[i][jb] = l;
+                    }
+                    lbuf[buf][tr * 8 + i] = l;
+                }
+                if (tr == jj) {
+                    a[jb][jb] = piv;
+                    dpiv[j] = piv;
+                }
+            }
+            if (tr == jj) {   // row j
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const int c = tc + 32 * q;
+                    ubuf[buf][tc * 8 + q] = (c > j && c < k) ? a[jb][q] : (T)0;
+                }
+            }
+            __syncthreads();
+            T l[R], u[R];
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                l[i] = lbuf[buf][tr * 8 + i];
+                u[i] = ubuf[buf][tc * 8 + i];
+            }
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int q = 0; q < R; ++q) a[i][q] -= l[i] * u[q];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int r = tr + 32 * i, c = tc + 32 * q;
+            if (r < k && c < k) {
+                if (r < c) a[i][q] /= dpiv[r];
+                A[(size_t)c * k + r] = a[i][q];
+            }
+        }
+}
+
+template <class T, int R>
+bool launch_schur_lu_reg(T* S, int k, int nb, int safe, hipStream_t st)
+{
+    if constexpr (R > (sizeof(T) == 4 ? 7 : 5)) {
+        return false;
+    } else {
+        if (k > 32 * R) return launch_schur_lu_reg<T, R + 1>(S, k, nb, safe, st);
+        hipLaunchKernelGGL((schur_lu_reg_kernel<T, R>), dim3(nb), dim3(1024), 0, st, S, k, safe);
+        SAP_LAUNCH_CHECK();
+        return true;
+    }
+}
+
 // S_i^{-1} from its L D U' factors: RC identity columns per workgroup.
 template <class T>
 __global__ __launch_bounds__(256) void schur_inverse_kernel(const T* __restrict__ S, int k, T* __restrict__ Sinv)
@@ -1071,11 +1178,13 @@ void launch_reduced_factor(const T* WV, int k, int nb, T* S, T* Sinv, int safe, 
     if (lds_lu <= LDS_LIMIT && k <= 256) {
         // LU in LDS; the inverse by the blocked matrix-core sweeps when they fit
         const bool mfma_inv = k <= 32 * STK_IT && StkLayout<T>(k).bytes() <= LDS_LIMIT;
-        SAP_HIP_CHECK(hipFuncSetAttribute((const void*)schur_lu_inv_kernel<T>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_lu));
-        hipLaunchKernelGGL(schur_lu_inv_kernel<T>, dim3(nb), dim3(1024), lds_lu, st, S, Sinv, k, safe,
-                           mfma_inv ? 0 : 1);
-        SAP_LAUNCH_CHECK();
+        if (!mfma_inv || !launch_schur_lu_reg<T, 1>(S, k, nb, safe, st)) {
+            SAP_HIP_CHECK(hipFuncSetAttribute((const void*)schur_lu_inv_kernel<T>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_lu));
+            hipLaunchKernelGGL(schur_lu_inv_kernel<T>, dim3(nb), dim3(1024), lds_lu, st, S, Sinv, k, safe,
+                               mfma_inv ? 0 : 1);
+            SAP_LAUNCH_CHECK();
+        }
         if (mfma_inv) {
             const size_t lds = StkLayout<T>(k).bytes();
             constexpr int NR = StkCfg<T>::NR;
