@@ -761,9 +761,15 @@ __global__ __launch_bounds__(256) void schur_gemm_kernel(const T* __restrict__ W
     const size_t kk = (size_t)k * k;
     const T* V = WV + 2 * i * kk;
     const T* W = WV + (2 * i + 1) * kk;
-    const int tr = threadIdx.x % GT;          // row within tile
-    const int tc = (threadIdx.x / GT) * 4;    // 4 columns per thread
-    T acc[4] = {0, 0, 0, 0};
+    // the 32 x 32 tile as four 16 x 16 quarters, one per wavefront, on the
+    // matrix cores: D = V^T W^T, so accumulator q of lane l holds
+    // (W V)(a0 + sa + l % 16, b0 + sb + row(l, q)) and the stores run along
+    // the columns of S
+    using MF = Mfma16<T>;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int sa = (wv & 1) * 16, sb = (wv >> 1) * 16;
+    const int lr = lane & 15, lg = lane >> 4;
+    typename MF::acc_t acc = {0, 0, 0, 0};
     for (int c0 = 0; c0 < k; c0 += GT) {
         for (int e = threadIdx.x; e < GT * GT; e += blockDim.x) {
             const int rr = e % GT, cc = e / GT;
@@ -772,17 +778,15 @@ __global__ __launch_bounds__(256) void schur_gemm_kernel(const T* __restrict__ W
             Vt[cc][rr] = (c0 + cc < k && b < k) ? V[(size_t)b * k + (c0 + cc)] : (T)0;   // V(c, b) at [c][b]
         }
         __syncthreads();
-        for (int cc = 0; cc < GT; ++cc) {
-            const T wv = Wt[tr][cc];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] += wv * Vt[cc][tc + q];
-        }
+        for (int st = 0; st < GT / 4; ++st)
+            acc = MF::mma(Vt[4 * st + lg][sb + lr], Wt[sa + lr][4 * st + lg], acc);
         __syncthreads();
     }
     T* Si = S + i * kk;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int a = a0 + tr, b = b0 + tc + q;
+        const int a = a0 + sa + lr, b = b0 + sb + MF::row(lane, q);
         if (a < k && b < k) Si[(size_t)b * k + a] = (a == b ? (T)1 : (T)0) - acc[q];
     }
 }

@@ -149,13 +149,29 @@ __device__ inline void lds_store_neg_row(float* p, const float (&v)[N])
 __device__ inline float pivot_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
 __device__ inline double pivot_rcp(double v) { return 1.0 / v; }
 
-// All ones when x < 0, else 0: one field extract of the sign bit.  The tile
-// loads combine their range predicates as sign masks of differences (ANDed),
-// so no per-element compare writes a scalar mask register (measured faster
-// there than compares; the panel stores, with one compare per element
-// against a per-thread limit, are cheaper with compares).
+// All ones when x < 0, else 0: one field extract of the sign bit.  Range
+// predicates are combined as sign masks of differences (ANDed), so no
+// per-element compare writes a scalar mask register.
 __device__ inline int neg_mask(int x) { return __builtin_amdgcn_sbfe(x, 31, 1); }
-
+// the sign of v flipped where m is all ones
+__device__ inline float flip_sign(float v, int m) { return __int_as_float(__float_as_int(v) ^ (m & (int)0x80000000)); }
+__device__ inline double flip_sign(double v, int m)
+{
+    const unsigned long long sb = (unsigned long long)((unsigned)m & 0x80000000u) << 32;
+    return __longlong_as_double((long long)((unsigned long long)__double_as_longlong(v) ^ sb));
+}
+// a where m is all ones, else b
+__device__ inline float pick(float a, float b, int m)
+{
+    return __int_as_float((__float_as_int(a) & m) | (__float_as_int(b) & ~m));
+}
+__device__ inline double pick(double a, double b, int m)
+{
+    const long long x = __double_as_longlong(a), y = __double_as_longlong(b);
+    const unsigned lo = ((unsigned)x & (unsigned)m) | ((unsigned)y & ~(unsigned)m);
+    const unsigned hi = ((unsigned)(x >> 32) & (unsigned)m) | ((unsigned)(y >> 32) & ~(unsigned)m);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 template <class T, int KT, int NW>
 __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __restrict__ B0, T* __restrict__ B1,
                                                                    const T* __restrict__ Bin, int k,
@@ -214,23 +230,17 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
         }
     };
     // factors of panel pp from panel buffer bq: its 32 columns (rows 0 .. HR-1:
-    // L, D, U' of the diagonal block and L21) and its 32 U' rows (values are
-    // gathered from LDS first and stored afterwards, so the LDS reads overlap
-    // instead of each store waiting for its own read).  The stores are
-    // unconditional buffer stores: entries outside the band or the partition
-    // get an offset past the end of the buffer and are dropped, so every
-    // panel issues the same number of stores (pp < 0: none land).
-    // Stored by wavefronts 1 .. NW-1 (st = thread index - 64, NTS = 32 SG
-    // threads): wavefront 0 factors the next diagonal block meanwhile.
-    // Thread st stores rows st % SG + SG i of panel column st / SG (runs of
-    // SG consecutive rows per lane group) and columns st / 32 + SG i of U' row
-    // st % 32 (runs of 32), so every bound is a per-thread limit set once per
-    // panel and each element costs an add and a compare.
+    // L, D, U' of the diagonal block and L21) and its 32 U' rows, coalesced
+    // (values are gathered from LDS first and stored afterwards, so the LDS
+    // reads overlap instead of each store waiting for its own read).  The
+    // stores are unconditional buffer stores: entries outside the band get
+    // an offset past the end of the buffer and are dropped, so every panel
+    // issues the same number of stores (pp < 0: none land).
+    // Stored by wavefronts 1 .. NW-1 (st = thread index - 64): wavefront 0
+    // factors the next diagonal block meanwhile.
     constexpr int NTS = NT - 64;
-    static_assert(NTS % 32 == 0, "panel stores: 32 thread groups");
-    constexpr int SG = NTS / 32;
-    constexpr int N1 = (HR + SG - 1) / SG, N2 = (NTR + SG - 1) / SG;
     auto store_panel = [&](int64_t pp, const T* bq, int st) {
+        constexpr int N1 = (32 * HR + NTS - 1) / NTS, N2 = (32 * NTR + NTS - 1) / NTS;
         const int64_t pj0 = 32 * max(pp, (int64_t)0);
         const int obq = rel0 + dir * (int)(pj0 * w + k);
         const int prem = pp < 0 ? 0 : (int)min(len - pj0, (int64_t)(1 << 30));
@@ -240,41 +250,25 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
         const T* qrp = bq + LY::rp_off;   // pivot reciprocals (0 past the partition's end)
         T v1[N1], v2[N2];
         int o1[N1], o2[N2];
-        // column c keeps rows r with lo <= r < hi: inside the partition and
-        // the band (c - k <= r <= c + k)
-        const int c = st / SG, r0 = st - c * SG;
-        const int lo = max(c - k, 0);
-        const int hi = c < pnb ? min(prem, c + k + 1) : 0;
-        const unsigned span1 = (unsigned)max(hi - lo, 0);
-        const int ob1 = (obq + dir * (c * wm1 + r0)) * (int)sizeof(T);
 #pragma unroll
         for (int i = 0; i < N1; ++i) {
-            const int r = r0 + SG * i;
-            const bool ok = (unsigned)(r - lo) < span1;
-            const int rl = SG * i + SG - 1 < HR ? r : min(r, HR - 1);   // LDS row (masked rows past HR read row HR - 1)
-            const T v = qPn[rl * PS + c];
-            if (SG * i + SG <= 32) {            // diagonal block: U' (r < c) scaled by 1 / pivot
-                v1[i] = r < c ? v * qrp[r] : v;
-            } else if (SG * i >= 32) {          // L21, kept negated in LDS
-                v1[i] = -v;
-            } else {
-                v1[i] = r >= 32 ? -v : (r < c ? v * qrp[r & 31] : v);
-            }
-            o1[i] = ok ? ob1 + dir * SG * (int)sizeof(T) * i : -1;
+            const int e = st + NTS * i;
+            const int c = e / HR, r = e - c * HR;
+            const int ok = neg_mask((e - 32 * HR) & (c - pnb) & (r - prem) & (r - c - k - 1) & (c - r - k - 1));
+            const int lrow = ~neg_mask(r - 32);   // L21 rows, kept negated in LDS
+            const int upper = neg_mask(r - c);    // U' of the diagonal block: scaled by 1 / pivot
+            const T v = qPn[(r * PS + c) & ok];
+            const T f = qrp[r & ~lrow & 31];
+            v1[i] = flip_sign(v * pick(f, (T)1, upper), lrow);
+            o1[i] = (((obq + dir * (c * wm1 + r)) * (int)sizeof(T)) & ok) | ~ok;
         }
-        // U' row m keeps columns 32 + cc with cc < NTR, inside the partition
-        // (32 + cc < prem) and the band (32 + cc - m <= k)
-        const int m = st & 31, cc0 = st >> 5;
-        const unsigned span2 = (unsigned)(m < pnb ? max(min(min(NTR, prem - 32), k + m - 31), 0) : 0);
-        const T rpm = qrp[m];
-        const int ob2 = (obq + dir * ((32 + cc0) * wm1 + m)) * (int)sizeof(T);
 #pragma unroll
         for (int i = 0; i < N2; ++i) {
-            const int cc = cc0 + SG * i;
-            const bool ok = (unsigned)cc < span2;
-            const int cl = SG * i + SG - 1 < NTR ? cc : min(cc, NTR - 1);
-            v2[i] = qUb[cl * PS + m] * rpm;
-            o2[i] = ok ? ob2 + dir * SG * wm1 * (int)sizeof(T) * i : -1;
+            const int e = st + NTS * i;
+            const int cc = e >> 5, m = e & 31;
+            const int ok = neg_mask((e - 32 * NTR) & (m - pnb) & (32 + cc - prem) & (32 + cc - m - k - 1));
+            v2[i] = qUb[(cc * PS + m) & ok] * qrp[m];
+            o2[i] = (((obq + dir * ((32 + cc) * wm1 + m)) * (int)sizeof(T)) & ok) | ~ok;
         }
 #pragma unroll
         for (int i = 0; i < N1; ++i) buffer_store<T>(v1[i], rsrc, o1[i]);
