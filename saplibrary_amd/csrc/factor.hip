@@ -1057,15 +1057,43 @@ __global__ __launch_bounds__(256) void schur_inverse_kernel(const T* __restrict_
     }
 }
 
+// 1 for each +-0 among the words of v (FP32: each word; FP64: each pair)
+template <class T>
+__device__ inline unsigned zeros_in(uint4 v)
+{
+    if constexpr (sizeof(T) == 4) {
+        return ((v.x << 1) == 0u) + ((v.y << 1) == 0u) + ((v.z << 1) == 0u) + ((v.w << 1) == 0u);
+    } else {
+        return ((v.x | (v.y << 1)) == 0u) + ((v.z | (v.w << 1)) == 0u);
+    }
+}
+
+// zeros in p[0, n): elements up to 16-byte alignment and past the last full
+// 16 bytes one at a time, the rest in 16-byte loads
+template <class T>
+__device__ inline unsigned count_zeros_range(const T* __restrict__ p, int64_t n, int64_t t, int64_t stride)
+{
+    constexpr int V = 16 / sizeof(T);
+    int64_t head = (int64_t)(((16 - ((uintptr_t)p & 15)) & 15) / sizeof(T));
+    if (head > n) head = n;
+    const int64_t nv = (n - head) / V;
+    unsigned c = 0;
+    for (int64_t i = t; i < head; i += stride) c += p[i] == (T)0;
+    const uint4* q = reinterpret_cast<const uint4*>(p + head);
+    for (int64_t i = t; i < nv; i += stride) c += zeros_in<T>(q[i]);
+    for (int64_t i = head + nv * V + t; i < n; i += stride) c += p[i] == (T)0;
+    return c;
+}
+
 template <class T>
 __global__ void count_zero_kernel(const T* __restrict__ A, int64_t na, const T* __restrict__ B, int64_t nb,
                                   unsigned long long* out)
 {
-    // zeros in A[0, na) and B[0, nb), one grid-stride pass over both ranges
-    unsigned local = 0;
+    // zeros in A[0, na) and B[0, nb)
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += stride)
-        local += (i < na ? A[i] : B[i - na]) == (T)0;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned local = count_zeros_range<T>(A, na, t, stride);
+    if (nb > 0) local += count_zeros_range<T>(B, nb, t, stride);
     for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
     if ((threadIdx.x & 63) == 0 && local) atomicAdd(out, (unsigned long long)local);
 }
