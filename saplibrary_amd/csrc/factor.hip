@@ -371,17 +371,21 @@ template <class T>
 __global__ void extract_coupling_kernel(const T* __restrict__ B, int k, PartInfo pi, T* __restrict__ offB,
                                         T* __restrict__ offC)
 {
+    // boundary blockIdx.x; columns b = blockIdx.y, blockIdx.y + gridDim.y, ...
+    // of both blocks, rows a across the threads (runs along the columns)
     const int i = blockIdx.x;
     int64_t s1, l1;
     pi.bounds(i + 1, s1, l1);
     const int64_t w = 2 * k + 1;
     const size_t kk = (size_t)k * k;
-    for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
-        const int b = e / k, a = e % k;
-        const int64_t rb = s1 - k + a, cb = s1 + b;        // B_i(a, b)
-        offB[i * kk + e] = (b <= a) ? B[cb * w + k + rb - cb] : (T)0;
-        const int64_t rc = s1 + a, cc = s1 - k + b;        // C_{i+1}(a, b)
-        offC[i * kk + e] = (a <= b) ? B[cc * w + k + rc - cc] : (T)0;
+    for (int b = blockIdx.y; b < k; b += gridDim.y) {
+        for (int a = threadIdx.x; a < k; a += blockDim.x) {
+            const size_t e = (size_t)b * k + a;
+            const int64_t rb = s1 - k + a, cb = s1 + b;        // B_i(a, b)
+            offB[i * kk + e] = (b <= a) ? B[cb * w + k + rb - cb] : (T)0;
+            const int64_t rc = s1 + a, cc = s1 - k + b;        // C_{i+1}(a, b)
+            offC[i * kk + e] = (a <= b) ? B[cc * w + k + rc - cc] : (T)0;
+        }
     }
 }
 
@@ -1157,7 +1161,8 @@ template <class T>
 void launch_extract_coupling(const T* B, int k, PartInfo pi, T* offB, T* offC, hipStream_t st)
 {
     if (pi.P < 2) return;
-    hipLaunchKernelGGL(extract_coupling_kernel<T>, dim3(pi.P - 1), dim3(256), 0, st, B, k, pi, offB, offC);
+    hipLaunchKernelGGL(extract_coupling_kernel<T>, dim3(pi.P - 1, std::min(k, 32)), dim3(256), 0, st, B, k, pi, offB,
+                       offC);
     SAP_LAUNCH_CHECK();
 }
 

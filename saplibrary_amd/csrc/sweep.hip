@@ -720,6 +720,38 @@ __device__ void gemv_cm(const T* __restrict__ M, int k, const T* x, T* y, T beta
     }
 }
 
+// The same with 1024 threads: thread (a mod 256, q) sums columns
+// [q k / 4, (q + 1) k / 4) in order, and the four partial sums are added as
+// (p0 + p1) + (p2 + p3) through part[4][256] in LDS, so each output row
+// has four times the loads in flight.  Ends with a barrier (y is complete).
+template <class T>
+__device__ void gemv_cm4(const T* __restrict__ M, int k, const T* x, T* y, T beta_src_sign, const T* src, T* part)
+{
+    const int t = threadIdx.x & 255, q = threadIdx.x >> 8;
+    const int b0 = (k * q) / 4, b1 = (k * (q + 1)) / 4;
+    for (int a0 = 0; a0 < k; a0 += 256) {
+        const int a = a0 + t;
+        T acc = (T)0;
+        if (a < k) {
+            // eight loads issued together, then added in column order
+            int b = b0;
+            for (; b + 8 <= b1; b += 8) {
+                T m[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) m[u] = M[(size_t)(b + u) * k + a];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc += m[u] * x[b + u];
+            }
+            for (; b < b1; ++b) acc += M[(size_t)b * k + a] * x[b];
+        }
+        part[q * 256 + t] = acc;
+        __syncthreads();
+        if (q == 0 && a < k)
+            y[a] = (src ? src[a] : (T)0) + beta_src_sign * ((part[t] + part[256 + t]) + (part[512 + t] + part[768 + t]));
+        __syncthreads();
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void interface_kernel(const T* __restrict__ WV, const T* __restrict__ Sinv,
                                                         const T* __restrict__ offB, const T* __restrict__ offC, int k,
@@ -749,6 +781,37 @@ __global__ __launch_bounds__(256) void interface_kernel(const T* __restrict__ WV
     T* c = corr + (size_t)i * 2 * k;
     gemv_cm<T>(offB + i * kk, k, xt, c, (T)1, nullptr);         // B_i xt
     gemv_cm<T>(offC + i * kk, k, xb, c + k, (T)1, nullptr);     // C_{i+1} xb
+}
+
+// interface_kernel with 1024 threads and gemv_cm4
+template <class T>
+__global__ __launch_bounds__(1024) void interface4_kernel(const T* __restrict__ WV, const T* __restrict__ Sinv,
+                                                          const T* __restrict__ offB, const T* __restrict__ offC,
+                                                          int k, PartInfo pi, const T* __restrict__ g,
+                                                          T* __restrict__ corr)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* part = reinterpret_cast<T*>(smem_raw);
+    T* gb = part + 1024;
+    T* gt = gb + k;
+    T* zt = gt + k;
+    T* xt = zt + k;
+    T* xb = xt + k;
+    const int i = blockIdx.x;
+    int64_t s1, l1;
+    pi.bounds(i + 1, s1, l1);
+    const size_t kk = (size_t)k * k;
+    for (int a = threadIdx.x; a < k; a += blockDim.x) {
+        gb[a] = g[s1 - k + a];
+        gt[a] = g[s1 + a];
+    }
+    __syncthreads();
+    gemv_cm4<T>(WV + (2 * i + 1) * kk, k, gb, zt, (T)-1, gt, part);   // zt = gt - W gb
+    gemv_cm4<T>(Sinv + i * kk, k, zt, xt, (T)1, nullptr, part);        // xt = S^{-1} zt
+    gemv_cm4<T>(WV + 2 * i * kk, k, xt, xb, (T)-1, gb, part);          // xb = gb - V xt
+    T* c = corr + (size_t)i * 2 * k;
+    gemv_cm4<T>(offB + i * kk, k, xt, c, (T)1, nullptr, part);         // B_i xt
+    gemv_cm4<T>(offC + i * kk, k, xb, c + k, (T)1, nullptr, part);     // C_{i+1} xb
 }
 
 template <class T>
@@ -822,8 +885,15 @@ void launch_interface_solve(const T* WV, const T* Sinv, const T* offB, const T* 
                             T* corr, hipStream_t st)
 {
     if (pi.P < 2) return;
-    const size_t lds = sizeof(T) * 5 * (size_t)k;
-    hipLaunchKernelGGL(interface_kernel<T>, dim3(pi.P - 1), dim3(256), lds, st, WV, Sinv, offB, offC, k, pi, g, corr);
+    const size_t lds4 = sizeof(T) * (1024 + 5 * (size_t)k);
+    if (lds4 <= 64 * 1024) {
+        hipLaunchKernelGGL(interface4_kernel<T>, dim3(pi.P - 1), dim3(1024), lds4, st, WV, Sinv, offB, offC, k, pi, g,
+                           corr);
+    } else {
+        const size_t lds = sizeof(T) * 5 * (size_t)k;
+        hipLaunchKernelGGL(interface_kernel<T>, dim3(pi.P - 1), dim3(256), lds, st, WV, Sinv, offB, offC, k, pi, g,
+                           corr);
+    }
     SAP_LAUNCH_CHECK();
 }
 
