@@ -646,10 +646,24 @@ __global__ __launch_bounds__(STK_THREADS) void spike_tips_mfma_kernel(const T* _
     const T* const F = which == 0 ? B : Bul;
     const int64_t o = which == 0 ? s0 + l0 - k : s1;   // first row / column of the corner block
     // Y = the coupling block's columns r0 .. r0 + nrhs - 1 (zero elsewhere), LT = 0
+    // (neighbouring threads read neighbouring rows t of one column, four
+    // loads in flight per thread)
     const T* src = (which == 0 ? offB : offC) + i * kk;
-    for (int e = tid; e < L.kpad * YS; e += STK_THREADS) {
-        const int t = e / YS, r = e - t * YS;
-        Y[e] = (t < k && r < nrhs) ? src[(size_t)(r0 + r) * k + t] : (T)0;
+    const int ytot = L.kpad * YS;
+    for (int e0 = 0; e0 < ytot; e0 += 4 * STK_THREADS) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + tid + u * STK_THREADS;
+            const int r = e / L.kpad, t = e - r * L.kpad;
+            v[u] = (e < ytot && t < k && r < nrhs) ? src[(size_t)(r0 + r) * k + t] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + tid + u * STK_THREADS;
+            const int r = e / L.kpad, t = e - r * L.kpad;
+            if (e < ytot) Y[t * YS + r] = v[u];
+        }
     }
     for (int e = tid; e < 32 * L.lts; e += STK_THREADS) LT[e] = (T)0;
     __syncthreads();
@@ -775,11 +789,23 @@ __global__ __launch_bounds__(256) void schur_gemm_kernel(const T* __restrict__ W
     const int lr = lane & 15, lg = lane >> 4;
     typename MF::acc_t acc = {0, 0, 0, 0};
     for (int c0 = 0; c0 < k; c0 += GT) {
-        for (int e = threadIdx.x; e < GT * GT; e += blockDim.x) {
+        // both tiles' loads issued together (GT * GT / 256 per thread), then stored
+        constexpr int NE = GT * GT / 256;
+        T wv[NE], vv[NE];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = threadIdx.x + 256 * u;
             const int rr = e % GT, cc = e / GT;
             const int a = a0 + rr, c = c0 + cc, b = b0 + rr;
-            Wt[rr][cc] = (a < k && c < k) ? W[(size_t)c * k + a] : (T)0;   // W(a, c)
-            Vt[cc][rr] = (c0 + cc < k && b < k) ? V[(size_t)b * k + (c0 + cc)] : (T)0;   // V(c, b) at [c][b]
+            wv[u] = (a < k && c < k) ? W[(size_t)c * k + a] : (T)0;   // W(a, c)
+            vv[u] = (c0 + cc < k && b < k) ? V[(size_t)b * k + (c0 + cc)] : (T)0;   // V(c, b)
+        }
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = threadIdx.x + 256 * u;
+            const int rr = e % GT, cc = e / GT;
+            Wt[rr][cc] = wv[u];
+            Vt[cc][rr] = vv[u];   // V(c, b) at [c][b]
         }
         __syncthreads();
 #pragma unroll
