@@ -193,6 +193,7 @@ public:
     // true-residual check; x its returned solution), else null; a pool
     // vector valid until the next vec().
     const double* last_res_ = nullptr;
+    double last_res_dot_ = -1.0;   // (last_res_, last_res_) as dot() gave it
 
     Solver(int P, const sap_options& o) : opt(o), P_req(P)
     {
@@ -287,7 +288,8 @@ public:
 
     // ------------------------------------------- BiCGStab(L), sap/bicgstab2.h:31-354
     template <int L>
-    void bicgstabl(double* x, const double* b, BiCGStabLMonitor& monitor, bool x_is_zero, bool correction)
+    void bicgstabl(double* x, const double* b, BiCGStabLMonitor& monitor, bool x_is_zero, bool correction,
+                   double b_dot)
     {
         // for a zero guess the initial residual is b, whose norm the monitor holds
         const double eps = 1e-20;
@@ -335,8 +337,12 @@ public:
             axpby(b, APxx, rr[0], 1.0, -1.0);
             pxx_valid = true;
             r_norm_act = nrm2(rr[0]);
+            const double rdot = hscalar_[0];
             const bool done = monitor.finished(r_norm_act);
-            if (done && monitor.converged()) last_res_ = rr[0];
+            if (done && monitor.converged()) {
+                last_res_ = rr[0];
+                last_res_dot_ = rdot;
+            }
             return done;
         };
 
@@ -349,11 +355,20 @@ public:
         // the preconditioner; the refinement forms b - A x afresh before it
         // declares convergence.)
         bool first_step = correction && x_is_zero;
+        // With a zero guess rr(0) = r0 = b on the first step, so (rr(0), r0)
+        // is (b, b), which the caller may already hold as dot() gives it
+        // (b_dot >= 0): the same kernel on the same values, so the same bits.
+        bool rho_known = x_is_zero && b_dot >= 0.0;
         while (true) {
             rho0 = -omega * rho0;
             monitor.increment(0.25f);
             for (int j = 0; j < L; ++j) {
-                rho1 = dot(rr[j], r0);
+                if (rho_known) {
+                    rho1 = b_dot;
+                    rho_known = false;
+                } else {
+                    rho1 = dot(rr[j], r0);
+                }
                 if (rho0 == 0) { monitor.stop(-10, "rho0 is zero"); break; }
                 const double beta = alpha * rho1 / rho0;
                 rho0 = rho1;
@@ -369,6 +384,7 @@ public:
                     const double* vs[3] = {rr[0], uu[0], xx};
                     nrm2s(3, vs, nv);   // |rr(0)|, and |uu(0)|, |xx| for the stagnation test below
                 }
+                const double rr0_dot = hscalar_[0];
                 r_norm_act = r_norm = nv[0];
                 // rr(j+1) <- A P^{-1} rr(j) of the recursive rr(j).  The
                 // reference forms it before the true-residual check; here it
@@ -389,7 +405,10 @@ public:
                         pxx_valid = true;
                         r_norm_act = r_norm;
                         const bool done = monitor.finished(r_norm_act);
-                        if (done && monitor.converged()) last_res_ = rr[0];
+                        if (done && monitor.converged()) {
+                            last_res_ = rr[0];
+                            last_res_dot_ = rr0_dot;
+                        }
                         if (done) break;
                         prec(rr[0], Pv);
                     } else {
@@ -581,9 +600,10 @@ public:
 
     // One Krylov solve of A x = b (the reference's solve path).
     // b_norm: |b| when the caller has it (as nrm2 gives it), else negative;
-    // x_is_zero: start from x = 0 without reading x
+    // x_is_zero: start from x = 0 without reading x; b_dot: (b, b) as dot()
+    // gives it when the caller has it, else negative
     int krylov(double* x, const double* b, double rtol, double atol, bool x_is_zero, bool correction, float* its,
-               double* rnorm, double* rhs_norm, double b_norm = -1.0)
+               double* rnorm, double* rhs_norm, double b_norm = -1.0, double b_dot = -1.0)
     {
         last_res_ = nullptr;
         if (use_minres()) {
@@ -599,8 +619,8 @@ public:
         }
         bmon.reset(opt.maxNumIterations, 8, rtol, atol);   // maxmSteps = 8 (sap/solver.h:314-319)
         bmon.init(b_norm >= 0 ? b_norm : nrm2(b));
-        if (opt.solverType == SAP_BiCGStab1) bicgstabl<1>(x, b, bmon, x_is_zero, correction);
-        else bicgstabl<2>(x, b, bmon, x_is_zero, correction);
+        if (opt.solverType == SAP_BiCGStab1) bicgstabl<1>(x, b, bmon, x_is_zero, correction, b_dot);
+        else bicgstabl<2>(x, b, bmon, x_is_zero, correction, b_dot);
         *its += bmon.iterations;
         *rnorm = bmon.rNorm;
         *rhs_norm = bmon.rhsNorm;
@@ -751,6 +771,7 @@ public:
                 const double* vs[2] = {bv, xv};
                 nrm2s(2, vs, bx);
             }
+            const double b_dot = hscalar_[0];   // (b, b) as dot() gives it
             bn = bx[0];
             const double tol = opt.absTol + opt.relTol * bn;
             // r = b - A x.  A*0 = 0 exactly, so a zero initial guess needs no
@@ -762,6 +783,7 @@ public:
             bool have_r = bx[1] == 0.0;
             if (have_r) copy(bv, r);
             double r_known = bn;   // |r| while have_r (the norm nrm2 gives for it)
+            double r_dot = b_dot;  // and (r, r) as dot() gives it
             bool recurred = false;
             code = -1;
             for (int outer = 0;; ++outer) {
@@ -770,6 +792,7 @@ public:
                     axpby(bv, Ax, r, 1.0, -1.0);
                     recurred = false;
                     rn = nrm2(r);
+                    r_dot = hscalar_[0];
                 } else {
                     rn = r_known;
                 }
@@ -778,6 +801,7 @@ public:
                     axpby(bv, Ax, r, 1.0, -1.0);
                     recurred = false;
                     rn = nrm2(r);
+                    r_dot = hscalar_[0];
                 }
                 have_r = false;
                 if (std::isnan(rn)) { code = -2; break; }
@@ -786,7 +810,7 @@ public:
                 const double rt = std::max(opt.refineTol, 0.5 * tol / rn);
                 vec_next_ = mark;
                 double inner_r, inner_b;
-                const int c = krylov(dv, r, rt, 0.0, true, true, &its, &inner_r, &inner_b, rn);
+                const int c = krylov(dv, r, rt, 0.0, true, true, &its, &inner_r, &inner_b, rn, r_dot);
                 stats.numRefine++;
                 axpy(dv, xv, 1.0);
                 if (c == -2) { code = -2; break; }
@@ -794,6 +818,7 @@ public:
                     copy(last_res_, r);
                     have_r = recurred = true;
                     r_known = inner_r;   // the monitor's norm of that residual
+                    r_dot = last_res_dot_;
                 }
             }
             message = code == 1 ? "Converged"
