@@ -75,6 +75,18 @@ __global__ void axpby_kernel(int64_t n, double alpha, const double* __restrict__
         z[i] = alpha * x[i] + beta * y[i];
 }
 
+// z = alpha x + beta y with alpha = sign * (num / *den) read on the device
+// (the same division the host would make, so the same alpha)
+__global__ void axpby_ratio_kernel(int64_t n, double num, const double* __restrict__ den, double sign,
+                                   const double* __restrict__ x, double beta, const double* __restrict__ y,
+                                   double* __restrict__ z)
+{
+    const double alpha = sign * (num / *den);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        z[i] = alpha * x[i] + beta * y[i];
+}
+
 __global__ void copy_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ y)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -141,6 +153,13 @@ void launch_dot(int64_t n, const double* a, const double* b, double* d_partials,
 void launch_axpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* z, hipStream_t st)
 {
     hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, alpha, x, beta, y, z);
+    SAP_LAUNCH_CHECK();
+}
+
+void launch_axpy_ratio(int64_t n, double num, const double* den, double sign, const double* x, double* y,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(axpby_ratio_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, num, den, sign, x, 1.0, y, y);
     SAP_LAUNCH_CHECK();
 }
 

@@ -159,6 +159,10 @@ void launch_dots(int64_t n, int nd, const double* const* a, const double* const*
                  double* out, hipStream_t st);
 void launch_axpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* z, hipStream_t st);
 void launch_copy(int64_t n, const double* x, double* y, hipStream_t st);
+// y += sign * (num / *den) x with *den in device memory (as launch_axpby with
+// alpha = sign * (num / den) and beta = 1)
+void launch_axpy_ratio(int64_t n, double num, const double* den, double sign, const double* x, double* y,
+                       hipStream_t st);
 void launch_fill(int64_t n, double v, double* x, hipStream_t st);
 // dst[q] = src (q < nd <= 4) and zero[q] = 0 (q < nz <= 4) in one launch
 void launch_spread(int64_t n, const double* src, double* const* dst, int nd, double* const* zero, int nz,

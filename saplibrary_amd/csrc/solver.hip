@@ -185,6 +185,7 @@ public:
     std::string message;
 
     DevBuf<double> work_, partials_;
+    DevBuf<double> dgam_{1};   // a BiCGStab(l) gamma, kept on the device
     double* hscalar_ = nullptr;
     double* dscalar_map_ = nullptr;   // hscalar_ in the device's address space
     std::vector<double*> vecs_;
@@ -375,16 +376,27 @@ public:
                 for (int i = 0; i <= j; ++i) axpby(rr[i], uu[i], uu[i], 1.0, -beta);
                 prec(uu[j], Pv);
                 spmv(Pv, uu[j + 1]);
-                const double gam = dot(uu[j + 1], r0);
-                if (gam == 0) { monitor.stop(-11, "gamma is zero"); break; }
-                alpha = rho0 / gam;
-                for (int i = 0; i <= j; ++i) axpy(uu[i + 1], rr[i], -alpha);
+                // gamma = (uu(j+1), r0) stays on the device: the updates
+                // rr(i) -= alpha uu(i+1) take alpha = rho0 / gamma from there,
+                // and the host reads gamma with the norms below (one
+                // synchronization instead of two; the same division, so the
+                // same alpha)
+                {
+                    const double* ga[1] = {uu[j + 1]};
+                    const double* gb[1] = {r0};
+                    launch_dots(n, 1, ga, gb, partials_.get(), dgam_.get(), stream);
+                }
+                for (int i = 0; i <= j; ++i) launch_axpy_ratio(n, rho0, dgam_.get(), -1.0, uu[i + 1], rr[i], stream);
+                launch_copy(1, dgam_.get(), dscalar_map_ + 4, stream);
                 double nv[3];
                 {
                     const double* vs[3] = {rr[0], uu[0], xx};
                     nrm2s(3, vs, nv);   // |rr(0)|, and |uu(0)|, |xx| for the stagnation test below
                 }
                 const double rr0_dot = hscalar_[0];
+                const double gam = hscalar_[4];
+                if (gam == 0) { monitor.stop(-11, "gamma is zero"); break; }
+                alpha = rho0 / gam;
                 r_norm_act = r_norm = nv[0];
                 // rr(j+1) <- A P^{-1} rr(j) of the recursive rr(j).  The
                 // reference forms it before the true-residual check; here it
