@@ -366,19 +366,19 @@ template void launch_band_to_colmajor<double>(int64_t, int, const double*, doubl
 
 // The first k and last k columns of partitions p0 .. p1-1 (each column one
 // contiguous run of 2k+1 band entries), copied from in to out.
+// (workgroup (p - p0, y) copies columns y, y + gridDim.y, ... of the 2k;
+// each column's 2k+1 entries are contiguous)
 template <class T>
-__global__ void copy_partition_edges_kernel(const T* __restrict__ in, T* __restrict__ out, int k, PartInfo pi, int p0,
-                                            int64_t total)
+__global__ void copy_partition_edges_kernel(const T* __restrict__ in, T* __restrict__ out, int k, PartInfo pi, int p0)
 {
-    const int64_t w = 2 * (int64_t)k + 1, per = 2 * (int64_t)k * w;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const int p = p0 + (int)(i / per);
-        const int64_t e = i % per, c = e / w;
-        int64_t s, len;
-        pi.bounds(p, s, len);
+    const int64_t w = 2 * (int64_t)k + 1;
+    int64_t s, len;
+    pi.bounds(p0 + (int)blockIdx.x, s, len);
+    for (int c = blockIdx.y; c < 2 * k; c += gridDim.y) {
         const int64_t col = c < k ? s + c : s + len - 2 * k + c;
-        out[col * w + e % w] = in[col * w + e % w];
+        const T* src = in + col * w;
+        T* dst = out + col * w;
+        for (int e = threadIdx.x; e < w; e += blockDim.x) dst[e] = src[e];
     }
 }
 
@@ -386,10 +386,8 @@ template <class T>
 void launch_copy_partition_edges(const T* in, T* out, int k, PartInfo pi, int p0, int p1, hipStream_t st)
 {
     if (k <= 0 || p1 <= p0) return;
-    const int64_t total = (int64_t)(p1 - p0) * 2 * k * (2 * (int64_t)k + 1);
-    int64_t g = (total + 255) / 256;
-    if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(copy_partition_edges_kernel<T>, dim3((unsigned)g), dim3(256), 0, st, in, out, k, pi, p0, total);
+    hipLaunchKernelGGL(copy_partition_edges_kernel<T>, dim3((unsigned)(p1 - p0), (unsigned)std::min(2 * k, 64)),
+                       dim3(256), 0, st, in, out, k, pi, p0);
     SAP_LAUNCH_CHECK();
 }
 template void launch_copy_partition_edges<float>(const float*, float*, int, PartInfo, int, int, hipStream_t);
