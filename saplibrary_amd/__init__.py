@@ -417,11 +417,26 @@ class Solver:
                 self._A = BandedMatrix(entries.view(self._A.band.shape))
         return rc == 1
 
+    def _n(self):
+        if self._A is None:
+            raise SapError(-4, "Illegal call to solve() before setup().")
+        return self._A.num_rows
+
     def solve(self, spmv, b, x):
         """Solves A x = b (sap/solver.h:477-562); x holds the initial guess.
         spmv: None (built-in SpMV of the setup matrix), an MVBanded / SpmvCusp
-        of that matrix, or any callable f(v, Av) on device tensors."""
-        n = b.shape[0]
+        of that matrix, or any callable f(v, Av) on device tensors.
+        b and x are vectors of the setup's n rows, or n x nrhs matrices whose
+        columns are solved one after another (the C ABI takes them
+        column-major with leading dimension n; they are transposed here and
+        x is written back)."""
+        if self._A is None:
+            # the library reports Illegal_solve itself
+            _check(lib().sap_solve(self.h, SPMV_FN(), None, _dev_f64(b, "b"), _dev_f64(x, "x"), 1), self.h)
+        n = self._n()
+        if b.dim() not in (1, 2) or b.shape != x.shape or b.shape[0] != n:
+            raise ValueError(f"b and x must both be ({n},) or ({n}, nrhs); got {tuple(b.shape)} and {tuple(x.shape)}")
+        nrhs = 1 if b.dim() == 1 else b.shape[1]
         if spmv is None or (isinstance(spmv, (MVBanded, SpmvCusp)) and spmv.A is self._A):
             fn = SPMV_FN()
         else:
@@ -431,12 +446,22 @@ class Solver:
                 spmv(v, av)
             fn = SPMV_FN(_cb)
         self._keep = fn
-        rc = lib().sap_solve(self.h, fn, None, _dev_f64(b, "b"), _dev_f64(x, "x"), 1 if b.dim() == 1 else b.shape[1])
+        if nrhs > 1 or b.dim() == 2:
+            bc = b.t().contiguous()
+            xc = x.t().contiguous()
+        else:
+            bc, xc = b, x
+        rc = lib().sap_solve(self.h, fn, None, _dev_f64(bc, "b"), _dev_f64(xc, "x"), nrhs)
         _check(rc, self.h)
+        if xc is not x:
+            x.copy_(xc.t())
         return rc == 1
 
     def precond_apply(self, v, z):
         """z = M^{-1} v (Precond::operator(), sap/precond.h:1370-1386)."""
+        n = self._n()
+        if v.numel() != n or z.numel() != n:
+            raise ValueError(f"v and z must hold the setup's {n} rows")
         _check(lib().sap_precond_apply(self.h, _dev_f64(v, "v"), _dev_f64(z, "z")), self.h)
         return z
 

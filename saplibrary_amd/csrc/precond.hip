@@ -87,9 +87,13 @@ void Precond<T>::begin_setup(int64_t n_, int k_)
     n = n_;
     k = k_;
     time_toBanded = time_offDiags = time_bandLU = time_bandUL = time_assembly = time_fullLU = 0;
-    // Partition clamp of convertToBandedMatrix (sap/precond.h:2116-2117):
-    // each partition needs k+1 <= n_p and 2k <= n_p.
-    const int64_t maxP = std::max<int64_t>(1, n / std::max(k + 1, 2 * k));
+    // Partition clamp of convertToBandedMatrix (sap/precond.h:2116-2117,
+    // :2265-2266): each partition needs k+1 <= n_p and 2k <= n_p.  Every
+    // path uses it: with shorter partitions the 2k-row interface windows of
+    // neighbouring boundaries overlap, and the reduced solve and
+    // purification of one boundary would change the other's values.
+    const int64_t rows = std::max<int64_t>(k + 1, 2 * (int64_t)k);
+    const int64_t maxP = std::max<int64_t>(1, n / rows);
     P = (int)std::min<int64_t>(cfg.numPartitions, maxP);
     if (P < 1) P = 1;
     const int64_t w = 2 * k + 1;
@@ -217,11 +221,17 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
     //     with every pivot boosted as the reference's variable-bandwidth
     //     kernels do.  The host assembly runs for its band extents
     //     (Stats::nuKf, flops_LU).
-    const int Pvar = std::min(cfg.numPartitions, std::max((int)(n_ / (R.k + 1)), 1));
+    //     The reference clamps the partition count to n / (k+1) on this
+    //     path (sap/precond.h:1777-1778); partitions shorter than 2k rows
+    //     make neighbouring interface windows overlap (the reference's
+    //     in-place reduced sweeps, sap/device/sweep_band_var.cuh:21-191, then
+    //     race), so the banded path's clamp n / max(k+1, 2k) is used here
+    //     too, and the assembly below uses the same partition count.
+    begin_setup(n_, R.k);
+    const int Pvar = P;
     var_mode_ = rc.variable && Pvar > 1 && R.k > 0;
 
     // (4) band of the reordered matrix, vectors of the permutations
-    begin_setup(n_, R.k);
     t = wall_now_ms();
     if (var_mode_) {
         const VarBand<T> VB = assemble_var((int)n, R.k, Pvar, R.rp, R.ci, R.val);

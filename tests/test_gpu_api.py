@@ -1,0 +1,173 @@
+"""The solver's interface and monitor paths on the GPU, through the C ABI.
+
+  * Several right-hand sides: an (n, nrhs) b solves each column as the
+    one-column solve does (bit for bit: same kernels, same order).
+  * Length checks: vectors of the wrong length are rejected before any
+    device access.
+  * Monitor codes against the oracle (sap/monitor.h:153-335,
+    sap/bicgstab2.h, sap/minres.h): BiCGStab(2) and BiCGStab(1) stagnation
+    (-3, "Stagnation encountered") with an FP32 preconditioner and no
+    refinement, the best iterate returned on that path (its true residual
+    within 2x of the oracle's), BiCGStab(1) converging (iteration count
+    within a quarter step), MINRES with an indefinite preconditioner (-10)
+    and with M = A (10, "beta2 = 0 ...").
+  * The C++ interface (include/sap/solver.hpp) through a compiled program:
+    setup, solve with MVBanded and a user functor, update(), CSR setup with
+    reordering and SpmvCusp, Illegal_solve before setup.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def sap():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import saplibrary_amd
+    saplibrary_amd.lib()
+    return saplibrary_amd
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def test_several_right_hand_sides_match_single_solves(sap, oracle):
+    n, k, P = 8000, 20, 4
+    band = oracle.gen_banded(n, k, 0.9, 201)
+    B = np.stack([oracle.gen_vector(n, 202), oracle.gen_vector(n, 203), oracle.gen_vector(n, 204)], axis=1)
+    A = sap.BandedMatrix(dev(band))
+    g = sap.Solver(P, sap.Options(factMethod=sap.LU_UL, relTol=1e-10, refine=6))
+    g.setup(A)
+    X = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    assert g.solve(None, dev(B), X)
+    Xh = host(X)
+    for c in range(3):
+        x1 = torch.zeros(n, dtype=torch.float64, device="cuda")
+        assert g.solve(None, dev(B[:, c]), x1)
+        np.testing.assert_array_equal(Xh[:, c], host(x1))
+        r = B[:, c] - oracle.band_spmv(band, Xh[:, c])
+        assert np.linalg.norm(r) / np.linalg.norm(B[:, c]) <= 1e-10
+
+
+def test_wrong_lengths_are_rejected(sap, oracle):
+    n, k = 1000, 5
+    band = oracle.gen_banded(n, k, 1.0, 211)
+    g = sap.Solver(2, sap.Options(factMethod=sap.LU_UL))
+    g.setup(sap.BandedMatrix(dev(band)))
+    short = torch.zeros(n - 1, dtype=torch.float64, device="cuda")
+    full = torch.zeros(n, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        g.solve(None, short, short)
+    with pytest.raises(ValueError):
+        g.solve(None, full, torch.zeros((n, 2), dtype=torch.float64, device="cuda"))
+    with pytest.raises(ValueError):
+        g.precond_apply(short, full)
+
+
+STAGNATION_CASES = [
+    # solver type, oracle name, relTol
+    ("BiCGStab2", "bicgstab2", 1e-12),
+    ("BiCGStab1", "bicgstab1", 1e-12),
+    ("BiCGStab2", "bicgstab2", 1e-14),
+]
+
+
+@pytest.mark.parametrize("stype,oname,tol", STAGNATION_CASES)
+def test_stagnation_and_best_iterate_match_oracle(sap, oracle, stype, oname, tol):
+    # an FP32 preconditioner without refinement cannot push the true
+    # residual below ~1e-7, so the recursive residual meets the tolerance
+    # while the true one does not: 8 checks later the monitor stops with -3
+    # and the solve returns the best iterate (sap/bicgstab2.h:316-353)
+    n, k, P = 3000, 10, 4
+    band = oracle.gen_banded(n, k, 1.0, 5)
+    b = oracle.gen_vector(n, 6)
+    opts = sap.Options(solverType=getattr(sap, stype), factMethod=sap.LU_UL, precondType=sap.Spike,
+                       relTol=tol, maxNumIterations=100)
+    g = sap.Solver(P, opts)
+    g.setup(sap.BandedMatrix(dev(band)))
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert not g.solve(None, dev(b), x)
+    c = oracle.Solver(P, solver=oname, rel_tol=tol, fact="LU_UL", precond="Spike", max_iters=100)
+    c.setup(band)
+    okc, xc = c.solve(b)
+    assert not okc and c.stats()["code"] == -3
+    assert g.getMonitorCode() == -3 and g.getMonitorMessage() == "Stagnation encountered"
+    assert abs(g.getStats().numIterations - c.stats()["num_iterations"]) <= 1.0
+    rg = np.linalg.norm(b - oracle.band_spmv(band, host(x))) / np.linalg.norm(b)
+    rc = np.linalg.norm(b - oracle.band_spmv(band, xc)) / np.linalg.norm(b)
+    assert rg <= 2 * rc and rc <= 2 * rg, (rg, rc)
+
+
+def test_bicgstab1_converges_like_oracle(sap, oracle):
+    n, k, P = 20000, 20, 10
+    band = oracle.gen_banded(n, k, 1.0, 221)
+    xt = oracle.gen_vector(n, 222)
+    b = oracle.band_spmv(band, xt)
+    opts = sap.Options(solverType=sap.BiCGStab1, factMethod=sap.LU_UL, precValueType=sap.PREC_DOUBLE, relTol=1e-12)
+    g = sap.Solver(P, opts)
+    g.setup(sap.BandedMatrix(dev(band)))
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert g.solve(None, dev(b), x)
+    c = oracle.Solver(P, solver="bicgstab1", rel_tol=1e-12, fact="LU_UL", prec_double=True)
+    c.setup(band)
+    okc, _ = c.solve(b)
+    assert okc and g.getMonitorCode() == 1
+    assert abs(g.getStats().numIterations - c.stats()["num_iterations"]) <= 0.25
+    assert np.linalg.norm(b - oracle.band_spmv(band, host(x))) / np.linalg.norm(b) <= 1e-12
+
+
+def _spd_band(oracle, n, k, seed):
+    from tests import spike_dense as SD
+    Ad = SD.dense_from_band(oracle.gen_banded(n, k, 0.0, seed))
+    Ad = 0.5 * (Ad + Ad.T)
+    Ad[np.diag_indices(n)] = np.abs(Ad).sum(axis=1) * 1.1 + 0.1
+    return np.ascontiguousarray(SD.band_from_dense(Ad, k))
+
+
+@pytest.mark.parametrize("case", ["indefinite_preconditioner", "exact_preconditioner"])
+def test_minres_stop_codes_match_oracle(sap, oracle, case):
+    n, k = 3000, 10
+    bandS = _spd_band(oracle, n, k, 95)
+    b = oracle.gen_vector(n, 6)
+    if case == "indefinite_preconditioner":
+        band, P, expect = np.ascontiguousarray(-bandS), 6, -10     # M = -SPD: (r, M^{-1} r) < 0
+    else:
+        band, P, expect = bandS, 1, 10                             # M = A in FP64: beta2 = 0
+    g = sap.Solver(P, sap.Options(solverType=sap.MINRES, precondType=sap.Block, precValueType=sap.PREC_DOUBLE,
+                                  relTol=1e-30, maxNumIterations=300))
+    g.setup(sap.BandedMatrix(dev(band)))
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    okg = g.solve(None, dev(b), x)
+    c = oracle.Solver(P, solver="minres", precond="Block", prec_double=True, rel_tol=1e-30, max_iters=300)
+    c.setup(band)
+    okc, _ = c.solve(b)
+    assert c.stats()["code"] == expect
+    assert g.getMonitorCode() == expect and bool(okg) == bool(okc)
+    assert g.getMonitorMessage() == c.message()
+    assert abs(g.getStats().numIterations - c.stats()["num_iterations"]) <= 1
+
+
+def test_cpp_interface_program_runs(sap):
+    exe = os.path.join(ROOT, "tests", "facade", "facade_solve")
+    if not os.path.exists(exe):
+        pytest.fail("tests/facade/facade_solve is not built (run __graft_entry__.build())")
+    print("[api] running tests/facade/facade_solve", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ALL PASSED" in r.stdout

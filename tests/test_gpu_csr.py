@@ -223,6 +223,43 @@ def test_csr_variable_bandwidth_matches_oracle(sap, oracle, prec):
     np.testing.assert_allclose(host(z), expect, rtol=0, atol=atol)
 
 
+@pytest.mark.parametrize("variable", [True, False])
+def test_reordering_path_partition_clamp(sap, oracle, variable):
+    """One partition count on the reordering path: a request above
+    n / max(k+1, 2k) is clamped to it (the reference clamps with n / (k+1)
+    there, sap/precond.h:1777-1778, which allows partitions shorter than
+    2k rows whose interface windows overlap), the statistics (numPartitions,
+    nuKf from the variable-bandwidth extents) describe that partitioning,
+    and M^{-1} v matches the oracle's LU_only chain with it."""
+    n = 1500
+    rp, ci, val = sparse_band(n, 60, 6, 56)
+    ref = G.reorder(rp, ci, val, prec="double")
+    k = ref["k"]
+    P_req = n // (k + 1)
+    P = n // (2 * k)
+    assert P < P_req, (n, k)
+    opts = sap.Options(precValueType=sap.PREC_DOUBLE, factMethod=sap.LU_only, variableBandwidth=int(variable))
+    g = sap.Solver(P_req, opts)
+    g.setup(sap.CsrMatrix.from_host(rp, ci, val))
+    st = g.getStats()
+    assert g.numPartitions == st.numPartitions == P and st.bandwidth == k
+    if variable:
+        var = G.assemble_var(n, k, P, ref["rp"], ref["ci"], ref["val"], prec="double")
+        nukf = (2.0 * k * n - (var["ks_row"].sum() + var["ks_col"].sum())) / (2.0 * k * n)
+        assert st.nuKf == pytest.approx(nukf, rel=1e-12)
+    band = G.csr_to_band(n, k, ref["rp"], ref["ci"], ref["val"])
+    cpu = oracle.Solver(P_req, fact="LU_only", prec_double=True)
+    assert cpu.setup(band) == 0 and cpu.num_partitions == P
+    v = oracle.gen_vector(n, 57)
+    w = np.zeros(n)
+    w[ref["row_perm"]] = v * ref["row_scale"]
+    expect = np.zeros(n)
+    expect[ref["col_reord"]] = cpu.apply(w) * ref["col_scale"][ref["col_reord"]]
+    z = torch.empty(n, dtype=torch.float64, device="cuda")
+    g.precond_apply(dev(v), z)
+    np.testing.assert_allclose(host(z), expect, rtol=0, atol=1e-10 * np.abs(expect).max())
+
+
 def test_csr_options_rejected(sap):
     rp, ci, val = sparse_band(200, 10, 3, 70)
     A = sap.CsrMatrix.from_host(rp, ci, val)
