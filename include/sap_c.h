@@ -271,6 +271,21 @@ int         sap_reordering_var_get(const sap_reordering* r, int* ks, int64_t* bo
                                    double* offdiags, int* ks_col, int* ks_row, int* diag_row_offsets,
                                    int* diag_col_indices, double* diag_values);
 
+/* Matrix Market input (host only, no GPU), the file format the reference's
+ * drivers read with cusp::io::read_matrix_market_file into a CSR matrix
+ * (examples/matrix_market/driver_mm.cu:135-140).  Coordinate and array
+ * files; real, integer and pattern fields; general, symmetric and
+ * skew-symmetric storage (the other triangle is generated).  The CSR is
+ * zero-based, sorted by (row, column), duplicates kept as separate entries.
+ * NULL on failure, with *err set (SAP_ERR_BADARG for a malformed or missing
+ * file, SAP_ERR_OOM) and the reason copied into msg (msg_len bytes). */
+typedef struct sap_mm_matrix sap_mm_matrix;
+sap_mm_matrix* sap_mm_read(const char* path, int* err, char* msg, int msg_len);
+int         sap_mm_info(const sap_mm_matrix* m, int* rows, int* cols, int64_t* nnz);
+/* copies (rows + 1) offsets, nnz column indices and nnz values (NULL skips) */
+int         sap_mm_get_csr(const sap_mm_matrix* m, int* row_offsets, int* col_indices, double* values);
+void        sap_mm_free(sap_mm_matrix* m);
+
 /* Kernel-family timing since the last sap_reset_profile (profile = 1). */
 int         sap_get_profile(const sap_solver* s, sap_kernel_profile* out, int max_entries);
 void        sap_reset_profile(sap_solver* s);

@@ -134,7 +134,8 @@ EXPORTS = (
     "sap_band_spmv", "sap_get_profile", "sap_reset_profile", "sap_get_factors", "sap_get_spikes",
     "sap_num_partitions", "sap_gen_banded", "sap_gen_vector", "sap_setup_csr", "sap_csr_spmv",
     "sap_reorder_csr", "sap_reordering_info", "sap_reordering_get", "sap_reordering_free", "sap_update",
-    "sap_reordering_assemble_var", "sap_reordering_var_get",
+    "sap_reordering_assemble_var", "sap_reordering_var_get", "sap_mm_read", "sap_mm_info", "sap_mm_get_csr",
+    "sap_mm_free",
 )
 
 _lib = None
@@ -187,6 +188,12 @@ def lib():
     L.sap_reordering_assemble_var.argtypes = [vp, i]
     L.sap_reordering_var_get.argtypes = [vp, ip, ctypes.POINTER(i64), ip, ip, ip, ip, ip, ip, ip, dp, dp, ip, ip,
                                          ip, ip, dp]
+    L.sap_mm_read.argtypes = [ctypes.c_char_p, ip, ctypes.c_char_p, i]
+    L.sap_mm_read.restype = vp
+    L.sap_mm_info.argtypes = [vp, ip, ip, ctypes.POINTER(i64)]
+    L.sap_mm_get_csr.argtypes = [vp, ip, ip, dp]
+    L.sap_mm_free.argtypes = [vp]
+    L.sap_mm_free.restype = None
     _lib = L
     return L
 
@@ -268,6 +275,14 @@ class CsrMatrix:
                          torch.as_tensor(column_indices, dtype=torch.int32).to(device).contiguous(),
                          torch.as_tensor(values, dtype=torch.float64).to(device).contiguous())
 
+    @staticmethod
+    def from_matrix_market(path, device="cuda"):
+        """Reads a Matrix Market file (read_matrix_market) onto the device."""
+        rows, cols, rp, ci, val = read_matrix_market(path)
+        if rows != cols:
+            raise ValueError(f"{path}: the matrix must be square ({rows} x {cols})")
+        return CsrMatrix.from_host(rp, ci, val, device=device)
+
     def _ptrs(self):
         return (_dev_i32(self.row_offsets, "row_offsets"), _dev_i32(self.column_indices, "column_indices"),
                 _dev_f64(self.values, "values"))
@@ -278,6 +293,33 @@ def csr_spmv(A, x, y):
     rp, ci, val = A._ptrs()
     _check(lib().sap_csr_spmv(A.num_rows, A.num_entries, rp, ci, val, _dev_f64(x, "x"), _dev_f64(y, "y"), None))
     return y
+
+
+def read_matrix_market(path):
+    """A Matrix Market file as host CSR arrays (row_offsets, column_indices,
+    values; int32, int32, float64) through the library's reader (the
+    cusp::io::read_matrix_market_file call of the reference's drivers,
+    examples/matrix_market/driver_mm.cu:135-140).  Returns
+    (rows, cols, row_offsets, column_indices, values); raises SapError."""
+    import numpy as np
+    L = lib()
+    err = ctypes.c_int(0)
+    msg = ctypes.create_string_buffer(512)
+    h = L.sap_mm_read(os.fsencode(path), ctypes.byref(err), msg, len(msg))
+    if not h:
+        raise SapError(err.value, msg.value.decode(errors="replace"))
+    h = ctypes.c_void_p(h)
+    try:
+        rows, cols, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        _check(L.sap_mm_info(h, ctypes.byref(rows), ctypes.byref(cols), ctypes.byref(nnz)))
+        rp = np.empty(rows.value + 1, dtype=np.int32)
+        ci = np.empty(max(nnz.value, 1), dtype=np.int32)
+        val = np.empty(max(nnz.value, 1))
+        ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+        _check(L.sap_mm_get_csr(h, rp.ctypes.data_as(ip), ci.ctypes.data_as(ip), val.ctypes.data_as(dp)))
+        return rows.value, cols.value, rp, ci[:nnz.value], val[:nnz.value]
+    finally:
+        L.sap_mm_free(h)
 
 
 class SpmvCusp:

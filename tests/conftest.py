@@ -18,3 +18,12 @@ def oracle():
     from oracle import pyoracle
     pyoracle.lib()
     return pyoracle
+
+
+@pytest.fixture(scope="session")
+def sap():
+    """The package with its HIP library loaded (host-only entry points need
+    no GPU; modules with GPU tests define their own fixture)."""
+    import saplibrary_amd
+    saplibrary_amd.lib()
+    return saplibrary_amd
