@@ -215,6 +215,23 @@ int         sap_update(sap_solver* s, const double* d_entries, int64_t count);
 int         sap_solve(sap_solver* s, sap_spmv_fn spmv, void* ctx,
                       const double* d_b, double* d_x, int nrhs);
 
+/* sap_solve without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (serving loops that
+ * call solve repeatedly, harnesses that forbid hipMalloc).  sap_solve
+ * allocates the solver's Krylov vector pool on its first call; this variant
+ * takes the pool (32 vectors of n doubles) and the reduction partials from
+ * a static device array of the library instead: 32 * SAP_NOALLOC_MAX_N +
+ * 4096 doubles (32 MiB + 32 KiB), one copy per GPU, held until the process
+ * ends.  n above SAP_NOALLOC_MAX_N returns SAP_ERR_BADARG, profile = 1
+ * returns SAP_ERR_UNSUPPORTED.  The setup's buffers are used as by
+ * sap_solve; the array keeps the last call's contents (every vector is
+ * written before it is read).  All calls on one GPU share the array, so
+ * calls that overlap on one GPU (any solver, any stream) race.  Results are
+ * those of sap_solve, bit for bit. */
+#define SAP_NOALLOC_MAX_N 131072
+int         sap_solve_noalloc(sap_solver* s, sap_spmv_fn spmv, void* ctx,
+                              const double* d_b, double* d_x, int nrhs);
+
 int         sap_get_stats(const sap_solver* s, sap_stats* out);
 int         sap_monitor_code(const sap_solver* s);
 const char* sap_monitor_message(const sap_solver* s);

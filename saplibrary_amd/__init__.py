@@ -28,6 +28,8 @@ LU_UL, LU_only = 0, 1
 # sap::PreconditionerType (sap/common.h:92-96)
 Spike, Block, NoPrecond = 0, 1, 2
 PREC_FLOAT, PREC_DOUBLE = 0, 1
+# largest n of sap_solve_noalloc (SAP_NOALLOC_MAX_N of include/sap_c.h)
+NOALLOC_MAX_N = 131072
 
 # sap::system_error::Reason (sap/exception.h:16-23) plus the ABI's own codes
 ERRORS = {
@@ -135,7 +137,7 @@ EXPORTS = (
     "sap_num_partitions", "sap_gen_banded", "sap_gen_vector", "sap_setup_csr", "sap_csr_spmv",
     "sap_reorder_csr", "sap_reordering_info", "sap_reordering_get", "sap_reordering_free", "sap_update",
     "sap_reordering_assemble_var", "sap_reordering_var_get", "sap_mm_read", "sap_mm_info", "sap_mm_get_csr",
-    "sap_mm_free",
+    "sap_mm_free", "sap_solve_noalloc",
 )
 
 _lib = None
@@ -159,6 +161,7 @@ def lib():
     L.sap_set_stream.argtypes = [vp, vp]
     L.sap_setup_banded.argtypes = [vp, i, i, vp]
     L.sap_solve.argtypes = [vp, SPMV_FN, vp, vp, vp, i]
+    L.sap_solve_noalloc.argtypes = [vp, SPMV_FN, vp, vp, vp, i]
     L.sap_get_stats.argtypes = [vp, ctypes.POINTER(Stats)]
     L.sap_monitor_code.argtypes = [vp]
     L.sap_monitor_message.argtypes = [vp]
@@ -472,9 +475,18 @@ class Solver:
         columns are solved one after another (the C ABI takes them
         column-major with leading dimension n; they are transposed here and
         x is written back)."""
+        return self._solve(lib().sap_solve, spmv, b, x)
+
+    def solve_noalloc(self, spmv, b, x):
+        """solve() through sap_solve_noalloc: the Krylov vectors live in the
+        library's static device array instead of a pool allocated on the
+        first solve (n <= NOALLOC_MAX_N, profile off).  Same results."""
+        return self._solve(lib().sap_solve_noalloc, spmv, b, x)
+
+    def _solve(self, entry, spmv, b, x):
         if self._A is None:
             # the library reports Illegal_solve itself
-            _check(lib().sap_solve(self.h, SPMV_FN(), None, _dev_f64(b, "b"), _dev_f64(x, "x"), 1), self.h)
+            _check(entry(self.h, SPMV_FN(), None, _dev_f64(b, "b"), _dev_f64(x, "x"), 1), self.h)
         n = self._n()
         if b.dim() not in (1, 2) or b.shape != x.shape or b.shape[0] != n:
             raise ValueError(f"b and x must both be ({n},) or ({n}, nrhs); got {tuple(b.shape)} and {tuple(x.shape)}")
@@ -493,7 +505,7 @@ class Solver:
             xc = x.t().contiguous()
         else:
             bc, xc = b, x
-        rc = lib().sap_solve(self.h, fn, None, _dev_f64(bc, "b"), _dev_f64(xc, "x"), nrhs)
+        rc = entry(self.h, fn, None, _dev_f64(bc, "b"), _dev_f64(xc, "x"), nrhs)
         _check(rc, self.h)
         if xc is not x:
             x.copy_(xc.t())

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "precond.hpp"
@@ -186,6 +187,12 @@ public:
 
     DevBuf<double> work_, partials_;
     DevBuf<double> dgam_{1};   // a BiCGStab(l) gamma, kept on the device
+    // When set (the allocation-free solve), the vector pool and the
+    // reduction partials are taken from this caller-supplied device memory
+    // instead of work_ / partials_.
+    double* ext_work_ = nullptr;
+    size_t ext_work_size_ = 0;
+    double* ext_partials_ = nullptr;
     double* hscalar_ = nullptr;
     double* dscalar_map_ = nullptr;   // hscalar_ in the device's address space
     std::vector<double*> vecs_;
@@ -242,17 +249,25 @@ public:
     void reserve_vectors(int count)
     {
         const size_t need = (size_t)count * n;
-        if (work_.size() < need) work_.resize(need);
+        double* base;
+        if (ext_work_) {
+            if (ext_work_size_ < need) throw Error(SAP_ERR_BADARG, "solve work space too small");
+            base = ext_work_;
+        } else {
+            if (work_.size() < need) work_.resize(need);
+            partials_.resize(dot_partials_size());
+            base = work_.get();
+        }
         vecs_.clear();
-        for (int i = 0; i < count; ++i) vecs_.push_back(work_.get() + (size_t)i * n);
+        for (int i = 0; i < count; ++i) vecs_.push_back(base + (size_t)i * n);
         vec_next_ = 0;
-        partials_.resize(dot_partials_size());
     }
     double* vec() { return vecs_.at(vec_next_++); }
+    double* partials() const { return ext_partials_ ? ext_partials_ : partials_.get(); }
 
     double dot(const double* a, const double* b)
     {
-        launch_dot(n, a, b, partials_.get(), dscalar_map_, stream);
+        launch_dot(n, a, b, partials(), dscalar_map_, stream);
         SAP_HIP_CHECK(hipStreamSynchronize(stream));
         return hscalar_[0];
     }
@@ -261,7 +276,7 @@ public:
     // (each value as nrm2 gives it)
     void nrm2s(int m, const double* const* v, double* out)
     {
-        launch_dots(n, m, v, v, partials_.get(), dscalar_map_, stream);
+        launch_dots(n, m, v, v, partials(), dscalar_map_, stream);
         SAP_HIP_CHECK(hipStreamSynchronize(stream));
         for (int q = 0; q < m; ++q) out[q] = std::sqrt(hscalar_[q]);
     }
@@ -384,7 +399,7 @@ public:
                 {
                     const double* ga[1] = {uu[j + 1]};
                     const double* gb[1] = {r0};
-                    launch_dots(n, 1, ga, gb, partials_.get(), dgam_.get(), stream);
+                    launch_dots(n, 1, ga, gb, partials(), dgam_.get(), stream);
                 }
                 for (int i = 0; i <= j; ++i) launch_axpy_ratio(n, rho0, dgam_.get(), -1.0, uu[i + 1], rr[i], stream);
                 launch_copy(1, dgam_.get(), dscalar_map_ + 4, stream);
@@ -862,7 +877,57 @@ struct sap_solver {
     std::string err;
 };
 
+// ------------------------------------------- allocation-free solve: scratch
+// The Krylov vector pool (kSolveVectors vectors of up to SAP_NOALLOC_MAX_N
+// doubles) and the reduction partials of sap_solve_noalloc live in this
+// array, one copy per GPU for the life of the process (32 MiB + 32 KiB).
+// Host code never passes the array's name to a kernel: it is a host-side
+// stand-in; the device address comes from hipGetSymbolAddress, once per
+// device, and is kept in g_scratch_ptr.
+constexpr int kSolveVectors = 32;           // what Solver::solve reserves
+constexpr size_t kScratchPartials = 4096;   // >= dot_partials_size()
+constexpr size_t kScratchDoubles = (size_t)kSolveVectors * SAP_NOALLOC_MAX_N + kScratchPartials;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of the scratch array (looked up on the first
+// call on each device).
+double* scratch_on_current_device()
+{
+    int dev = 0;
+    SAP_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error(SAP_ERR_UNSUPPORTED, "device ordinal beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        SAP_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Points the solver's vector pool and reduction partials at the scratch
+// array for the lifetime of the object.
+struct ScratchPool {
+    Solver& s;
+    ScratchPool(Solver& s_, double* base) : s(s_)
+    {
+        s.ext_work_ = base;
+        s.ext_work_size_ = (size_t)kSolveVectors * SAP_NOALLOC_MAX_N;
+        s.ext_partials_ = base + s.ext_work_size_;
+    }
+    ~ScratchPool()
+    {
+        s.ext_work_ = nullptr;
+        s.ext_work_size_ = 0;
+        s.ext_partials_ = nullptr;
+    }
+};
 
 template <class F>
 int guarded(sap_solver* h, F&& f)
@@ -1000,6 +1065,37 @@ int sap_solve(sap_solver* h, sap_spmv_fn spmv, void* ctx, const double* d_b, dou
         int all = 1;
         for (int c = 0; c < nrhs; ++c) {
             const int r = h->s->solve(spmv, ctx, d_b + (size_t)c * h->s->n, d_x + (size_t)c * h->s->n);
+            if (r < 0) return r;
+            all &= r;
+        }
+        return all;
+    });
+}
+
+int sap_solve_noalloc(sap_solver* h, sap_spmv_fn spmv, void* ctx, const double* d_b, double* d_x, int nrhs)
+{
+    if (!h) return SAP_ERR_BADARG;
+    if (!d_b || !d_x || nrhs < 1) { h->err = "bad solve arguments"; return SAP_ERR_BADARG; }
+    Solver& s = *h->s;
+    if (!s.setup_done) return s.solve(spmv, ctx, d_b, d_x);   // Illegal_solve, as sap_solve
+    if (s.n > SAP_NOALLOC_MAX_N) {
+        h->err = "sap_solve_noalloc: n = " + std::to_string(s.n) + " exceeds SAP_NOALLOC_MAX_N = " +
+                 std::to_string(SAP_NOALLOC_MAX_N) + " (use sap_solve)";
+        return SAP_ERR_BADARG;
+    }
+    if (s.opt.profile) {
+        h->err = "sap_solve_noalloc: profiling creates HIP events; use sap_solve";
+        return SAP_ERR_UNSUPPORTED;
+    }
+    if ((size_t)sap_amd::dot_partials_size() > kScratchPartials) {
+        h->err = "sap_solve_noalloc: reduction partials exceed the scratch array";
+        return SAP_ERR_HIP;
+    }
+    return guarded(h, [&] {
+        ScratchPool pool(s, scratch_on_current_device());
+        int all = 1;
+        for (int c = 0; c < nrhs; ++c) {
+            const int r = s.solve(spmv, ctx, d_b + (size_t)c * s.n, d_x + (size_t)c * s.n);
             if (r < 0) return r;
             all &= r;
         }
