@@ -108,6 +108,18 @@ typedef struct sap_options {
                                    right-preconditioned solve                       */
     double refineTol;           /* relative tolerance of each inner solve (1e-6)   */
     int    profile;             /* 1: time every kernel family with HIP events     */
+    int    batch;               /* nrhs > 1 with BiCGStab2 and the built-in SpMV:
+                                   1 (default) solves the columns together (one
+                                   launch per kernel family for all of them, the
+                                   scalars on the device), 0 one after another     */
+    int    graph;               /* batched solves: 1 (default) replays each
+                                   BiCGStab(2) iteration from a captured HIP graph
+                                   (off while profiling)                            */
+    int    autotune;            /* 1: on the first setup of each half-bandwidth and
+                                   precision, time the candidate conversion tile
+                                   widths and sweep staging depths and keep the
+                                   fastest for the process (results unchanged);
+                                   0 (default): fixed choices                      */
 } sap_options;
 
 /* Field-for-field mirror of sap::Stats (sap/solver.h:89-140), times in
@@ -211,7 +223,13 @@ int         sap_update(sap_solver* s, const double* d_entries, int64_t count);
 /* Solves A x = b.  d_x holds the initial guess on entry and the solution on
  * return.  spmv == NULL uses the built-in SpMV of the setup matrix (banded or
  * CSR).
- * nrhs > 1 solves the columns of b/x (leading dimension n) one after another. */
+ * nrhs > 1 solves the columns of b/x (leading dimension n): with the
+ * options' batch = 1, BiCGStab2 and the built-in SpMV all columns together
+ * (each column's iterates are those of solving it alone; without
+ * refinement bit for bit), otherwise one after another.  Stats then hold
+ * the largest iteration count and relative residual over the columns; the
+ * monitor code and message are those of the first column that did not
+ * converge (of the last column when all did). */
 int         sap_solve(sap_solver* s, sap_spmv_fn spmv, void* ctx,
                       const double* d_b, double* d_x, int nrhs);
 
@@ -226,11 +244,40 @@ int         sap_solve(sap_solver* s, sap_spmv_fn spmv, void* ctx,
  * returns SAP_ERR_UNSUPPORTED.  The setup's buffers are used as by
  * sap_solve; the array keeps the last call's contents (every vector is
  * written before it is read).  All calls on one GPU share the array, so
- * calls that overlap on one GPU (any solver, any stream) race.  Results are
- * those of sap_solve, bit for bit. */
+ * calls that overlap on one GPU (any solver, any stream) race.  Several
+ * right-hand sides are solved one after another; results are those of
+ * sap_solve with batch = 0, bit for bit. */
 #define SAP_NOALLOC_MAX_N 131072
 int         sap_solve_noalloc(sap_solver* s, sap_spmv_fn spmv, void* ctx,
                               const double* d_b, double* d_x, int nrhs);
+
+/* Many independent banded systems solved together.  No counterpart in the
+ * reference (its kernels index right-hand sides by block, but a Precond
+ * applies one vector at a time, sap/precond.h:1370-1386): each system gets
+ * its own partitions and preconditioner (as sap_setup_banded builds it),
+ * and all right-hand sides of all systems are one batch of BiCGStab(2)
+ * solves (options as for sap_solve with several right-hand sides; only
+ * BiCGStab2).  Each column's iterates are those of solving it alone.
+ *   sap_batch_create: nsys systems, numPartitions[s] partitions each; NULL
+ *     on bad options.
+ *   sap_batch_setup_banded: host arrays of nsys sizes, half-bandwidths and
+ *     device pointers to the row-major bands (kept non-owning, as by
+ *     sap_setup_banded).
+ *   sap_batch_solve: d_b[s], d_x[s] device pointers to the column-major
+ *     n[s] x nrhs[s] right-hand sides and initial guesses / solutions;
+ *     returns 1 when every column converged, 0 otherwise.
+ *   sap_batch_result: column c (systems in order, nrhs[s] columns each) of
+ *     the last solve: monitor code, iterations, relative residual.
+ *   sap_batch_times: wall times of the last setup and solve (ms). */
+typedef struct sap_batch sap_batch;
+sap_batch*  sap_batch_create(int nsys, const int* numPartitions, const sap_options* opts);
+void        sap_batch_destroy(sap_batch* b);
+int         sap_batch_set_stream(sap_batch* b, void* hip_stream);
+int         sap_batch_setup_banded(sap_batch* b, const int* n, const int* k, const double* const* d_bands);
+int         sap_batch_solve(sap_batch* b, const double* const* d_b, double* const* d_x, const int* nrhs);
+int         sap_batch_result(const sap_batch* b, int column, int* code, float* iterations, double* rel_residual);
+int         sap_batch_times(const sap_batch* b, double* setup_ms, double* solve_ms);
+const char* sap_batch_last_error(const sap_batch* b);
 
 int         sap_get_stats(const sap_solver* s, sap_stats* out);
 int         sap_monitor_code(const sap_solver* s);
@@ -302,6 +349,22 @@ int         sap_mm_info(const sap_mm_matrix* m, int* rows, int* cols, int64_t* n
 /* copies (rows + 1) offsets, nnz column indices and nnz values (NULL skips) */
 int         sap_mm_get_csr(const sap_mm_matrix* m, int* row_offsets, int* col_indices, double* values);
 void        sap_mm_free(sap_mm_matrix* m);
+
+/* The autotuner's choices so far (options.autotune = 1), one entry per
+ * (family, half-bandwidth, element size): family 0 = conversion tile width
+ * (columns per workgroup), 1 = sweep staging depth (LDS block buffers).
+ * Returns the number of entries (copies at most max_entries). */
+typedef struct sap_tune_entry {
+    int    family;
+    int    k;
+    int    elem_bytes;
+    int    value;
+    int    default_value;
+    int    candidates;
+    double best_ms;
+    double default_ms;
+} sap_tune_entry;
+int         sap_autotune_entries(sap_tune_entry* out, int max_entries);
 
 /* Kernel-family timing since the last sap_reset_profile (profile = 1). */
 int         sap_get_profile(const sap_solver* s, sap_kernel_profile* out, int max_entries);

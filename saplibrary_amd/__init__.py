@@ -82,6 +82,9 @@ class Options(ctypes.Structure):
         ("refine", ctypes.c_int),
         ("refineTol", ctypes.c_double),
         ("profile", ctypes.c_int),
+        ("batch", ctypes.c_int),
+        ("graph", ctypes.c_int),
+        ("autotune", ctypes.c_int),
     ]
 
     def __init__(self, **kw):
@@ -137,8 +140,27 @@ EXPORTS = (
     "sap_num_partitions", "sap_gen_banded", "sap_gen_vector", "sap_setup_csr", "sap_csr_spmv",
     "sap_reorder_csr", "sap_reordering_info", "sap_reordering_get", "sap_reordering_free", "sap_update",
     "sap_reordering_assemble_var", "sap_reordering_var_get", "sap_mm_read", "sap_mm_info", "sap_mm_get_csr",
-    "sap_mm_free", "sap_solve_noalloc",
+    "sap_mm_free", "sap_solve_noalloc", "sap_batch_create", "sap_batch_destroy", "sap_batch_set_stream",
+    "sap_batch_setup_banded", "sap_batch_solve", "sap_batch_result", "sap_batch_times", "sap_batch_last_error",
+    "sap_autotune_entries",
 )
+
+class TuneEntry(ctypes.Structure):
+    """sap_tune_entry: one choice of the autotuner."""
+
+    _fields_ = [("family", ctypes.c_int), ("k", ctypes.c_int), ("elem_bytes", ctypes.c_int), ("value", ctypes.c_int),
+                ("default_value", ctypes.c_int), ("candidates", ctypes.c_int), ("best_ms", ctypes.c_double),
+                ("default_ms", ctypes.c_double)]
+
+
+def autotune_entries():
+    """The autotuner's choices so far: [dict] (family 0 = conversion tile
+    width, 1 = sweep staging depth)."""
+    m = lib().sap_autotune_entries(None, 0)
+    arr = (TuneEntry * max(m, 1))()
+    lib().sap_autotune_entries(arr, m)
+    return [{f: getattr(arr[j], f) for f, _ in TuneEntry._fields_} for j in range(m)]
+
 
 _lib = None
 
@@ -197,6 +219,18 @@ def lib():
     L.sap_mm_get_csr.argtypes = [vp, ip, ip, dp]
     L.sap_mm_free.argtypes = [vp]
     L.sap_mm_free.restype = None
+    L.sap_batch_create.argtypes = [i, ip, ctypes.POINTER(Options)]
+    L.sap_batch_create.restype = vp
+    L.sap_batch_destroy.argtypes = [vp]
+    L.sap_batch_destroy.restype = None
+    L.sap_batch_set_stream.argtypes = [vp, vp]
+    L.sap_batch_setup_banded.argtypes = [vp, ip, ip, ctypes.POINTER(vp)]
+    L.sap_batch_solve.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ip]
+    L.sap_batch_result.argtypes = [vp, i, ip, ctypes.POINTER(ctypes.c_float), dp]
+    L.sap_batch_times.argtypes = [vp, dp, dp]
+    L.sap_autotune_entries.argtypes = [ctypes.POINTER(TuneEntry), i]
+    L.sap_batch_last_error.argtypes = [vp]
+    L.sap_batch_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -562,6 +596,80 @@ class Solver:
         _check(lib().sap_get_spikes(self.h, wv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                     s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), wv.size), self.h)
         return wv, s
+
+
+class SystemBatch:
+    """Independent banded systems whose right-hand sides are solved together
+    (sap_batch_* of the C ABI): one BiCGStab(2) batch over the columns of all
+    systems, each system with its own preconditioner."""
+
+    def __init__(self, numPartitions, opts=None):
+        self.opts = opts if opts is not None else Options()
+        P = (ctypes.c_int * len(numPartitions))(*numPartitions)
+        h = lib().sap_batch_create(len(numPartitions), P, ctypes.byref(self.opts))
+        if not h:
+            raise SapError(-9, "sap_batch_create rejected the options")
+        self.h = ctypes.c_void_p(h)
+        self.nsys = len(numPartitions)
+        self._A = None
+        self._ncols = 0
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.sap_batch_destroy(h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc < 0:
+            raise SapError(rc, lib().sap_batch_last_error(self.h).decode())
+        return rc
+
+    def setup(self, mats):
+        """mats: one BandedMatrix per system."""
+        if len(mats) != self.nsys:
+            raise ValueError(f"{self.nsys} matrices expected")
+        n = (ctypes.c_int * self.nsys)(*[A.num_rows for A in mats])
+        k = (ctypes.c_int * self.nsys)(*[A.m_k for A in mats])
+        p = (ctypes.c_void_p * self.nsys)(*[_dev_f64(A.band, "A").value for A in mats])
+        self._check(lib().sap_batch_setup_banded(self.h, n, k, p))
+        self._A = list(mats)
+
+    def solve(self, B, X):
+        """B[s], X[s]: (n_s, nrhs_s) float64 device tensors (X: initial
+        guesses, overwritten with the solutions).  Returns True when every
+        column converged."""
+        if self._A is None:
+            raise SapError(-4, "Illegal call to solve() before setup().")
+        bc, xc, nr = [], [], []
+        for A, b, x in zip(self._A, B, X):
+            if b.dim() != 2 or b.shape != x.shape or b.shape[0] != A.num_rows:
+                raise ValueError("B[s] and X[s] must be (n_s, nrhs_s)")
+            bc.append(b.t().contiguous())
+            xc.append(x.t().contiguous())
+            nr.append(b.shape[1])
+        pb = (ctypes.c_void_p * self.nsys)(*[_dev_f64(t, "B").value for t in bc])
+        px = (ctypes.c_void_p * self.nsys)(*[_dev_f64(t, "X").value for t in xc])
+        rc = self._check(lib().sap_batch_solve(self.h, pb, px, (ctypes.c_int * self.nsys)(*nr)))
+        for x, t in zip(X, xc):
+            x.copy_(t.t())
+        self._ncols = sum(nr)
+        return rc == 1
+
+    def results(self):
+        """[(code, iterations, relative residual)] per column, systems in order."""
+        out = []
+        for c in range(self._ncols):
+            code, its, rel = ctypes.c_int(), ctypes.c_float(), ctypes.c_double()
+            self._check(lib().sap_batch_result(self.h, c, ctypes.byref(code), ctypes.byref(its), ctypes.byref(rel)))
+            out.append((code.value, its.value, rel.value))
+        return out
+
+    def times(self):
+        """(setup ms, solve ms) of the last setup and solve."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().sap_batch_times(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
 
 def _wrap(ptr, n, device):

@@ -107,6 +107,63 @@ __global__ __launch_bounds__(256) void band_spmv_kernel(int64_t n, int k, const 
     if (lane == 0 && row < n) y[row] = s;
 }
 
+// Y(:, c) = A X(:, c) for RB columns per workgroup row (grid.y = column
+// chunks): each band entry is loaded once for the RB columns, and every
+// column gets band_spmv_kernel<G, NI>'s lane assignment, multiply-add order
+// and shuffle tree, so the same bits as a product of that column alone.
+// Columns whose gate flag is 0 are not computed.
+template <int G, int NI, int RB>
+__global__ __launch_bounds__(256) void band_spmm_kernel(int64_t n, int k, const double* __restrict__ band,
+                                                        const double* __restrict__ X, int64_t ldx,
+                                                        double* __restrict__ Y, int64_t ldy, int nr,
+                                                        const int* __restrict__ gate)
+{
+    const int w = 2 * k + 1;
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int c0 = blockIdx.y * RB;
+    bool on[RB];
+#pragma unroll
+    for (int c = 0; c < RB; ++c) on[c] = c0 + c < nr && (!gate || gate[c0 + c]);
+    double s[RB];
+#pragma unroll
+    for (int c = 0; c < RB; ++c) s[c] = 0.0;
+    if (row < n) {
+        const double* r = band + row * w;
+        const int dlo = (int)(row < k ? k - row : 0);
+        const int64_t dhi64 = n - 1 - row + k;
+        const int dhi = (int)(dhi64 < w - 1 ? dhi64 : w - 1);
+        const double* xr = X + (int64_t)c0 * ldx + row - k;
+        for (int d0 = dlo + lane; d0 <= dhi; d0 += NI * G) {
+            double av[NI];
+#pragma unroll
+            for (int m = 0; m < NI; ++m) {
+                const int d = d0 + m * G;
+                av[m] = d <= dhi ? __builtin_nontemporal_load(r + d) : 0.0;
+            }
+#pragma unroll
+            for (int c = 0; c < RB; ++c) {
+                if (!on[c]) continue;
+                double xv[NI];
+#pragma unroll
+                for (int m = 0; m < NI; ++m) {
+                    const int d = d0 + m * G;
+                    xv[m] = d <= dhi ? xr[c * ldx + d] : 0.0;
+                }
+#pragma unroll
+                for (int m = 0; m < NI; ++m)
+                    if (d0 + m * G <= dhi) s[c] = fma(av[m], xv[m], s[c]);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < RB; ++c) {
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) s[c] += __shfl_down(s[c], off, G);
+        if (lane == 0 && row < n && on[c]) Y[(int64_t)(c0 + c) * ldy + row] = s[c];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Column-major band from the row-major input: B[c*(2k+1) + d] = A(c+d-k, c),
 // cast to the preconditioner type (bandedMatrixTranspose,
@@ -286,6 +343,28 @@ __global__ __launch_bounds__(256) void csr_spmv_kernel(int64_t n, const int* __r
     if (lane == 0 && row < n) y[row] = s;
 }
 
+// csr_spmv_kernel for column blockIdx.y of X / Y (same arithmetic per
+// column); columns whose gate flag is 0 are skipped
+template <int G>
+__global__ __launch_bounds__(256) void csr_spmm_kernel(int64_t n, const int* __restrict__ rp,
+                                                       const int* __restrict__ ci, const double* __restrict__ val,
+                                                       const double* __restrict__ X, int64_t ldx,
+                                                       double* __restrict__ Y, int64_t ldy,
+                                                       const int* __restrict__ gate)
+{
+    if (gate && !gate[blockIdx.y]) return;
+    const double* x = X + blockIdx.y * ldx;
+    double* y = Y + blockIdx.y * ldy;
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    double s = 0.0;
+    if (row < n)
+        for (int j = rp[row] + lane; j < rp[row + 1]; j += G) s = fma(val[j], x[ci[j]], s);
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_down(s, off, G);
+    if (lane == 0 && row < n) y[row] = s;
+}
+
 // ---------------------------------------------------------------------------
 // Vector permutations of the reordered preconditioner, in the
 // preconditioner type as the reference computes them on PrecVectors:
@@ -295,8 +374,12 @@ __global__ __launch_bounds__(256) void csr_spmv_kernel(int64_t n, const int* __r
 // ---------------------------------------------------------------------------
 template <class T, bool SCALE_AT_TARGET>
 __global__ void permute_scale_kernel(int64_t n, const double* __restrict__ v, const int* __restrict__ perm,
-                                     const T* __restrict__ scale, double* __restrict__ w)
+                                     const T* __restrict__ scale, double* __restrict__ w, int64_t ldv, int64_t ldw,
+                                     const int* __restrict__ gate)
 {
+    if (gate && !gate[blockIdx.y]) return;   // column blockIdx.y
+    v += blockIdx.y * ldv;
+    w += blockIdx.y * ldw;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const int p = perm[i];
@@ -340,14 +423,65 @@ void launch_band_spmv(int64_t n, int k, const double* band, const double* x, dou
     SAP_LAUNCH_CHECK();
 }
 
-template <class T>
-void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st, T* B2)
+void launch_band_spmm(int64_t n, int k, const double* band, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                      int nr, const int* gate, hipStream_t st)
 {
     const int w = 2 * k + 1;
-    // column tiles of up to 64 columns within 150 KiB of LDS; bands too wide
-    // for 8 columns use the diagonal-box kernel
+    const int bs = 256;
+    constexpr int RB = 4;
+    const unsigned chunks = (unsigned)((nr + RB - 1) / RB);
+    if (w >= 192) {
+        const int64_t blocks = (n * 64 + bs - 1) / bs;
+        hipLaunchKernelGGL((band_spmm_kernel<64, 8, RB>), dim3((unsigned)blocks, chunks), dim3(bs), 0, st, n, k, band,
+                           X, ldx, Y, ldy, nr, gate);
+    } else if (w >= 48) {
+        const int64_t blocks = (n * 16 + bs - 1) / bs;
+        hipLaunchKernelGGL((band_spmm_kernel<16, 12, RB>), dim3((unsigned)blocks, chunks), dim3(bs), 0, st, n, k,
+                           band, X, ldx, Y, ldy, nr, gate);
+    } else {
+        const int64_t blocks = (n * 4 + bs - 1) / bs;
+        hipLaunchKernelGGL((band_spmm_kernel<4, 12, RB>), dim3((unsigned)blocks, chunks), dim3(bs), 0, st, n, k, band,
+                           X, ldx, Y, ldy, nr, gate);
+    }
+    SAP_LAUNCH_CHECK();
+}
+
+void launch_csr_spmm(int64_t n, int64_t nnz, const int* rp, const int* ci, const double* val, const double* X,
+                     int64_t ldx, double* Y, int64_t ldy, int nr, const int* gate, hipStream_t st)
+{
+    const int bs = 256;
+    const double avg = n > 0 ? (double)nnz / (double)n : 0.0;
+    if (avg >= 24) {
+        const int64_t blocks = (n * 32 + bs - 1) / bs;
+        hipLaunchKernelGGL(csr_spmm_kernel<32>, dim3((unsigned)blocks, nr), dim3(bs), 0, st, n, rp, ci, val, X, ldx,
+                           Y, ldy, gate);
+    } else if (avg >= 6) {
+        const int64_t blocks = (n * 8 + bs - 1) / bs;
+        hipLaunchKernelGGL(csr_spmm_kernel<8>, dim3((unsigned)blocks, nr), dim3(bs), 0, st, n, rp, ci, val, X, ldx, Y,
+                           ldy, gate);
+    } else {
+        const int64_t blocks = (n * 2 + bs - 1) / bs;
+        hipLaunchKernelGGL(csr_spmm_kernel<2>, dim3((unsigned)blocks, nr), dim3(bs), 0, st, n, rp, ci, val, X, ldx, Y,
+                           ldy, gate);
+    }
+    SAP_LAUNCH_CHECK();
+}
+
+template <class T>
+int band_to_colmajor_max_tile(int k)
+{
+    // column tiles of up to 64 columns within 150 KiB of LDS
     constexpr size_t budget = 150 * 1024;
-    const int tc = (int)std::min<size_t>(64, budget / ((size_t)(w + 1) * sizeof(T)));
+    return (int)std::min<size_t>(64, budget / ((size_t)(2 * k + 2) * sizeof(T)));
+}
+
+template <class T>
+void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st, T* B2, int tile)
+{
+    const int w = 2 * k + 1;
+    // bands too wide for 8 columns of LDS use the diagonal-box kernel
+    int tc = band_to_colmajor_max_tile<T>(k);
+    if (tile >= 8 && tile < tc) tc = tile;
     if (tc >= 8) {
         const size_t lds = (size_t)tc * (w + 1) * sizeof(T);
         SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_to_colmajor_cols_kernel<T>,
@@ -361,8 +495,10 @@ void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipS
     SAP_LAUNCH_CHECK();
 }
 
-template void launch_band_to_colmajor<float>(int64_t, int, const double*, float*, hipStream_t, float*);
-template void launch_band_to_colmajor<double>(int64_t, int, const double*, double*, hipStream_t, double*);
+template void launch_band_to_colmajor<float>(int64_t, int, const double*, float*, hipStream_t, float*, int);
+template void launch_band_to_colmajor<double>(int64_t, int, const double*, double*, hipStream_t, double*, int);
+template int band_to_colmajor_max_tile<float>(int);
+template int band_to_colmajor_max_tile<double>(int);
 
 // The first k and last k columns of partitions p0 .. p1-1 (each column one
 // contiguous run of 2k+1 band entries), copied from in to out.
@@ -449,18 +585,21 @@ void launch_csr_spmv(int64_t n, int64_t nnz, const int* rp, const int* ci, const
 
 template <class T>
 void launch_permute_scale(int64_t n, const double* v, const int* perm, const T* scale, bool scale_at_target,
-                          double* w, hipStream_t st)
+                          double* w, hipStream_t st, int nr, int64_t ldv, int64_t ldw, const int* gate)
 {
     int64_t g = (n + 255) / 256;
     if (g > 8192) g = 8192;
     if (scale_at_target)
-        hipLaunchKernelGGL((permute_scale_kernel<T, true>), dim3((unsigned)g), dim3(256), 0, st, n, v, perm, scale, w);
+        hipLaunchKernelGGL((permute_scale_kernel<T, true>), dim3((unsigned)g, nr), dim3(256), 0, st, n, v, perm, scale,
+                           w, ldv, ldw, gate);
     else
-        hipLaunchKernelGGL((permute_scale_kernel<T, false>), dim3((unsigned)g), dim3(256), 0, st, n, v, perm, scale, w);
+        hipLaunchKernelGGL((permute_scale_kernel<T, false>), dim3((unsigned)g, nr), dim3(256), 0, st, n, v, perm,
+                           scale, w, ldv, ldw, gate);
     SAP_LAUNCH_CHECK();
 }
-template void launch_permute_scale<float>(int64_t, const double*, const int*, const float*, bool, double*, hipStream_t);
+template void launch_permute_scale<float>(int64_t, const double*, const int*, const float*, bool, double*, hipStream_t,
+                                          int, int64_t, int64_t, const int*);
 template void launch_permute_scale<double>(int64_t, const double*, const int*, const double*, bool, double*,
-                                           hipStream_t);
+                                           hipStream_t, int, int64_t, int64_t, const int*);
 
 }  // namespace sap_amd

@@ -4,6 +4,8 @@
 // block combines them in a fixed order.
 #include "sap_internal.hpp"
 
+#include <algorithm>
+
 namespace sap_amd {
 
 namespace {
@@ -67,12 +69,26 @@ __global__ __launch_bounds__(RED_THREADS) void reduce_final_kernel(int m, int nd
     }
 }
 
+struct SpreadArgs {
+    const double* src;
+    double* dst[4];
+    double* zero[4];
+    int nd, nz;
+};
+
+// one element of z = alpha x + beta y (shared by the single and batched
+// kernels, so both round the same way)
+__device__ __forceinline__ double axpby_elem(double alpha, double x, double beta, double y)
+{
+    return alpha * x + beta * y;
+}
+
 __global__ void axpby_kernel(int64_t n, double alpha, const double* __restrict__ x, double beta,
                              const double* __restrict__ y, double* __restrict__ z)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        z[i] = alpha * x[i] + beta * y[i];
+        z[i] = axpby_elem(alpha, x[i], beta, y[i]);
 }
 
 // z = alpha x + beta y with alpha = sign * (num / *den) read on the device
@@ -84,7 +100,111 @@ __global__ void axpby_ratio_kernel(int64_t n, double num, const double* __restri
     const double alpha = sign * (num / *den);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        z[i] = alpha * x[i] + beta * y[i];
+        z[i] = axpby_elem(alpha, x[i], beta, y[i]);
+}
+
+// ------------------------------------------------------ batched (columns)
+// Column c of a batch vector spans [off[c], off[c] + len[c]); grid.y is the
+// column.  Each column gets the arithmetic of the single-vector kernel above
+// on that column alone (same grid-stride partition, same reduction tree), so
+// the same bits.  A column whose gate flag is 0 is skipped.
+struct BDotPtrs {
+    const double* a[3];
+    const double* b[3];
+};
+
+template <int ND>
+__global__ __launch_bounds__(RED_THREADS) void bdot_partial_kernel(const int64_t* __restrict__ off,
+                                                                   const int* __restrict__ len, BDotPtrs d,
+                                                                   const int* __restrict__ gate,
+                                                                   double* __restrict__ partials)
+{
+    const int c = blockIdx.y;
+    if (gate && !gate[c]) return;
+    const int64_t n = len[c];
+    const int64_t o = off[c];
+    double s[ND];
+#pragma unroll
+    for (int q = 0; q < ND; ++q) s[q] = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+#pragma unroll
+        for (int q = 0; q < ND; ++q) s[q] = fma(d.a[q][o + i], d.b[q][o + i], s[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+        const double t = block_reduce_sum(s[q]);
+        if (threadIdx.x == 0) partials[((size_t)c * ND + q) * RED_BLOCKS + blockIdx.x] = t;
+        __syncthreads();
+    }
+}
+
+// out(q, c) at out[c * out_stride + q]
+__global__ __launch_bounds__(RED_THREADS) void bdot_final_kernel(int nd, const double* __restrict__ partials,
+                                                                 const int* __restrict__ gate, double* __restrict__ out,
+                                                                 int out_stride)
+{
+    const int c = blockIdx.x;
+    if (gate && !gate[c]) return;
+    const double* pc = partials + (size_t)c * nd * RED_BLOCKS;
+    for (int q = 0; q < nd; ++q) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < RED_BLOCKS; i += blockDim.x) s += pc[q * RED_BLOCKS + i];
+        s = block_reduce_sum(s);
+        if (threadIdx.x == 0) out[(size_t)c * out_stride + q] = s;
+        __syncthreads();
+    }
+}
+
+// coefficient of column c: sign * S[c * s_stride + field] (field >= 0) or cst
+__device__ __forceinline__ double bcoef(const BCoef& k, const double* S, int s_stride, int c)
+{
+    return k.field >= 0 ? k.sign * S[(size_t)c * s_stride + k.field] : k.cst;
+}
+
+__global__ void baxpby_kernel(const int64_t* __restrict__ off, const int* __restrict__ len, BCoef ka,
+                              const double* __restrict__ x, BCoef kb, const double* __restrict__ y,
+                              double* __restrict__ z, const double* __restrict__ S, int s_stride,
+                              const int* __restrict__ gate)
+{
+    const int c = blockIdx.y;
+    if (gate && !gate[c]) return;
+    const double alpha = bcoef(ka, S, s_stride, c);
+    const double beta = bcoef(kb, S, s_stride, c);
+    const int64_t n = len[c];
+    const int64_t o = off[c];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        z[o + i] = axpby_elem(alpha, x[o + i], beta, y[o + i]);
+}
+
+__global__ void bcopy_kernel(const int64_t* __restrict__ off, const int* __restrict__ len,
+                             const double* __restrict__ x, double* __restrict__ y, const int* __restrict__ gate)
+{
+    const int c = blockIdx.y;
+    if (gate && !gate[c]) return;
+    const int64_t n = len[c];
+    const int64_t o = off[c];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[o + i] = x[o + i];
+}
+
+// dst[q] = src (q < nd), zero[q] = 0 (q < nz), per gated column
+__global__ void bspread_kernel(const int64_t* __restrict__ off, const int* __restrict__ len, SpreadArgs a,
+                               const int* __restrict__ gate)
+{
+    const int c = blockIdx.y;
+    if (gate && !gate[c]) return;
+    const int64_t n = len[c];
+    const int64_t o = off[c];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (a.nd > 0) {
+            const double v = a.src[o + i];
+            for (int q = 0; q < a.nd; ++q) a.dst[q][o + i] = v;
+        }
+        for (int q = 0; q < a.nz; ++q) a.zero[q][o + i] = 0.0;
+    }
 }
 
 __global__ void copy_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ y)
@@ -99,12 +219,6 @@ __global__ void fill_kernel(int64_t n, double v, double* __restrict__ x)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = v;
 }
 
-struct SpreadArgs {
-    const double* src;
-    double* dst[4];
-    double* zero[4];
-    int nd, nz;
-};
 
 // dst[q] = src for q < nd and zero[q] = 0 for q < nz, in one pass
 __global__ void spread_kernel(int64_t n, SpreadArgs a)
@@ -130,6 +244,65 @@ inline unsigned grid_for(int64_t n)
 }  // namespace
 
 int dot_partials_size() { return 3 * RED_BLOCKS; }
+
+int bdot_partials_size(int cols) { return 3 * RED_BLOCKS * cols; }
+
+// the batched grids: up to 8192 / cols workgroups per column (at least 64)
+inline unsigned bgrid_for(int64_t maxlen, int cols)
+{
+    int64_t g = (maxlen + 255) / 256;
+    const int64_t cap = std::max<int64_t>(64, 8192 / std::max(cols, 1));
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
+void launch_bdots(const BatchCols& bc, int nd, const double* const* a, const double* const* b, const int* gate,
+                  double* d_partials, double* out, int out_stride, hipStream_t st)
+{
+    if (nd < 1 || nd > 3) throw Error(SAP_ERR_BADARG, "launch_bdots: 1 to 3 products");
+    BDotPtrs d{};
+    for (int q = 0; q < nd; ++q) { d.a[q] = a[q]; d.b[q] = b[q]; }
+    const dim3 grid(RED_BLOCKS, bc.cols);
+    if (nd == 1) hipLaunchKernelGGL(bdot_partial_kernel<1>, grid, dim3(RED_THREADS), 0, st, bc.off, bc.len, d, gate, d_partials);
+    else if (nd == 2) hipLaunchKernelGGL(bdot_partial_kernel<2>, grid, dim3(RED_THREADS), 0, st, bc.off, bc.len, d, gate, d_partials);
+    else hipLaunchKernelGGL(bdot_partial_kernel<3>, grid, dim3(RED_THREADS), 0, st, bc.off, bc.len, d, gate, d_partials);
+    SAP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bdot_final_kernel, dim3(bc.cols), dim3(RED_THREADS), 0, st, nd, d_partials, gate, out,
+                       out_stride);
+    SAP_LAUNCH_CHECK();
+}
+
+void launch_baxpby(const BatchCols& bc, BCoef ka, const double* x, BCoef kb, const double* y, double* z,
+                   const double* S, int s_stride, const int* gate, hipStream_t st)
+{
+    hipLaunchKernelGGL(baxpby_kernel, dim3(bgrid_for(bc.maxlen, bc.cols), bc.cols), dim3(256), 0, st, bc.off, bc.len,
+                       ka, x, kb, y, z, S, s_stride, gate);
+    SAP_LAUNCH_CHECK();
+}
+
+void launch_bcopy(const BatchCols& bc, const double* x, double* y, const int* gate, hipStream_t st)
+{
+    hipLaunchKernelGGL(bcopy_kernel, dim3(bgrid_for(bc.maxlen, bc.cols), bc.cols), dim3(256), 0, st, bc.off, bc.len,
+                       x, y, gate);
+    SAP_LAUNCH_CHECK();
+}
+
+void launch_bspread(const BatchCols& bc, const double* src, double* const* dst, int nd, double* const* zero, int nz,
+                    const int* gate, hipStream_t st)
+{
+    if (nd < 0 || nd > 4 || nz < 0 || nz > 4 || (nd > 0 && !src))
+        throw Error(SAP_ERR_BADARG, "launch_bspread: at most four copies and four zero fills");
+    SpreadArgs a{};
+    a.src = src;
+    for (int q = 0; q < nd; ++q) a.dst[q] = dst[q];
+    for (int q = 0; q < nz; ++q) a.zero[q] = zero[q];
+    a.nd = nd;
+    a.nz = nz;
+    hipLaunchKernelGGL(bspread_kernel, dim3(bgrid_for(bc.maxlen, bc.cols), bc.cols), dim3(256), 0, st, bc.off,
+                       bc.len, a, gate);
+    SAP_LAUNCH_CHECK();
+}
 
 void launch_dots(int64_t n, int nd, const double* const* a, const double* const* b, double* d_partials,
                  double* out, hipStream_t st)

@@ -86,6 +86,8 @@ void Precond<T>::begin_setup(int64_t n_, int k_)
 {
     n = n_;
     k = k_;
+    nr_cap_ = 0;   // apply_multi's buffers are sized again for this setup
+    nbuf_ = 0;     // the sweeps' staging depth is tuned again for this k
     time_toBanded = time_offDiags = time_bandLU = time_bandUL = time_assembly = time_fullLU = 0;
     // Partition clamp of convertToBandedMatrix (sap/precond.h:2116-2117,
     // :2265-2266): each partition needs k+1 <= n_p and 2k <= n_p.  Every
@@ -131,16 +133,62 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
     upd_scale_.release();
     tm_.clear();
     tm_.mark(0, stream);
+    tile_ = 0;
+    if (cfg.autotune) {
+        // conversion tile width (a copy: every width writes the same band)
+        const int tmax = band_to_colmajor_max_tile<T>(k);
+        if (tmax >= 8) {
+            std::vector<int> cand;
+            for (int t = 8; t <= tmax; t *= 2) cand.push_back(t);
+            if (cand.back() != tmax) cand.push_back(tmax);
+            T* b2 = sep_in_ ? nullptr : Bul_.get();
+            tile_ = Autotuner::instance().tune(
+                TUNE_B2C_TILE, k, (int)sizeof(T), tmax, cand,
+                [&](int t) { launch_band_to_colmajor<T>(n, k, d_band, band_in(), stream, b2, t); }, stream);
+            tm_.clear();
+            tm_.mark(0, stream);
+        }
+    }
     {
         ProfScope ps(prof, "band_to_colmajor");
         // the UL factorizations' copy of the band, when they factor in
         // place, is written in the same pass
-        launch_band_to_colmajor<T>(n, k, d_band, band_in(), stream, sep_in_ ? nullptr : Bul_.get());
+        launch_band_to_colmajor<T>(n, k, d_band, band_in(), stream, sep_in_ ? nullptr : Bul_.get(), tile_);
         bul_filled_ = !sep_in_ && Bul_.get() != nullptr;
     }
     tm_.mark(1, stream);
     tm_.span(&time_toBanded, 0, 1);
-    return factor_band();
+    const int rc = factor_band();
+    if (rc == 0) tune_sweeps();
+    return rc;
+}
+
+// Staging depth of the triangular sweeps, timed on full applications of
+// this preconditioner (the depth does not change the arithmetic).
+template <class T>
+void Precond<T>::tune_sweeps()
+{
+    nbuf_ = 0;
+    if (!cfg.autotune || k == 0 || cfg.precondType == SAP_None) return;
+    const int nmax = sweep_max_nbuf<T>(k);
+    if (nmax < 4) return;
+    std::vector<int> cand;
+    for (int b = 4; b <= nmax; ++b) cand.push_back(b);
+    if (cand.size() < 2) return;
+    DevBuf<double> tmp(2 * (size_t)n);
+    launch_fill(n, 1.0, tmp.get(), stream);
+    const bool on = prof && prof->enabled;
+    if (on) prof->enabled = false;
+    try {
+        nbuf_ = Autotuner::instance().tune(
+            TUNE_SWEEP_NBUF, k, (int)sizeof(T), nmax, cand,
+            [&](int b) { nbuf_ = b; solve_band(tmp.get(), tmp.get() + n); }, stream);
+    } catch (...) {
+        if (on) prof->enabled = true;
+        throw;
+    }
+    if (on) prof->enabled = true;
+    SAP_HIP_CHECK(hipStreamSynchronize(stream));
 }
 
 template <class T>
@@ -179,7 +227,9 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
         }
         tm_.mark(1, stream);
         tm_.span(&time_toBanded, 0, 1);
-        return factor_band();
+        const int rc0 = factor_band();
+        if (rc0 == 0) tune_sweeps();
+        return rc0;
     }
 
     // (1) matrix to the host (transformToBandedMatrix, sap/precond.h:1656-1713)
@@ -289,7 +339,9 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
     }
     SAP_HIP_CHECK(hipEventRecord(ev.e[1], stream));
     time_toBanded = elapsed_ms(ev.e[0], ev.e[1]);   // waits for the fill before the uploads are freed
-    return factor_band();
+    const int rc1 = factor_band();
+    if (rc1 == 0) tune_sweeps();
+    return rc1;
 }
 
 template <class T>
@@ -310,7 +362,7 @@ bool Precond<T>::update(const double* d_entries, int64_t count, int* rc)
     {
         ProfScope ps(prof, "update_band");
         if (src_ == Source::band) {
-            launch_band_to_colmajor<T>(n, k, d_entries, band_in(), stream, sep_in_ ? nullptr : Bul_.get());
+            launch_band_to_colmajor<T>(n, k, d_entries, band_in(), stream, sep_in_ ? nullptr : Bul_.get(), tile_);
             bul_filled_ = !sep_in_ && Bul_.get() != nullptr;
         } else {
             SAP_HIP_CHECK(hipMemsetAsync(band_in(), 0, band_in_bytes(), stream));
@@ -519,6 +571,8 @@ template <class T>
 void Precond<T>::solve_band(const double* v, double* z)
 {
     const PartInfo pi{n, P};
+    Rhs rh;
+    rh.nbuf = nbuf_;
     if (k == 0) {
         ProfScope ps(prof, "diag_divide");
         launch_diag_divide<T>(B_.get(), n, v, z, stream);
@@ -527,10 +581,10 @@ void Precond<T>::solve_band(const double* v, double* z)
     if (spike_active()) {
         if (lu_ul()) {
             ProfScope ps(prof, "sweep_interface");
-            launch_sweep_interface<T>(B_.get(), Bul_.get(), k, pi, v, g_.get(), scr0_.get(), scr1_.get(), stream);
+            launch_sweep_interface<T>(B_.get(), Bul_.get(), k, pi, v, g_.get(), scr0_.get(), scr1_.get(), stream, rh);
         } else {
             ProfScope ps(prof, "sweep_interface");
-            launch_sweep_full<T>(B_.get(), B_.get(), k, pi, 0, v, nullptr, nullptr, g_.get(), scr0_.get(), stream);
+            launch_sweep_full<T>(B_.get(), B_.get(), k, pi, 0, v, nullptr, nullptr, g_.get(), scr0_.get(), stream, rh);
         }
         {
             ProfScope ps(prof, "interface_solve");
@@ -539,10 +593,90 @@ void Precond<T>::solve_band(const double* v, double* z)
         }
         ProfScope ps(prof, "sweep_full");
         launch_sweep_full<T>(B_.get(), Bul_.get(), k, pi, lu_ul() ? 1 : 0, v, corr_.get(), z, nullptr, scr0_.get(),
-                             stream);
+                             stream, rh);
     } else {
         ProfScope ps(prof, "sweep_full");
-        launch_sweep_full<T>(B_.get(), B_.get(), k, pi, 0, v, nullptr, z, nullptr, scr0_.get(), stream);
+        launch_sweep_full<T>(B_.get(), B_.get(), k, pi, 0, v, nullptr, z, nullptr, scr0_.get(), stream, rh);
+    }
+}
+
+template <class T>
+void Precond<T>::reserve_multi(int nr)
+{
+    if (nr <= nr_cap_) return;
+    const size_t m = (size_t)nr;
+    gm_.resize(m * n);
+    scr0m_.resize(m * n);
+    if (spike_active()) {
+        corrm_.resize(m * 2 * k * (P - 1));
+        if (lu_ul()) scr1m_.resize(m * n);
+    }
+    if (reorder_) {
+        vpm_.resize(m * n);
+        zpm_.resize(m * n);
+    }
+    nr_cap_ = nr;
+}
+
+template <class T>
+void Precond<T>::apply_multi(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate)
+{
+    if (nr < 1) return;
+    if (cfg.precondType == SAP_None) {
+        if (gate) throw Error(SAP_ERR_UNSUPPORTED, "gated apply without a preconditioner");
+        for (int r = 0; r < nr; ++r) launch_copy(n, V + r * ldv, Z + r * ldz, stream);
+        return;
+    }
+    reserve_multi(nr);
+    if (!reorder_) {
+        solve_band_multi(V, ldv, Z, ldz, nr, gate);
+        return;
+    }
+    {
+        ProfScope ps(prof, "permute");
+        launch_permute_scale<T>(n, V, row_perm_.get(), scale_ ? row_scale_.get() : nullptr, false, vpm_.get(), stream,
+                                nr, ldv, n, gate);
+    }
+    solve_band_multi(vpm_.get(), n, zpm_.get(), n, nr, gate);
+    ProfScope ps(prof, "permute");
+    launch_permute_scale<T>(n, zpm_.get(), col_reord_.get(), scale_ ? col_scale_.get() : nullptr, true, Z, stream, nr,
+                            n, ldz, gate);
+}
+
+template <class T>
+void Precond<T>::solve_band_multi(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate)
+{
+    const PartInfo pi{n, P};
+    Rhs rh;
+    rh.nr = nr;
+    rh.ldv = ldv;
+    rh.ldo = ldz;
+    rh.gate = gate;
+    rh.nbuf = nbuf_;
+    if (k == 0) {
+        ProfScope ps(prof, "diag_divide");
+        launch_diag_divide<T>(B_.get(), n, V, Z, stream, rh);
+        return;
+    }
+    if (spike_active()) {
+        if (lu_ul()) {
+            ProfScope ps(prof, "sweep_interface");
+            launch_sweep_interface<T>(B_.get(), Bul_.get(), k, pi, V, gm_.get(), scr0m_.get(), scr1m_.get(), stream, rh);
+        } else {
+            ProfScope ps(prof, "sweep_interface");
+            launch_sweep_full<T>(B_.get(), B_.get(), k, pi, 0, V, nullptr, nullptr, gm_.get(), scr0m_.get(), stream, rh);
+        }
+        {
+            ProfScope ps(prof, "interface_solve");
+            launch_interface_solve<T>(WV_.get(), Sinv_.get(), offB_.get(), offC_.get(), k, pi, gm_.get(), corrm_.get(),
+                                      stream, rh);
+        }
+        ProfScope ps(prof, "sweep_full");
+        launch_sweep_full<T>(B_.get(), Bul_.get(), k, pi, lu_ul() ? 1 : 0, V, corrm_.get(), Z, nullptr, scr0m_.get(),
+                             stream, rh);
+    } else {
+        ProfScope ps(prof, "sweep_full");
+        launch_sweep_full<T>(B_.get(), B_.get(), k, pi, 0, V, nullptr, Z, nullptr, scr0m_.get(), stream, rh);
     }
 }
 

@@ -2,6 +2,7 @@
 // sap/precond.h:53-510), constant-bandwidth band.
 #pragma once
 
+#include "autotune.hpp"
 #include "graph.hpp"
 #include "sap_internal.hpp"
 
@@ -12,6 +13,7 @@ struct PrecondConfig {
     int factMethod = SAP_LU_only;
     int precondType = SAP_Spike;
     int safeFactorization = 0;
+    int autotune = 0;         // time the tunable launch parameters on first use (autotune.hpp)
 };
 
 // Reordering options of a CSR setup, as sap::Solver forwards them to
@@ -136,6 +138,15 @@ public:
     // (solve, sap/precond.h:1393-1406: left permutation and scaling, band
     // solve, right permutation and scaling when the setup reordered).
     void apply(const double* v, double* z);
+    // Z(:, r) = M^{-1} V(:, r) for nr columns (column r at V + r*ldv, Z +
+    // r*ldz), every kernel family in one launch for all columns; columns
+    // whose gate flag (device array of nr ints, may be null) is 0 are
+    // skipped.  Each column gets the arithmetic of apply(), so the same
+    // bits.  Work buffers grow to nr columns on first use.
+    void apply_multi(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate = nullptr);
+    // Allocates apply_multi's work buffers for nr columns (so that a later
+    // apply_multi, e.g. under graph capture, allocates nothing).
+    void reserve_multi(int nr);
     bool reordered() const { return reorder_; }
     // Refactors with new values of the matrix given to the last setup, same
     // pattern (Precond::update, sap/precond.h:751-900): count doubles in
@@ -191,6 +202,14 @@ private:
     int factor_band();
     int factor_band_steps();
     void solve_band(const double* v, double* z);
+    // tuned launch parameters (0: the untuned defaults)
+    int tile_ = 0, nbuf_ = 0;
+    void tune_sweeps();
+    void solve_band_multi(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate);
+    // apply_multi's work buffers (nr_cap_ columns)
+    DevBuf<T> gm_, corrm_, scr0m_, scr1m_;
+    DevBuf<double> vpm_, zpm_;
+    int nr_cap_ = 0;
     void factor(const std::vector<FactorJob>& jobs);
     // true when a pivot of rows [row0, row0 + rows) of band, or of rows
     // [row1, row1 + rows1) of band1 (if given), is below 1e-7 in magnitude

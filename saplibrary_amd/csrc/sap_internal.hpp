@@ -131,8 +131,20 @@ __device__ __forceinline__ void lds_barrier()
 void launch_gen_banded(int64_t n, int k, double dom, uint64_t seed, double* band, hipStream_t st);
 void launch_gen_vector(int64_t n, uint64_t seed, double* x, hipStream_t st);
 void launch_band_spmv(int64_t n, int k, const double* band, const double* x, double* y, hipStream_t st);
+// Y(:, c) = A X(:, c), c < nr (column c at X + c*ldx, Y + c*ldy), each
+// column bit-identical to launch_band_spmv / launch_csr_spmv of it alone;
+// gate (device, nr flags, may be null) skips the columns whose flag is 0
+void launch_band_spmm(int64_t n, int k, const double* band, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                      int nr, const int* gate, hipStream_t st);
+void launch_csr_spmm(int64_t n, int64_t nnz, const int* rp, const int* ci, const double* val, const double* X,
+                     int64_t ldx, double* Y, int64_t ldy, int nr, const int* gate, hipStream_t st);
+// tile: columns per workgroup (0: the widest that fits, at most 64; the
+// arithmetic is a copy, so any tile gives the same band)
 template <class T>
-void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st, T* B2 = nullptr);
+void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipStream_t st, T* B2 = nullptr,
+                             int tile = 0);
+template <class T>
+int band_to_colmajor_max_tile(int k);
 // columns [s, s + k) and [s + len - k, s + len) of partitions p0 .. p1-1
 // (each partition has at least 2k rows), in -> out
 template <class T>
@@ -148,9 +160,11 @@ void launch_csr_bandwidth(int64_t n, const int* rp, const int* ci, int* out, hip
 void launch_csr_spmv(int64_t n, int64_t nnz, const int* rp, const int* ci, const double* val, const double* x,
                      double* y, hipStream_t st);
 // w[perm[i]] = (T)v[i] * scale[scale_at_target ? perm[i] : i]  (scale may be null)
+// (nr columns at v + r*ldv -> w + r*ldw; gate as for the sweeps)
 template <class T>
 void launch_permute_scale(int64_t n, const double* v, const int* perm, const T* scale, bool scale_at_target,
-                          double* w, hipStream_t st);
+                          double* w, hipStream_t st, int nr = 1, int64_t ldv = 0, int64_t ldw = 0,
+                          const int* gate = nullptr);
 
 // blas.hip (FP64 vectors, results of reductions left in device memory)
 void launch_dot(int64_t n, const double* a, const double* b, double* d_partials, double* d_out, hipStream_t st);
@@ -168,6 +182,36 @@ void launch_fill(int64_t n, double v, double* x, hipStream_t st);
 void launch_spread(int64_t n, const double* src, double* const* dst, int nd, double* const* zero, int nz,
                    hipStream_t st);
 int  dot_partials_size();
+
+// Batched vectors: column c of a batch vector spans [off[c], off[c] +
+// len[c]) (device arrays of cols entries); maxlen = the longest column.
+// Every batched kernel gives each column the arithmetic of the
+// single-vector kernel on that column alone (the same bits); columns whose
+// gate flag (device array of cols ints, may be null) is 0 are skipped.
+struct BatchCols {
+    const int64_t* off = nullptr;
+    const int* len = nullptr;
+    int cols = 0;
+    int64_t maxlen = 0;
+};
+// coefficient of column c: sign * S[c * s_stride + field] for field >= 0,
+// else cst
+struct BCoef {
+    int field = -1;
+    double sign = 1.0;
+    double cst = 0.0;
+};
+inline BCoef bconst(double v) { BCoef k; k.cst = v; return k; }
+inline BCoef bfield(int f, double sign = 1.0) { BCoef k; k.field = f; k.sign = sign; return k; }
+int  bdot_partials_size(int cols);
+// out[c * out_stride + q] = a[q](:, c) . b[q](:, c) for q < nd (1..3)
+void launch_bdots(const BatchCols& bc, int nd, const double* const* a, const double* const* b, const int* gate,
+                  double* d_partials, double* out, int out_stride, hipStream_t st);
+void launch_baxpby(const BatchCols& bc, BCoef ka, const double* x, BCoef kb, const double* y, double* z,
+                   const double* S, int s_stride, const int* gate, hipStream_t st);
+void launch_bcopy(const BatchCols& bc, const double* x, double* y, const int* gate, hipStream_t st);
+void launch_bspread(const BatchCols& bc, const double* src, double* const* dst, int nd, double* const* zero, int nz,
+                    const int* gate, hipStream_t st);
 
 // factor.hip
 struct FactorJob {          // one banded LU (dir = +1) or UL (dir = -1) of a partition
@@ -203,17 +247,31 @@ template <class T>
 void launch_reduced_factor(const T* WV, int k, int nb, T* S, T* Sinv, int safe, hipStream_t st);
 
 // sweep.hip
+// Right-hand sides of one launch: nr columns; column r of the input v is at
+// v + r*ldv and of the FP64 output at out + r*ldo; the work vectors (g,
+// scr*, outt: n per column; corr: 2k(P-1) per column) hold nr consecutive
+// blocks.  gate (device, nr flags, may be null) skips the columns whose flag
+// is 0.
+struct Rhs {
+    int nr = 1;
+    int64_t ldv = 0, ldo = 0;
+    const int* gate = nullptr;
+    int nbuf = 0;   // LDS block buffers of the staged sweeps (0: as many as fit)
+};
+// the staged sweeps' largest LDS block-buffer count for k (0: not staged)
+template <class T>
+int sweep_max_nbuf(int k);
 template <class T>
 void launch_sweep_interface(const T* B, const T* Bul, int k, PartInfo pi, const double* v, T* g, T* scr0, T* scr1,
-                            hipStream_t st);
+                            hipStream_t st, const Rhs& rh = Rhs());
 template <class T>
 void launch_interface_solve(const T* WV, const T* Sinv, const T* offB, const T* offC, int k, PartInfo pi,
-                            const T* g, T* corr, hipStream_t st);
+                            const T* g, T* corr, hipStream_t st, const Rhs& rh = Rhs());
 template <class T>
 void launch_sweep_full(const T* B, const T* Bul, int k, PartInfo pi, int last_is_ul, const double* v, const T* corr,
-                       double* outd, T* outt, T* scr, hipStream_t st);
+                       double* outd, T* outt, T* scr, hipStream_t st, const Rhs& rh = Rhs());
 template <class T>
-void launch_diag_divide(const T* B, int64_t n, const double* v, double* out, hipStream_t st);
+void launch_diag_divide(const T* B, int64_t n, const double* v, double* out, hipStream_t st, const Rhs& rh = Rhs());
 // zeros in A[0, na) and B[0, nb) (B may be null when nb = 0)
 template <class T>
 int64_t count_band_zeros(const T* A, int64_t na, const T* B, int64_t nb, unsigned long long* d_cnt, hipStream_t st);

@@ -13,6 +13,7 @@
 #include <mutex>
 #include <string>
 
+#include "batch.hpp"
 #include "precond.hpp"
 
 namespace sap_amd {
@@ -217,6 +218,8 @@ public:
     ~Solver()
     {
         if (hscalar_) (void)hipHostFree(hscalar_);
+        batch_.reset();
+        if (own_stream_) (void)hipStreamDestroy(own_stream_);
     }
 
     void make_precond()
@@ -226,6 +229,7 @@ public:
         c.factMethod = opt.factMethod;
         c.precondType = opt.precondType;
         c.safeFactorization = opt.safeFactorization;
+        c.autotune = opt.autotune;
         if (opt.precValueType == SAP_PREC_DOUBLE) {
             pd = std::make_unique<Precond<double>>();
             pd->cfg = c; pd->stream = stream; pd->prof = &prof;
@@ -768,6 +772,129 @@ public:
         return 1;
     }
 
+    // ------------------------------------------ batched solve (nrhs > 1)
+    // The operators of one system for nr columns (column c at c * n).
+    template <class PC>
+    struct OneSystemOps : BatchOps {
+        Solver& s;
+        PC& pc;
+        int nr;
+        OneSystemOps(Solver& s_, PC& p, int r) : s(s_), pc(p), nr(r) {}
+        void apply(const double* v, double* z, const int* gate) override
+        {
+            pc.apply_multi(v, s.n, z, s.n, nr, gate);
+        }
+        void spmv(const double* x, double* y, const int* gate) override
+        {
+            ProfScope ps(&s.prof, "spmv");
+            if (s.d_rp) launch_csr_spmm(s.n, s.nnz, s.d_rp, s.d_ci, s.d_val, x, s.n, y, s.n, nr, gate, s.stream);
+            else launch_band_spmm(s.n, s.k, s.d_band, x, s.n, y, s.n, nr, gate, s.stream);
+        }
+    };
+    std::unique_ptr<BatchKrylov> batch_;
+    int64_t batch_n_ = 0;
+
+    bool can_batch(sap_spmv_fn fn, int nrhs) const
+    {
+        return nrhs > 1 && opt.batch && !fn && opt.solverType == SAP_BiCGStab2;
+    }
+
+    static const char* bicgstab_message(int c, bool refined)
+    {
+        switch (c) {
+        case 1: return "Converged";
+        case -1: return refined ? "Maximum number of refinement steps was reached"
+                                : "Maximum number of iterations was reached";
+        case -2: return "Residual norm is NaN";
+        case -3: return "Stagnation encountered";
+        case -10: return "rho0 is zero";
+        case -11: return "gamma is zero";
+        case -12: return "a sigma value is zero";
+        default: return "";
+        }
+    }
+
+    // A HIP graph cannot be captured on the null stream: a batched solve
+    // with the graph on a solver that uses the null stream runs on a stream
+    // of its own, after the work already queued on the null stream (the
+    // solve ends with a synchronization, so later work is ordered after it).
+    hipStream_t own_stream_ = nullptr;
+    struct StreamSwap {
+        Solver& s;
+        hipStream_t saved;
+        bool swapped = false;
+        StreamSwap(Solver& s_, bool need) : s(s_), saved(s_.stream)
+        {
+            if (!need || saved != nullptr) return;
+            if (!s.own_stream_) SAP_HIP_CHECK(hipStreamCreateWithFlags(&s.own_stream_, hipStreamNonBlocking));
+            SAP_HIP_CHECK(hipStreamSynchronize(nullptr));
+            s.set_stream(s.own_stream_);
+            swapped = true;
+        }
+        ~StreamSwap()
+        {
+            if (swapped) s.set_stream(saved);
+        }
+    };
+
+    // All nr columns of b / x (leading dimension n) solved together.
+    int solve_batch(const double* d_b, double* d_x, int nr)
+    {
+        if (!setup_done) return SAP_ERR_ILLEGAL_SOLVE;
+        StreamSwap swap(*this, opt.graph != 0 && !opt.profile);
+        user_spmv = nullptr;
+        stats.numApplies = stats.numSpmv = stats.numRefine = 0;
+        if (!batch_ || batch_->columns() != nr || batch_n_ != n || batch_->stream() != stream) {
+            batch_.reset();
+            std::vector<int64_t> off(nr);
+            std::vector<int> len(nr, (int)n);
+            for (int c = 0; c < nr; ++c) off[c] = (int64_t)c * n;
+            batch_ = std::make_unique<BatchKrylov>(stream, off, len);
+            batch_n_ = n;
+        }
+        // the preconditioner's work buffers exist before any graph capture
+        if (pf) pf->reserve_multi(nr);
+        else pd->reserve_multi(nr);
+        BatchSettings bs;
+        bs.relTol = opt.relTol;
+        bs.absTol = opt.absTol;
+        bs.refineTol = opt.refineTol;
+        bs.maxIt = opt.maxNumIterations;
+        bs.refine = opt.refine;
+        bs.graph = opt.graph != 0 && !opt.profile;
+        const double t0 = wall_ms();
+        std::vector<BatchResult> res;
+        if (pf) {
+            OneSystemOps<Precond<float>> ops(*this, *pf, nr);
+            batch_->solve(ops, d_b, d_x, bs, res);
+        } else {
+            OneSystemOps<Precond<double>> ops(*this, *pd, nr);
+            batch_->solve(ops, d_b, d_x, bs, res);
+        }
+        stats.timeSolve = wall_ms() - t0;
+        float its = 0.f;
+        double rel = 0.0;
+        int first_bad = -1;
+        for (int c = 0; c < nr; ++c) {
+            its = std::max(its, res[c].iterations);
+            const double rr = res[c].rNorm / res[c].rhsNorm;
+            if (c == 0 || rr > rel || std::isnan(rr)) {
+                rel = rr;
+                stats.residualNorm = res[c].rNorm;
+                stats.rhsNorm = res[c].rhsNorm;
+            }
+            if (res[c].code <= 0 && first_bad < 0) first_bad = c;
+        }
+        const BatchResult& pick = res[first_bad >= 0 ? first_bad : nr - 1];
+        code = pick.code;
+        message = bicgstab_message(code, opt.refine > 0);
+        bmon.code = code;
+        bmon.message = message;
+        stats.relResidualNorm = rel;
+        stats.numIterations = its;
+        return first_bad < 0 ? 1 : 0;
+    }
+
     int solve(sap_spmv_fn fn, void* ctx, const double* d_b, double* d_x)
     {
         if (!setup_done) return SAP_ERR_ILLEGAL_SOLVE;
@@ -866,6 +993,161 @@ public:
     }
 };
 
+// ---------------------------------------------------------- many systems
+// Independent banded systems, each with its own preconditioner (its own
+// Solver, setup as sap_setup_banded does), whose right-hand sides are solved
+// together by one batched BiCGStab(2): the columns of all systems are the
+// columns of one batch, every Krylov vector kernel covers all of them in
+// one launch, and each iteration is one graph replay.
+class SystemBatch {
+public:
+    sap_options opt;
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<Solver>> sys;
+    std::vector<BatchResult> res;        // per column of the last solve
+    std::vector<int> col0;               // first column of each system
+    double timeSetup = 0, timeSolve = 0;
+
+    SystemBatch(int nsys, const int* P, const sap_options& o) : opt(o)
+    {
+        for (int s = 0; s < nsys; ++s) sys.push_back(std::make_unique<Solver>(P[s], o));
+    }
+    void set_stream(hipStream_t st)
+    {
+        null_user_ = false;
+        switch_stream(st);
+    }
+    void switch_stream(hipStream_t st)
+    {
+        if (st == stream) return;
+        stream = st;
+        for (auto& p : sys) p->set_stream(st);
+        kry_.reset();
+    }
+    int setup_banded(const int* n, const int* k, const double* const* bands)
+    {
+        const double t0 = wall_ms();
+        for (size_t s = 0; s < sys.size(); ++s) {
+            const int rc = sys[s]->setup_banded(n[s], k[s], bands[s]);
+            if (rc != 0) return rc;
+        }
+        timeSetup = wall_ms() - t0;
+        return 0;
+    }
+
+    struct Ops : BatchOps {
+        SystemBatch& b;
+        explicit Ops(SystemBatch& b_) : b(b_) {}
+        void apply(const double* v, double* z, const int* gate) override
+        {
+            for (size_t s = 0; s < b.sys.size(); ++s) {
+                Solver& q = *b.sys[s];
+                const int64_t o = b.base_[s];
+                const int* gs = gate ? gate + b.col0[s] : nullptr;
+                if (q.pf) q.pf->apply_multi(v + o, q.n, z + o, q.n, b.nr_[s], gs);
+                else q.pd->apply_multi(v + o, q.n, z + o, q.n, b.nr_[s], gs);
+            }
+        }
+        void spmv(const double* x, double* y, const int* gate) override
+        {
+            for (size_t s = 0; s < b.sys.size(); ++s) {
+                Solver& q = *b.sys[s];
+                const int64_t o = b.base_[s];
+                launch_band_spmm(q.n, q.k, q.d_band, x + o, q.n, y + o, q.n, b.nr_[s],
+                                 gate ? gate + b.col0[s] : nullptr, b.stream);
+            }
+        }
+    };
+
+    // B[s], X[s]: column-major n_s x nrhs[s] right-hand sides and initial
+    // guesses / solutions of system s.  Returns 1 when every column
+    // converged.
+    int solve(const double* const* B, double* const* X, const int* nrhs)
+    {
+        for (auto& p : sys)
+            if (!p->setup_done) return SAP_ERR_ILLEGAL_SOLVE;
+        // a graph cannot be captured on the null stream (see Solver): with
+        // the caller on the null stream the solves run on a stream of their
+        // own, after the work queued on the null stream so far
+        if (opt.graph && !opt.profile && (stream == nullptr || null_user_)) {
+            if (!own_stream_) SAP_HIP_CHECK(hipStreamCreateWithFlags(&own_stream_, hipStreamNonBlocking));
+            SAP_HIP_CHECK(hipStreamSynchronize(nullptr));
+            switch_stream(own_stream_);
+            null_user_ = true;
+        }
+        const int S = (int)sys.size();
+        bool same = kry_ && (int)nr_.size() == S;
+        for (int s = 0; same && s < S; ++s) same = nr_[s] == nrhs[s] && n_[s] == sys[s]->n;
+        if (!same) {
+            kry_.reset();
+            nr_.assign(nrhs, nrhs + S);
+            n_.resize(S);
+            base_.resize(S);
+            col0.resize(S);
+            std::vector<int64_t> off;
+            std::vector<int> len;
+            int64_t at = 0;
+            for (int s = 0; s < S; ++s) {
+                n_[s] = sys[s]->n;
+                base_[s] = at;
+                col0[s] = (int)off.size();
+                for (int r = 0; r < nr_[s]; ++r) {
+                    off.push_back(at);
+                    len.push_back((int)n_[s]);
+                    at += n_[s];
+                }
+            }
+            kry_ = std::make_unique<BatchKrylov>(stream, off, len);
+            Bb_.resize(at);
+            Xb_.resize(at);
+        }
+        for (int s = 0; s < S; ++s) {
+            Solver& q = *sys[s];
+            if (q.pf) q.pf->reserve_multi(nr_[s]);
+            else q.pd->reserve_multi(nr_[s]);
+        }
+        const double t0 = wall_ms();
+        // batch_ is (re)made on the stream the solve runs on
+        for (int s = 0; s < S; ++s) {
+            const size_t bytes = sizeof(double) * (size_t)n_[s] * nr_[s];
+            SAP_HIP_CHECK(hipMemcpyAsync(Bb_.get() + base_[s], B[s], bytes, hipMemcpyDeviceToDevice, stream));
+            SAP_HIP_CHECK(hipMemcpyAsync(Xb_.get() + base_[s], X[s], bytes, hipMemcpyDeviceToDevice, stream));
+        }
+        BatchSettings bs;
+        bs.relTol = opt.relTol;
+        bs.absTol = opt.absTol;
+        bs.refineTol = opt.refineTol;
+        bs.maxIt = opt.maxNumIterations;
+        bs.refine = opt.refine;
+        bs.graph = opt.graph != 0 && !opt.profile;
+        Ops ops(*this);
+        kry_->solve(ops, Bb_.get(), Xb_.get(), bs, res);
+        for (int s = 0; s < S; ++s)
+            SAP_HIP_CHECK(hipMemcpyAsync(X[s], Xb_.get() + base_[s], sizeof(double) * (size_t)n_[s] * nr_[s],
+                                         hipMemcpyDeviceToDevice, stream));
+        SAP_HIP_CHECK(hipStreamSynchronize(stream));
+        timeSolve = wall_ms() - t0;
+        for (const auto& r : res)
+            if (r.code <= 0) return 0;
+        return 1;
+    }
+
+    ~SystemBatch()
+    {
+        kry_.reset();
+        sys.clear();
+        if (own_stream_) (void)hipStreamDestroy(own_stream_);
+    }
+
+private:
+    hipStream_t own_stream_ = nullptr;
+    bool null_user_ = false;   // the caller's stream is the null stream; the solves run on own_stream_
+    std::unique_ptr<BatchKrylov> kry_;
+    std::vector<int> nr_;
+    std::vector<int64_t> n_, base_;
+    DevBuf<double> Bb_, Xb_;
+};
+
 }  // namespace sap_amd
 
 // ====================================================================== C ABI
@@ -874,6 +1156,11 @@ using sap_amd::Solver;
 
 struct sap_solver {
     std::unique_ptr<Solver> s;
+    std::string err;
+};
+
+struct sap_batch {
+    std::unique_ptr<sap_amd::SystemBatch> b;
     std::string err;
 };
 
@@ -946,6 +1233,23 @@ int guarded(sap_solver* h, F&& f)
     }
 }
 
+template <class F>
+int guarded_batch(sap_batch* h, F&& f)
+{
+    try {
+        return f();
+    } catch (const Error& e) {
+        h->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc& e) {
+        h->err = e.what();
+        return SAP_ERR_OOM;
+    } catch (const std::exception& e) {
+        h->err = e.what();
+        return SAP_ERR_HIP;
+    }
+}
+
 int check_options(const sap_options* o, std::string& why)
 {
     if (o->gpuCount != 1) { why = "gpuCount > 1 is not supported (one GPU)"; return SAP_ERR_UNSUPPORTED; }
@@ -999,6 +1303,25 @@ void sap_default_options(sap_options* o)
     o->refine = 0;
     o->refineTol = 1e-6;
     o->profile = 0;
+    o->batch = 1;
+    o->graph = 1;
+    o->autotune = 0;
+}
+
+int sap_autotune_entries(sap_tune_entry* out, int max_entries)
+{
+    const auto e = sap_amd::Autotuner::instance().entries();
+    for (int i = 0; out && i < max_entries && i < (int)e.size(); ++i) {
+        out[i].family = e[i].family;
+        out[i].k = e[i].k;
+        out[i].elem_bytes = e[i].elem;
+        out[i].value = e[i].value;
+        out[i].default_value = e[i].default_value;
+        out[i].candidates = e[i].candidates;
+        out[i].best_ms = e[i].best_ms;
+        out[i].default_ms = e[i].default_ms;
+    }
+    return (int)e.size();
 }
 
 sap_solver* sap_create(int numPartitions, const sap_options* opts)
@@ -1062,6 +1385,7 @@ int sap_solve(sap_solver* h, sap_spmv_fn spmv, void* ctx, const double* d_b, dou
     if (!h) return SAP_ERR_BADARG;
     if (!d_b || !d_x || nrhs < 1) { h->err = "bad solve arguments"; return SAP_ERR_BADARG; }
     return guarded(h, [&] {
+        if (h->s->can_batch(spmv, nrhs)) return h->s->solve_batch(d_b, d_x, nrhs);
         int all = 1;
         for (int c = 0; c < nrhs; ++c) {
             const int r = h->s->solve(spmv, ctx, d_b + (size_t)c * h->s->n, d_x + (size_t)c * h->s->n);
@@ -1102,6 +1426,72 @@ int sap_solve_noalloc(sap_solver* h, sap_spmv_fn spmv, void* ctx, const double* 
         return all;
     });
 }
+
+sap_batch* sap_batch_create(int nsys, const int* numPartitions, const sap_options* opts)
+{
+    if (nsys < 1 || !numPartitions) return nullptr;
+    sap_options o;
+    if (opts) o = *opts;
+    else sap_default_options(&o);
+    std::string why;
+    if (check_options(&o, why) != SAP_OK || o.solverType != SAP_BiCGStab2) return nullptr;
+    for (int s = 0; s < nsys; ++s)
+        if (numPartitions[s] < 1) return nullptr;
+    auto* h = new (std::nothrow) sap_batch();
+    if (!h) return nullptr;
+    try {
+        h->b = std::make_unique<sap_amd::SystemBatch>(nsys, numPartitions, o);
+    } catch (...) {
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+void sap_batch_destroy(sap_batch* h) { delete h; }
+
+int sap_batch_set_stream(sap_batch* h, void* stream)
+{
+    if (!h) return SAP_ERR_BADARG;
+    h->b->set_stream((hipStream_t)stream);
+    return SAP_OK;
+}
+
+int sap_batch_setup_banded(sap_batch* h, const int* n, const int* k, const double* const* d_bands)
+{
+    if (!h || !n || !k || !d_bands) return SAP_ERR_BADARG;
+    for (size_t s = 0; s < h->b->sys.size(); ++s)
+        if (n[s] <= 0 || k[s] < 0 || !d_bands[s]) { h->err = "bad matrix arguments"; return SAP_ERR_BADARG; }
+    return guarded_batch(h, [&] { return h->b->setup_banded(n, k, d_bands); });
+}
+
+int sap_batch_solve(sap_batch* h, const double* const* d_b, double* const* d_x, const int* nrhs)
+{
+    if (!h || !d_b || !d_x || !nrhs) return SAP_ERR_BADARG;
+    for (size_t s = 0; s < h->b->sys.size(); ++s)
+        if (nrhs[s] < 1 || !d_b[s] || !d_x[s]) { h->err = "bad solve arguments"; return SAP_ERR_BADARG; }
+    return guarded_batch(h, [&] { return h->b->solve(d_b, d_x, nrhs); });
+}
+
+int sap_batch_result(const sap_batch* h, int column, int* code, float* iterations, double* rel_residual)
+{
+    if (!h || column < 0 || column >= (int)h->b->res.size()) return SAP_ERR_BADARG;
+    const auto& r = h->b->res[column];
+    if (code) *code = r.code;
+    if (iterations) *iterations = r.iterations;
+    if (rel_residual) *rel_residual = r.rNorm / r.rhsNorm;
+    return SAP_OK;
+}
+
+int sap_batch_times(const sap_batch* h, double* setup_ms, double* solve_ms)
+{
+    if (!h) return SAP_ERR_BADARG;
+    if (setup_ms) *setup_ms = h->b->timeSetup;
+    if (solve_ms) *solve_ms = h->b->timeSolve;
+    return SAP_OK;
+}
+
+const char* sap_batch_last_error(const sap_batch* h) { return h ? h->err.c_str() : ""; }
 
 int sap_get_stats(const sap_solver* h, sap_stats* out)
 {

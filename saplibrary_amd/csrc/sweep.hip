@@ -21,7 +21,9 @@
 // in blocks; wavefront 0 solves each block's unit-lower diagonal block with
 // a lane-broadcast chain, the other wavefronts apply the block to the next k
 // rows, and the pending values live in an LDS ring.  Three variants:
-//   MODE 2 (k <= 192 + BS, the default): multipliers and right-hand side
+//   MODE 2 (k <= 384 + BS while the staging fits, the default; above
+//          192 + BS each far-row lane takes two rows): multipliers and
+//          right-hand side
 //          staged HBM -> LDS by producer wavefronts several blocks ahead;
 //   MODE 1 (same k range): multipliers prefetched into registers one block
 //          ahead;
@@ -29,6 +31,8 @@
 #include "sap_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
+#include <unordered_map>
 
 namespace sap_amd {
 
@@ -299,7 +303,8 @@ __device__ __forceinline__ void lower_sweep_fast(const BandView<T>& V, int64_t l
 constexpr int SW_PROD = 3;           // producer wavefronts
 constexpr int SW_STREAMER = 4;
 constexpr int SW_PROD0 = 5;
-constexpr int SW_NI_MAX = 10;        // load instructions per producer per block (k <= 192 + BS)
+constexpr int SW_NI_MAX = 12;        // load instructions per producer per block (k <= 284 FP32, 335 FP64)
+constexpr int SW_FAR = 192;          // far rows of waves 1-3 per pass (one row per lane)
 constexpr int SW_THREADS = 512;
 constexpr int SW_DIN = 8;            // right-hand side steps in flight
 constexpr int SW_NIN = SW_DIN + 1;   // staging slots of 64 dwords
@@ -455,6 +460,8 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
         }
     }
     const unsigned mf0 = ~bits_below(2 * BS + ((int)threadIdx.x - 64) - k);
+    // k - BS > 192: each lane of waves 1-3 takes a second far row, t + 192
+    const unsigned mf1 = ~bits_below(2 * BS + SW_FAR + ((int)threadIdx.x - 64) - k);
     // slot = b % nbuf, kept as a 32-bit counter (a 64-bit modulo per step
     // costs more than the step's arithmetic)
     int slot = 0;
@@ -532,6 +539,20 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
                 const T acc = s0_ + s1_;
                 if (t < BS) acc2[(1 - par) * BS + t] = fv ? acc : (T)0;
                 else if (fv) pend[r & ring_mask] -= acc;
+                if (nfar > SW_FAR) {   // wave-uniform
+                    const int t2 = t + SW_FAR;
+                    const int64_t r2 = cp + 2 * BS + t2;
+                    const int dt2 = V.dir * (2 * BS + t2);
+                    T u0 = 0, u1 = 0;
+#pragma unroll
+                    for (int j = 0; j < BS; j += 2) {
+                        const int colb0 = j * G.segl + ((lo0 + j * dwm) & (E - 1)) + kadj - V.dir * j;
+                        const int colb1 = (j + 1) * G.segl + ((lo0 + (j + 1) * dwm) & (E - 1)) + kadj - V.dir * (j + 1);
+                        u0 += keep_if(buf[colb0 + dt2], mf1, j) * xp[j];
+                        u1 += keep_if(buf[colb1 + dt2], mf1, j + 1) * xp[j + 1];
+                    }
+                    if (t2 < nfar && r2 < len) pend[r2 & ring_mask] -= u0 + u1;
+                }
             } else if (t < BS) {
                 acc2[BS + t] = (T)0;
             }
@@ -567,6 +588,23 @@ __device__ inline T* lds_ring()
     return reinterpret_cast<T*>(smem_raw);
 }
 
+// Several right-hand sides: workgroup -> (job, right-hand side) for nj jobs
+// and nr right-hand sides.  The nr workgroups of one job get ids that are
+// equal mod 8, and workgroup ids are dealt to the 8 XCDs round-robin, so the
+// sweeps of one partition for all right-hand sides run on one XCD and read
+// its factors through that XCD's L2.  With nr = 1 the job is blockIdx.x.
+// Grid: ceil(nj / 8) * 8 * nr workgroups; false for the padding ones.
+__device__ inline bool job_and_rhs(int nj, int nr, int& job, int& r)
+{
+    const int b = blockIdx.x;
+    const int q = b >> 3;
+    r = q % nr;
+    job = (q / nr) * 8 + (b & 7);
+    return job < nj;
+}
+
+inline unsigned job_rhs_grid(int nj, int nr) { return (unsigned)(((nj + 7) / 8) * 8 * nr); }
+
 // ---------------------------------------------------------------------------
 // pass 1 (LU_UL): blockIdx.x = 2*i + side, i = boundary.
 //   side 0: partition i, LU factors (B), bottom k rows of A_i^{-1} v_i
@@ -579,14 +617,22 @@ __device__ inline T* lds_ring()
 template <class T, int MODE>
 __global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_interface_kernel(
     const T* __restrict__ B, const T* __restrict__ Bul, int k, PartInfo pi, const double* __restrict__ v,
-    T* __restrict__ g, T* __restrict__ scr0, T* __restrict__ scr1, int ring_mask, int nbuf)
+    T* __restrict__ g, T* __restrict__ scr0, T* __restrict__ scr1, int ring_mask, int nbuf, int nr, int64_t ldv,
+    const int* __restrict__ gate)
 {
     T* pend = lds_ring<T>();
     __shared__ T xblk[NBS];
     __shared__ T xblk2[2 * NBS];
     __shared__ T acc2[2 * NBS];
-    const int i = blockIdx.x >> 1;
-    const int side = blockIdx.x & 1;
+    int job, rh;
+    if (!job_and_rhs(2 * (pi.P - 1), nr, job, rh)) return;
+    if (gate && !gate[rh]) return;
+    v += rh * ldv;
+    g += rh * pi.n;
+    scr0 += rh * pi.n;
+    scr1 += rh * pi.n;
+    const int i = job >> 1;
+    const int side = job & 1;
     const int p = i + side;
     int64_t s, len;
     pi.bounds(p, s, len);
@@ -634,13 +680,20 @@ template <class T, int MODE>
 __global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_full_kernel(
     const T* __restrict__ B, const T* __restrict__ Bul, int k, PartInfo pi, int last_is_ul,
     const double* __restrict__ v, const T* __restrict__ corr, double* __restrict__ outd, T* __restrict__ outt,
-    T* __restrict__ scr, int ring_mask, int nbuf)
+    T* __restrict__ scr, int ring_mask, int nbuf, int nr, int64_t ldv, int64_t ldo, const int* __restrict__ gate)
 {
     T* pend = lds_ring<T>();
     __shared__ T xblk[NBS];
     __shared__ T xblk2[2 * NBS];
     __shared__ T acc2[2 * NBS];
-    const int p = blockIdx.x;
+    int p, rh;
+    if (!job_and_rhs(pi.P, nr, p, rh)) return;
+    if (gate && !gate[rh]) return;
+    v += rh * ldv;
+    if (corr) corr += rh * (int64_t)2 * k * (pi.P - 1);
+    if (outd) outd += rh * ldo;
+    if (outt) outt += rh * pi.n;
+    scr += rh * pi.n;
     int64_t s, len;
     pi.bounds(p, s, len);
     const bool ul = last_is_ul && p == pi.P - 1;
@@ -755,7 +808,8 @@ __device__ void gemv_cm4(const T* __restrict__ M, int k, const T* x, T* y, T bet
 template <class T>
 __global__ __launch_bounds__(256) void interface_kernel(const T* __restrict__ WV, const T* __restrict__ Sinv,
                                                         const T* __restrict__ offB, const T* __restrict__ offC, int k,
-                                                        PartInfo pi, const T* __restrict__ g, T* __restrict__ corr)
+                                                        PartInfo pi, const T* __restrict__ g, T* __restrict__ corr,
+                                                        const int* __restrict__ gate)
 {
     extern __shared__ unsigned char smem_raw[];
     T* gb = reinterpret_cast<T*>(smem_raw);
@@ -764,6 +818,9 @@ __global__ __launch_bounds__(256) void interface_kernel(const T* __restrict__ WV
     T* xt = zt + k;
     T* xb = xt + k;
     const int i = blockIdx.x;
+    if (gate && !gate[blockIdx.y]) return;
+    g += blockIdx.y * pi.n;
+    corr += blockIdx.y * (int64_t)2 * k * (pi.P - 1);
     int64_t s1, l1;
     pi.bounds(i + 1, s1, l1);
     const size_t kk = (size_t)k * k;
@@ -788,7 +845,7 @@ template <class T>
 __global__ __launch_bounds__(1024) void interface4_kernel(const T* __restrict__ WV, const T* __restrict__ Sinv,
                                                           const T* __restrict__ offB, const T* __restrict__ offC,
                                                           int k, PartInfo pi, const T* __restrict__ g,
-                                                          T* __restrict__ corr)
+                                                          T* __restrict__ corr, const int* __restrict__ gate)
 {
     extern __shared__ unsigned char smem_raw[];
     T* part = reinterpret_cast<T*>(smem_raw);
@@ -798,6 +855,9 @@ __global__ __launch_bounds__(1024) void interface4_kernel(const T* __restrict__ 
     T* xt = zt + k;
     T* xb = xt + k;
     const int i = blockIdx.x;
+    if (gate && !gate[blockIdx.y]) return;
+    g += blockIdx.y * pi.n;
+    corr += blockIdx.y * (int64_t)2 * k * (pi.P - 1);
     int64_t s1, l1;
     pi.bounds(i + 1, s1, l1);
     const size_t kk = (size_t)k * k;
@@ -816,8 +876,11 @@ __global__ __launch_bounds__(1024) void interface4_kernel(const T* __restrict__ 
 
 template <class T>
 __global__ void diag_divide_kernel(const T* __restrict__ B, int64_t n, const double* __restrict__ v,
-                                   double* __restrict__ out)
+                                   double* __restrict__ out, int64_t ldv, int64_t ldo, const int* __restrict__ gate)
 {
+    if (gate && !gate[blockIdx.y]) return;
+    v += blockIdx.y * ldv;
+    out += blockIdx.y * ldo;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         out[i] = (double)((T)v[i] / B[i]);
@@ -837,104 +900,135 @@ bool sweep_fast(int k)
     return k <= 192 + sweep_bs<T>();
 }
 
+// the LDS-staged sweep with two far rows per lane (MODE 2 only)
+template <class T>
+bool sweep_staged_wide(int k)
+{
+    return k > 192 + sweep_bs<T>() && k <= 2 * SW_FAR + sweep_bs<T>();
+}
+
 // Sweep configuration: mode and (mode 2) number of LDS block buffers, as
 // many as fit next to the ring (at most 8) within the 63 outstanding loads a
 // producer wavefront can track.
 struct SweepCfg { int mode, nbuf; size_t lds; int threads; };
 
 template <class T>
-SweepCfg sweep_cfg(int k)
+SweepCfg sweep_cfg(int k, int nbuf_req = 0)
 {
     constexpr size_t LDS_MAX = 160 * 1024 - 4096;   // static LDS of the kernels + margin
     const size_t ring = sweep_lds<T>(k);
-    if (!sweep_fast<T>(k)) return {0, 0, ring, 256};
+    const bool wide = sweep_staged_wide<T>(k);
+    if (!sweep_fast<T>(k) && !wide) return {0, 0, ring, 256};
     const SweepChunks<T> G(k, sweep_bs<T>());
     const size_t bb = G.buf_elems() * sizeof(T);
     const size_t stage = SW_NIN * 64 * 4;
     const size_t guard = 256 * sizeof(T);   // masked reads past the last buffer stay inside the allocation
     int nbuf = (int)std::min<size_t>(8, (LDS_MAX - ring - stage - guard) / bb);
     nbuf = std::min(nbuf, 63 / G.ni + 3);
-    if (nbuf < 4 || G.ni > SW_NI_MAX) return {1, 0, ring, 256};
+    if (nbuf < 4 || G.ni > SW_NI_MAX) return {wide ? 0 : 1, 0, ring, 256};
+    if (nbuf_req >= 4 && nbuf_req < nbuf) nbuf = nbuf_req;
     return {2, nbuf, ring + stage + nbuf * bb + guard, SW_THREADS};
+}
+
+// Raises a kernel's dynamic LDS limit once per size (not a stream
+// operation, but kept out of the launch path so that launches can be
+// captured into a graph with nothing else in between).
+void set_lds_limit(const void* kernel, size_t lds)
+{
+    static std::mutex mu;
+    static std::unordered_map<const void*, size_t> done;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& have = done[kernel];
+    if (lds <= have) return;
+    SAP_HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    have = lds;
 }
 
 template <class T, int MODE>
 void launch_sweep_interface_t(const T* B, const T* Bul, int k, PartInfo pi, const double* v, T* g, T* scr0, T* scr1,
-                              const SweepCfg& c, hipStream_t st)
+                              const SweepCfg& c, const Rhs& rh, hipStream_t st)
 {
-    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)sweep_interface_kernel<T, MODE>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds));
-    hipLaunchKernelGGL((sweep_interface_kernel<T, MODE>), dim3(2 * (pi.P - 1)), dim3(c.threads), c.lds, st, B, Bul,
-                       k, pi, v, g, scr0, scr1, ring_size(k) - 1, c.nbuf);
+    set_lds_limit((const void*)sweep_interface_kernel<T, MODE>, c.lds);
+    hipLaunchKernelGGL((sweep_interface_kernel<T, MODE>), dim3(job_rhs_grid(2 * (pi.P - 1), rh.nr)), dim3(c.threads),
+                       c.lds, st, B, Bul, k, pi, v, g, scr0, scr1, ring_size(k) - 1, c.nbuf, rh.nr, rh.ldv, rh.gate);
     SAP_LAUNCH_CHECK();
 }
 
 template <class T>
 void launch_sweep_interface(const T* B, const T* Bul, int k, PartInfo pi, const double* v, T* g, T* scr0, T* scr1,
-                            hipStream_t st)
+                            hipStream_t st, const Rhs& rh)
 {
     if (pi.P < 2) return;
-    const SweepCfg c = sweep_cfg<T>(k);
-    if (c.mode == 2)      launch_sweep_interface_t<T, 2>(B, Bul, k, pi, v, g, scr0, scr1, c, st);
-    else if (c.mode == 1) launch_sweep_interface_t<T, 1>(B, Bul, k, pi, v, g, scr0, scr1, c, st);
-    else                  launch_sweep_interface_t<T, 0>(B, Bul, k, pi, v, g, scr0, scr1, c, st);
+    const SweepCfg c = sweep_cfg<T>(k, rh.nbuf);
+    if (c.mode == 2)      launch_sweep_interface_t<T, 2>(B, Bul, k, pi, v, g, scr0, scr1, c, rh, st);
+    else if (c.mode == 1) launch_sweep_interface_t<T, 1>(B, Bul, k, pi, v, g, scr0, scr1, c, rh, st);
+    else                  launch_sweep_interface_t<T, 0>(B, Bul, k, pi, v, g, scr0, scr1, c, rh, st);
 }
 
 template <class T>
 void launch_interface_solve(const T* WV, const T* Sinv, const T* offB, const T* offC, int k, PartInfo pi, const T* g,
-                            T* corr, hipStream_t st)
+                            T* corr, hipStream_t st, const Rhs& rh)
 {
     if (pi.P < 2) return;
     const size_t lds4 = sizeof(T) * (1024 + 5 * (size_t)k);
     if (lds4 <= 64 * 1024) {
-        hipLaunchKernelGGL(interface4_kernel<T>, dim3(pi.P - 1), dim3(1024), lds4, st, WV, Sinv, offB, offC, k, pi, g,
-                           corr);
+        hipLaunchKernelGGL(interface4_kernel<T>, dim3(pi.P - 1, rh.nr), dim3(1024), lds4, st, WV, Sinv, offB, offC, k,
+                           pi, g, corr, rh.gate);
     } else {
         const size_t lds = sizeof(T) * 5 * (size_t)k;
-        hipLaunchKernelGGL(interface_kernel<T>, dim3(pi.P - 1), dim3(256), lds, st, WV, Sinv, offB, offC, k, pi, g,
-                           corr);
+        hipLaunchKernelGGL(interface_kernel<T>, dim3(pi.P - 1, rh.nr), dim3(256), lds, st, WV, Sinv, offB, offC, k,
+                           pi, g, corr, rh.gate);
     }
     SAP_LAUNCH_CHECK();
 }
 
 template <class T, int MODE>
 void launch_sweep_full_t(const T* B, const T* Bul, int k, PartInfo pi, int last_is_ul, const double* v, const T* corr,
-                         double* outd, T* outt, T* scr, const SweepCfg& c, hipStream_t st)
+                         double* outd, T* outt, T* scr, const SweepCfg& c, const Rhs& rh, hipStream_t st)
 {
-    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)sweep_full_kernel<T, MODE>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)c.lds));
-    hipLaunchKernelGGL((sweep_full_kernel<T, MODE>), dim3(pi.P), dim3(c.threads), c.lds, st, B, Bul, k, pi,
-                       last_is_ul, v, corr, outd, outt, scr, ring_size(k) - 1, c.nbuf);
+    set_lds_limit((const void*)sweep_full_kernel<T, MODE>, c.lds);
+    hipLaunchKernelGGL((sweep_full_kernel<T, MODE>), dim3(job_rhs_grid(pi.P, rh.nr)), dim3(c.threads), c.lds, st, B,
+                       Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, ring_size(k) - 1, c.nbuf, rh.nr, rh.ldv,
+                       rh.ldo, rh.gate);
     SAP_LAUNCH_CHECK();
 }
 
 template <class T>
 void launch_sweep_full(const T* B, const T* Bul, int k, PartInfo pi, int last_is_ul, const double* v, const T* corr,
-                       double* outd, T* outt, T* scr, hipStream_t st)
+                       double* outd, T* outt, T* scr, hipStream_t st, const Rhs& rh)
 {
-    const SweepCfg c = sweep_cfg<T>(k);
-    if (c.mode == 2)      launch_sweep_full_t<T, 2>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, c, st);
-    else if (c.mode == 1) launch_sweep_full_t<T, 1>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, c, st);
-    else                  launch_sweep_full_t<T, 0>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, c, st);
+    const SweepCfg c = sweep_cfg<T>(k, rh.nbuf);
+    if (c.mode == 2)      launch_sweep_full_t<T, 2>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, c, rh, st);
+    else if (c.mode == 1) launch_sweep_full_t<T, 1>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, c, rh, st);
+    else                  launch_sweep_full_t<T, 0>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, c, rh, st);
 }
 
 template <class T>
-void launch_diag_divide(const T* B, int64_t n, const double* v, double* out, hipStream_t st)
+void launch_diag_divide(const T* B, int64_t n, const double* v, double* out, hipStream_t st, const Rhs& rh)
 {
     int64_t g = (n + 255) / 256;
     if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(diag_divide_kernel<T>, dim3((unsigned)g), dim3(256), 0, st, B, n, v, out);
+    hipLaunchKernelGGL(diag_divide_kernel<T>, dim3((unsigned)g, rh.nr), dim3(256), 0, st, B, n, v, out, rh.ldv,
+                       rh.ldo, rh.gate);
     SAP_LAUNCH_CHECK();
 }
 
+template <class T>
+int sweep_max_nbuf(int k)
+{
+    const SweepCfg c = sweep_cfg<T>(k);
+    return c.mode == 2 ? c.nbuf : 0;
+}
+
 #define SAP_INST(T)                                                                                               \
+    template int sweep_max_nbuf<T>(int);                                                                           \
     template void launch_sweep_interface<T>(const T*, const T*, int, PartInfo, const double*, T*, T*, T*,          \
-                                            hipStream_t);                                                          \
+                                            hipStream_t, const Rhs&);                                              \
     template void launch_interface_solve<T>(const T*, const T*, const T*, const T*, int, PartInfo, const T*, T*,   \
-                                            hipStream_t);                                                          \
+                                            hipStream_t, const Rhs&);                                              \
     template void launch_sweep_full<T>(const T*, const T*, int, PartInfo, int, const double*, const T*, double*,   \
-                                       T*, T*, hipStream_t);                                                       \
-    template void launch_diag_divide<T>(const T*, int64_t, const double*, double*, hipStream_t);
+                                       T*, T*, hipStream_t, const Rhs&);                                           \
+    template void launch_diag_divide<T>(const T*, int64_t, const double*, double*, hipStream_t, const Rhs&);
 SAP_INST(float)
 SAP_INST(double)
 #undef SAP_INST

@@ -1,7 +1,8 @@
 """The solver's interface and monitor paths on the GPU, through the C ABI.
 
-  * Several right-hand sides: an (n, nrhs) b solves each column as the
-    one-column solve does (bit for bit: same kernels, same order).
+  * Several right-hand sides one after another (batch=0): an (n, nrhs) b
+    solves each column as the one-column solve does (bit for bit: same
+    kernels, same order).
   * Length checks: vectors of the wrong length are rejected before any
     device access.
   * Monitor codes against the oracle (sap/monitor.h:153-335,
@@ -51,7 +52,9 @@ def test_several_right_hand_sides_match_single_solves(sap, oracle):
     band = oracle.gen_banded(n, k, 0.9, 201)
     B = np.stack([oracle.gen_vector(n, 202), oracle.gen_vector(n, 203), oracle.gen_vector(n, 204)], axis=1)
     A = sap.BandedMatrix(dev(band))
-    g = sap.Solver(P, sap.Options(factMethod=sap.LU_UL, relTol=1e-10, refine=6))
+    # batch=0: the columns one after another (the batched solve is covered
+    # by test_gpu_batch.py)
+    g = sap.Solver(P, sap.Options(factMethod=sap.LU_UL, relTol=1e-10, refine=6, batch=0))
     g.setup(A)
     X = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
     assert g.solve(None, dev(B), X)

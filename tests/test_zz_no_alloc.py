@@ -60,7 +60,9 @@ def test_noalloc_several_right_hand_sides(sap):
     B = torch.stack([sap.gen_vector(n, 40 + c) for c in range(3)], dim=1)
     xs = []
     for noalloc in (True, False):
-        s = sap.Solver(4, sap.Options(precondType=sap.Block, relTol=1e-10, refine=4))
+        # batch=0: sap_solve takes the columns one after another, as the
+        # allocation-free solve does
+        s = sap.Solver(4, sap.Options(precondType=sap.Block, relTol=1e-10, refine=4, batch=0))
         s.setup(A)
         X = torch.zeros_like(B)
         assert (s.solve_noalloc if noalloc else s.solve)(None, B, X)
