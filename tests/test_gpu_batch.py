@@ -193,3 +193,32 @@ def test_system_batch_equals_single_systems_bitwise(sap, oracle):
         Xs, its, codes = one_by_one(sap, mats[i], opts, P, Bs[i], np.zeros_like(Bs[i]))
         np.testing.assert_array_equal(host(Xd[i]), Xs)
         assert [r[1] for r in res[3 * i:3 * i + 3]] == its
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_autotuned_launches_give_the_same_bits(sap, oracle, prec):
+    """options.autotune = 1 times the conversion tile widths and the sweep
+    staging depths on the first setup of a half-bandwidth and keeps the
+    fastest; neither changes the arithmetic, so factors, M^{-1} v and the
+    solution equal the untuned run bit for bit, and the choices are listed."""
+    n, k, P = 20000, 100 + prec, 8
+    band = oracle.gen_banded(n, k, 0.8, 501)
+    A = sap.BandedMatrix(dev(band))
+    b = dev(oracle.gen_vector(n, 502))
+    out = []
+    for tune in (0, 1):
+        g = sap.Solver(P, sap.Options(factMethod=sap.LU_UL, relTol=1e-10, refine=6, autotune=tune,
+                                      precValueType=prec))
+        g.setup(A)
+        z = torch.empty(n, dtype=torch.float64, device="cuda")
+        g.precond_apply(b, z)
+        x = torch.zeros_like(b)
+        assert g.solve(None, b, x)
+        out.append((g.factors(), host(z), host(x)))
+    for a, c in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, c)
+    ent = [e for e in sap.autotune_entries() if e["k"] == k and e["elem_bytes"] == (8 if prec else 4)]
+    fams = {e["family"] for e in ent}
+    assert 0 in fams and 1 in fams
+    for e in ent:
+        assert e["candidates"] >= 2 and e["best_ms"] > 0 and e["best_ms"] <= e["default_ms"] + 1e-9
