@@ -115,6 +115,10 @@ typedef struct sap_options {
     int    graph;               /* batched solves: 1 (default) replays each
                                    BiCGStab(2) iteration from a captured HIP graph
                                    (off while profiling)                            */
+    int    fuse;                /* sap_batch_solve: 1 (default) runs the sweeps of all
+                                   systems as one launch when every system's
+                                   preconditioner is one sweep pass (Block, or one
+                                   partition); 0 one launch per system             */
     int    autotune;            /* 1: on the first setup of each half-bandwidth and
                                    precision, time the candidate conversion tile
                                    widths and sweep staging depths and keep the
@@ -204,8 +208,8 @@ int         sap_setup_banded(sap_solver* s, int n, int k, const double* d_band);
  * LU_only with every pivot boosted, on the constant-width band of the
  * reordered matrix.
  * Returns SAP_ERR_SINGULAR when DB finds no perfect matching,
- * SAP_ERR_NEG_DB_WEIGHT on a negative reduced cost, SAP_ERR_UNSUPPORTED for
- * testDB. */
+ * SAP_ERR_NEG_DB_WEIGHT on a negative reduced cost.  With testDB only the
+ * DB test runs (see sap_reordering_dominance). */
 int         sap_setup_csr(sap_solver* s, int n, int nnz, const int* d_row_offsets,
                           const int* d_col_indices, const double* d_values);
 
@@ -315,6 +319,16 @@ int         sap_reordering_get(const sap_reordering* r, int* db_perm, int* row_p
                                double* row_scale, double* col_scale, int* row_offsets,
                                int* col_indices, double* values);
 void        sap_reordering_free(sap_reordering* r);
+/* With the options' testDB (Graph::reorder's DB test, sap/graph.h:468-490,
+ * :537-561): the reordering stops after DB (no RCM, no drop-off; k is the
+ * bandwidth after DB) and these are the diagonal-dominance statistics of
+ * d_i = |a_ii| / sum_{j != i} |a_ij|: the minimum and the 1st percentile
+ * (entry n/100 of the sorted d) before and after DB (0 without testDB).
+ * A sap_setup_csr with testDB runs the same test, fills these Stats fields
+ * and builds no preconditioner (a following sap_solve returns
+ * SAP_ERR_ILLEGAL_SOLVE). */
+int         sap_reordering_dominance(const sap_reordering* r, double* diag_dom_ori, double* d_p1_ori,
+                                     double* diag_dom, double* d_p1);
 
 /* The variable-bandwidth assembly of the reordered matrix into
  * numPartitions partitions (clamped to max(n/(k+1), 1) as
@@ -365,6 +379,31 @@ typedef struct sap_tune_entry {
     double default_ms;
 } sap_tune_entry;
 int         sap_autotune_entries(sap_tune_entry* out, int max_entries);
+
+/* The learned option selector's network (no counterpart in the
+ * reference): a fully connected FP32 network n_in -> n_hidden -> n_hidden
+ * -> n_hidden -> n_out with ReLU, trained on the GPU by hand-written
+ * kernels (forward, backward of the masked mean squared error, Adam).
+ * Parameters are packed per layer (weights n_in_l x n_out_l row-major, then
+ * biases); d_* are device float arrays (X: batch x n_in, Y/T/M: batch x
+ * n_out, M = 1 where a target is known, 0 elsewhere).
+ *   sap_mlp_create: He-initialised weights from seed (NULL on bad sizes).
+ *   sap_mlp_gradient: the gradient of the loss with respect to every
+ *     parameter (host copy) and the loss, without updating.
+ *   sap_mlp_train: epochs full-batch Adam steps at learning rate lr; the
+ *     loss before each step into h_losses (may be NULL). */
+typedef struct sap_mlp sap_mlp;
+sap_mlp*    sap_mlp_create(int n_in, int n_hidden, int n_out, uint64_t seed);
+void        sap_mlp_destroy(sap_mlp* m);
+int64_t     sap_mlp_num_params(const sap_mlp* m);
+int         sap_mlp_get_params(const sap_mlp* m, float* h_out);
+int         sap_mlp_set_params(sap_mlp* m, const float* h_in);
+int         sap_mlp_forward(sap_mlp* m, const float* d_X, int batch, float* d_Y);
+int         sap_mlp_gradient(sap_mlp* m, const float* d_X, const float* d_T, const float* d_M, int batch,
+                             float* h_grad, float* h_loss);
+int         sap_mlp_train(sap_mlp* m, const float* d_X, const float* d_T, const float* d_M, int batch, int epochs,
+                          float lr, float* h_losses);
+const char* sap_mlp_last_error(const sap_mlp* m);
 
 /* Kernel-family timing since the last sap_reset_profile (profile = 1). */
 int         sap_get_profile(const sap_solver* s, sap_kernel_profile* out, int max_entries);

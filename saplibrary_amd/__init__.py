@@ -84,6 +84,7 @@ class Options(ctypes.Structure):
         ("profile", ctypes.c_int),
         ("batch", ctypes.c_int),
         ("graph", ctypes.c_int),
+        ("fuse", ctypes.c_int),
         ("autotune", ctypes.c_int),
     ]
 
@@ -142,7 +143,8 @@ EXPORTS = (
     "sap_reordering_assemble_var", "sap_reordering_var_get", "sap_mm_read", "sap_mm_info", "sap_mm_get_csr",
     "sap_mm_free", "sap_solve_noalloc", "sap_batch_create", "sap_batch_destroy", "sap_batch_set_stream",
     "sap_batch_setup_banded", "sap_batch_solve", "sap_batch_result", "sap_batch_times", "sap_batch_last_error",
-    "sap_autotune_entries",
+    "sap_autotune_entries", "sap_reordering_dominance", "sap_mlp_create", "sap_mlp_destroy", "sap_mlp_num_params", "sap_mlp_get_params",
+    "sap_mlp_set_params", "sap_mlp_forward", "sap_mlp_gradient", "sap_mlp_train", "sap_mlp_last_error",
 )
 
 class TuneEntry(ctypes.Structure):
@@ -208,6 +210,7 @@ def lib():
     L.sap_reorder_csr.restype = vp
     L.sap_reordering_info.argtypes = [vp, ip, ip, ip, dp, ip]
     L.sap_reordering_get.argtypes = [vp, ip, ip, ip, dp, dp, ip, ip, dp]
+    L.sap_reordering_dominance.argtypes = [vp, dp, dp, dp, dp]
     L.sap_reordering_free.argtypes = [vp]
     L.sap_reordering_free.restype = None
     L.sap_reordering_assemble_var.argtypes = [vp, i]
@@ -229,6 +232,20 @@ def lib():
     L.sap_batch_result.argtypes = [vp, i, ip, ctypes.POINTER(ctypes.c_float), dp]
     L.sap_batch_times.argtypes = [vp, dp, dp]
     L.sap_autotune_entries.argtypes = [ctypes.POINTER(TuneEntry), i]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.sap_mlp_create.argtypes = [i, i, i, ctypes.c_uint64]
+    L.sap_mlp_create.restype = vp
+    L.sap_mlp_destroy.argtypes = [vp]
+    L.sap_mlp_destroy.restype = None
+    L.sap_mlp_num_params.argtypes = [vp]
+    L.sap_mlp_num_params.restype = i64
+    L.sap_mlp_get_params.argtypes = [vp, fp]
+    L.sap_mlp_set_params.argtypes = [vp, fp]
+    L.sap_mlp_forward.argtypes = [vp, vp, i, vp]
+    L.sap_mlp_gradient.argtypes = [vp, vp, vp, vp, i, fp, fp]
+    L.sap_mlp_train.argtypes = [vp, vp, vp, vp, i, i, ctypes.c_float, fp]
+    L.sap_mlp_last_error.argtypes = [vp]
+    L.sap_mlp_last_error.restype = ctypes.c_char_p
     L.sap_batch_last_error.argtypes = [vp]
     L.sap_batch_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -407,6 +424,9 @@ def reorder_csr(row_offsets, column_indices, values, opts=None, numPartitions=No
                                     out["val"].ctypes.data_as(dp)))
         out["ci"] = out["ci"][:nnz.value]
         out["val"] = out["val"][:nnz.value]
+        dd = [ctypes.c_double() for _ in range(4)]
+        _check(L.sap_reordering_dominance(r, *(ctypes.byref(x) for x in dd)))
+        out["diag_dom_ori"], out["d_p1_ori"], out["diag_dom"], out["d_p1"] = (x.value for x in dd)
         out.update(k=k.value, k_db=kdb.value, k_reorder=kre.value, dropoff=drop.value)
         if numPartitions is not None:
             P = L.sap_reordering_assemble_var(r, numPartitions)
@@ -596,6 +616,72 @@ class Solver:
         _check(lib().sap_get_spikes(self.h, wv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                     s.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), wv.size), self.h)
         return wv, s
+
+
+class Mlp:
+    """The option selector's network (sap_mlp_* of the C ABI): in -> hidden
+    -> hidden -> hidden -> out, ReLU, FP32, trained by the library's own
+    HIP kernels.  Arrays are float32 device tensors."""
+
+    def __init__(self, n_in, n_out, n_hidden=64, seed=0):
+        h = lib().sap_mlp_create(n_in, n_hidden, n_out, seed)
+        if not h:
+            raise SapError(-8, "sap_mlp_create: bad sizes")
+        self.h = ctypes.c_void_p(h)
+        self.n_in, self.n_out, self.n_hidden = n_in, n_out, n_hidden
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.sap_mlp_destroy(h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc < 0:
+            raise SapError(rc, lib().sap_mlp_last_error(self.h).decode())
+        return rc
+
+    @staticmethod
+    def _f32(t, name):
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+            raise TypeError(f"{name} must be a contiguous float32 device tensor")
+        return ctypes.c_void_p(t.data_ptr())
+
+    def num_params(self):
+        return lib().sap_mlp_num_params(self.h)
+
+    def params(self):
+        import numpy as np
+        out = np.empty(self.num_params(), dtype=np.float32)
+        self._check(lib().sap_mlp_get_params(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params()
+        self._check(lib().sap_mlp_set_params(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def forward(self, X):
+        Y = torch.empty((X.shape[0], self.n_out), dtype=torch.float32, device=X.device)
+        self._check(lib().sap_mlp_forward(self.h, self._f32(X, "X"), X.shape[0], self._f32(Y, "Y")))
+        return Y
+
+    def gradient(self, X, T, M):
+        import numpy as np
+        g = np.empty(self.num_params(), dtype=np.float32)
+        loss = ctypes.c_float()
+        self._check(lib().sap_mlp_gradient(self.h, self._f32(X, "X"), self._f32(T, "T"), self._f32(M, "M"),
+                                           X.shape[0], g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                           ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, X, T, M, epochs, lr=1e-3):
+        import numpy as np
+        losses = np.empty(max(epochs, 1), dtype=np.float32)
+        self._check(lib().sap_mlp_train(self.h, self._f32(X, "X"), self._f32(T, "T"), self._f32(M, "M"), X.shape[0],
+                                        epochs, lr, losses.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return losses[:epochs]
 
 
 class SystemBatch:

@@ -423,12 +423,12 @@ void launch_band_spmv(int64_t n, int k, const double* band, const double* x, dou
     SAP_LAUNCH_CHECK();
 }
 
-void launch_band_spmm(int64_t n, int k, const double* band, const double* X, int64_t ldx, double* Y, int64_t ldy,
-                      int nr, const int* gate, hipStream_t st)
+template <int RB>
+void launch_band_spmm_rb(int64_t n, int k, const double* band, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                         int nr, const int* gate, hipStream_t st)
 {
     const int w = 2 * k + 1;
     const int bs = 256;
-    constexpr int RB = 4;
     const unsigned chunks = (unsigned)((nr + RB - 1) / RB);
     if (w >= 192) {
         const int64_t blocks = (n * 64 + bs - 1) / bs;
@@ -444,6 +444,15 @@ void launch_band_spmm(int64_t n, int k, const double* band, const double* X, int
                            X, ldx, Y, ldy, nr, gate);
     }
     SAP_LAUNCH_CHECK();
+}
+
+void launch_band_spmm(int64_t n, int k, const double* band, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                      int nr, const int* gate, hipStream_t st)
+{
+    // columns per workgroup row: the band is read once per RB columns
+    if (nr > 8) launch_band_spmm_rb<16>(n, k, band, X, ldx, Y, ldy, nr, gate, st);
+    else if (nr > 4) launch_band_spmm_rb<8>(n, k, band, X, ldx, Y, ldy, nr, gate, st);
+    else launch_band_spmm_rb<4>(n, k, band, X, ldx, Y, ldy, nr, gate, st);
 }
 
 void launch_csr_spmm(int64_t n, int64_t nnz, const int* rp, const int* ci, const double* val, const double* X,

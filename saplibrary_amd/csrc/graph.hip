@@ -453,6 +453,28 @@ int rcm(int n, const std::vector<int>& rp, const std::vector<int>& ci, std::vect
     return bandwidth;
 }
 
+// The diagonal-dominance statistics of the testDB mode (Graph::reorder,
+// sap/graph.h:468-490 before DB and :537-561 after): per row
+// d_i = |a_ii| / sum_{j != i} |a_ij| (in double, from the values in T),
+// sorted; the minimum and the entry at n/100.
+template <class T>
+void dominance_stats(int n, const std::vector<int>& rp, const std::vector<int>& ci, const std::vector<T>& val,
+                     double* dmin, double* dp1)
+{
+    std::vector<double> ds(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        double diag = 0.0, sums = 0.0;
+        for (int j = rp[i]; j < rp[i + 1]; ++j) {
+            if (ci[j] == i) diag = std::fabs(val[j]);
+            else sums += std::fabs(val[j]);
+        }
+        ds[i] = diag / sums;
+    }
+    std::sort(ds.begin(), ds.end());
+    *dmin = ds[0];
+    *dp1 = ds[n / 100];
+}
+
 }  // namespace
 
 template <class T>
@@ -485,6 +507,7 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
         return std::chrono::duration<double, std::milli>(clock::now() - a).count();
     };
     const auto t_reorder = clock::now();
+    if (o.testDB) dominance_stats(n, rp, ci, val, &R.diag_dom_ori, &R.d_p1_ori);
 
     // (2) DB (Graph::reorder / Graph::DB, sap/graph.h:448-579, :1216-1361)
     const auto t_db = clock::now();
@@ -543,6 +566,21 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
         for (int i = 0; i < n; ++i)
             for (int j = rp[i]; j < rp[i + 1]; ++j) kdb = std::max(kdb, std::abs(i - ci[j]));
         R.k_db = kdb;
+    }
+    if (o.testDB) {
+        // the DB test stops here and reports the bandwidth after DB
+        // (sap/graph.h:537-561)
+        dominance_stats(n, rp, ci, val, &R.diag_dom, &R.d_p1);
+        R.k_reorder = R.k = R.k_db;
+        R.time_reorder = since(t_reorder);
+        R.row_perm = R.db_perm;
+        R.col_reord.resize(n);
+        for (int i = 0; i < n; ++i) R.col_reord[i] = i;
+        R.rp.swap(rp);
+        R.ci.swap(ci);
+        R.val.swap(val);
+        R.entry_orig.swap(id);
+        return R;
     }
 
     // (3) RCM or identity (sap/graph.h:560-575)

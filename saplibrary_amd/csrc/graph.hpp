@@ -15,6 +15,7 @@ struct GraphOptions {
     bool doRCM = true;             // performReorder && maxBandwidth > 64 (sap/precond.h:1726-1728)
     int maxBandwidth = 0x7fffffff;
     double dropOffFrac = 0;
+    bool testDB = false;           // testDB: dominance statistics, then stop after DB
 };
 
 template <class T>
@@ -38,6 +39,9 @@ struct Reordering {
     std::vector<T> entry_scale;
     // host wall times, milliseconds
     double time_DB = 0, time_reorder = 0, time_dropOff = 0;
+    // testDB (sap/graph.h:468-490, :537-561): d_i = |a_ii| / sum_{j != i}
+    // |a_ij| sorted; its minimum and its entry n/100, before and after DB
+    double diag_dom_ori = 0, d_p1_ori = 0, diag_dom = 0, d_p1 = 0;
 };
 
 // The variable-bandwidth assembly of a reordered matrix into P partitions

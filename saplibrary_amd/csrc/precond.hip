@@ -253,6 +253,7 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
     go.doRCM = rc.maxBandwidth > 64;
     go.maxBandwidth = rc.maxBandwidth;
     go.dropOffFrac = rc.dropOffFraction;
+    go.testDB = rc.testDB;
     Reordering<T> R = reorder_csr<T>((int)n_, h_rp.data(), h_ci.data(), h_val.data(), go);
     time_DB = R.time_DB;
     time_reorder += R.time_reorder;
@@ -260,6 +261,18 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
     k_db = R.k_db;
     k_reorder = R.k_reorder;
     dropoff_actual = R.dropoff_actual;
+    diag_dom_ori = R.diag_dom_ori;
+    d_p1_ori = R.d_p1_ori;
+    diag_dom = R.diag_dom;
+    d_p1 = R.d_p1;
+    tested_db_only = rc.testDB;
+    if (rc.testDB) {
+        // the DB test ends the setup here (sap/precond.h:1755-1756)
+        n = n_;
+        k = R.k;
+        src_ = Source::none;
+        return 0;
+    }
 
     // (3) variable bandwidth (sap/precond.h:1777-1800): with more than one
     //     partition (clamped to n / (k+1)) and k > 0 the reference builds the
@@ -616,6 +629,16 @@ void Precond<T>::reserve_multi(int nr)
         zpm_.resize(m * n);
     }
     nr_cap_ = nr;
+}
+
+template <class T>
+bool Precond<T>::append_sweep_jobs(std::vector<SweepJob>& jobs, size_t& lds, int64_t voff, int64_t ooff, int nr,
+                                   int gate0)
+{
+    if (cfg.precondType == SAP_None || reorder_ || spike_active() || k == 0) return false;
+    reserve_multi(nr);
+    return sweep_jobs_append<T>(jobs, lds, B_.get(), B_.get(), k, PartInfo{n, P}, 0, nullptr, scr0m_.get(), voff, n,
+                                ooff, n, nr, gate0, nbuf_);
 }
 
 template <class T>

@@ -27,6 +27,7 @@ struct ReorderConfig {
     double dropOffFraction = 0;
     bool track = false;            // trackReordering: keep what update() needs
     bool variable = false;         // variableBandwidth (LU_only, every pivot boosted)
+    bool testDB = false;           // testDB: the DB statistics only, no preconditioner
 };
 
 // Setup timings: events recorded on the stream while a setup runs, turned
@@ -121,6 +122,10 @@ public:
     double time_DB = 0, time_reorder = 0, time_dropOff = 0, time_transfer = 0, time_cpu_assemble = 0;
     int k_db = 0, k_reorder = 0;    // stay 0 without reordering, as in the reference
     double dropoff_actual = 0;
+    // testDB statistics (0 unless the last CSR setup ran the DB test, which
+    // builds no preconditioner: tested_db_only)
+    double diag_dom_ori = 0, d_p1_ori = 0, diag_dom = 0, d_p1 = 0;
+    bool tested_db_only = false;
     BandProfile profile;            // ks_row / ks_col sums (empty without reordering)
     int64_t actual_nnz = 0;
 
@@ -147,6 +152,12 @@ public:
     // Allocates apply_multi's work buffers for nr columns (so that a later
     // apply_multi, e.g. under graph capture, allocates nothing).
     void reserve_multi(int nr);
+    // Appends the full-sweep jobs of this preconditioner for nr columns
+    // (input at voff + r*n, output at ooff + r*n, gates gate0 + r) to a
+    // launch that covers several systems; false (nothing appended) unless
+    // the apply is that one sweep pass: Block / SaP-D or one partition,
+    // no reordering, k > 0 and a width the staged sweep takes.
+    bool append_sweep_jobs(std::vector<SweepJob>& jobs, size_t& lds, int64_t voff, int64_t ooff, int nr, int gate0);
     bool reordered() const { return reorder_; }
     // Refactors with new values of the matrix given to the last setup, same
     // pattern (Precond::update, sap/precond.h:751-900): count doubles in

@@ -261,6 +261,30 @@ struct Rhs {
 // the staged sweeps' largest LDS block-buffer count for k (0: not staged)
 template <class T>
 int sweep_max_nbuf(int k);
+// Full sweeps (L, D^{-1}, U' of every partition) of several systems in one
+// launch: one job per (system, partition, right-hand side) in a device
+// table.  The input column of a job is V + voff and its output Z + ooff,
+// V and Z given at launch; gate[job.gate] switches a column off.
+struct SweepJob {
+    const void* B;        // the system's factor band (null: a padding job)
+    const void* Bul;      // its UL band (last partition under LU_UL)
+    const void* corr;     // reduced-system corrections of this column, or null
+    void* scr;            // n-element work vector of this column
+    int64_t voff, ooff;
+    int64_t n;
+    int P, k, part, gate, last_is_ul, ring_mask, nbuf, pad_;
+};
+// Appends the jobs of one system (nr columns at voff + r*ldv / ooff +
+// r*ldo, scr and corr holding nr consecutive blocks, gates gate0 + r) and
+// raises lds to what they need; false when its width does not take the
+// staged sweep (nothing appended).
+template <class T>
+bool sweep_jobs_append(std::vector<SweepJob>& jobs, size_t& lds, const T* B, const T* Bul, int k, PartInfo pi,
+                       int last_is_ul, const T* corr, T* scr, int64_t voff, int64_t ldv, int64_t ooff, int64_t ldo,
+                       int nr, int gate0, int nbuf_req);
+template <class T>
+void launch_sweep_full_jobs(const SweepJob* d_jobs, int njobs, size_t lds, const double* V, double* Z,
+                            const int* gate, hipStream_t st);
 template <class T>
 void launch_sweep_interface(const T* B, const T* Bul, int k, PartInfo pi, const double* v, T* g, T* scr0, T* scr1,
                             hipStream_t st, const Rhs& rh = Rhs());

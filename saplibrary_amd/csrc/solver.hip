@@ -681,6 +681,10 @@ public:
         stats.bandwidthReorder = p.k_reorder;
         stats.bandwidthDB = p.k_db;
         stats.actualDropOff = p.dropoff_actual;
+        stats.d_p1 = p.d_p1;
+        stats.d_p1_ori = p.d_p1_ori;
+        stats.diag_dom = p.diag_dom;
+        stats.diag_dom_ori = p.diag_dom_ori;
         // nuKf = (2 K n - sum ks_row - sum ks_col) / (2 K n) and flops_LU =
         // sum ks_row ks_col / (time_bandLU 1e6); the extents are only
         // assembled by the reordering path (empty otherwise, so nuKf = 1 and
@@ -728,12 +732,14 @@ public:
         rc.dropOffFraction = opt.dropOffFraction;
         rc.track = opt.trackReordering != 0;
         rc.variable = opt.variableBandwidth != 0;
+        rc.testDB = opt.testDB != 0;
         int r = pf ? pf->setup_csr(n, nnz, rp, ci, val, rc) : pd->setup_csr(n, nnz, rp, ci, val, rc);
         SAP_HIP_CHECK(hipStreamSynchronize(stream));
         if (r != 0) return r;
         k = pf ? pf->k : pd->k;
         if (pf) harvest(*pf, t0); else harvest(*pd, t0);
-        setup_done = true;
+        // the DB test builds no preconditioner: a later solve is illegal
+        setup_done = !rc.testDB;
         return 0;
     }
 
@@ -1040,6 +1046,15 @@ public:
         explicit Ops(SystemBatch& b_) : b(b_) {}
         void apply(const double* v, double* z, const int* gate) override
         {
+            if (b.fused_jobs_ > 0) {
+                // every system's sweeps in one launch
+                ProfScope ps(&b.sys[0]->prof, "sweep_full");
+                if (b.opt.precValueType == SAP_PREC_DOUBLE)
+                    launch_sweep_full_jobs<double>(b.fused_.get(), b.fused_jobs_, b.fused_lds_, v, z, gate, b.stream);
+                else
+                    launch_sweep_full_jobs<float>(b.fused_.get(), b.fused_jobs_, b.fused_lds_, v, z, gate, b.stream);
+                return;
+            }
             for (size_t s = 0; s < b.sys.size(); ++s) {
                 Solver& q = *b.sys[s];
                 const int64_t o = b.base_[s];
@@ -1106,6 +1121,28 @@ public:
             if (q.pf) q.pf->reserve_multi(nr_[s]);
             else q.pd->reserve_multi(nr_[s]);
         }
+        // one sweep launch for all systems when every system's apply is a
+        // single full sweep pass (the job table depends on the setup's
+        // buffers, so it is built for every solve)
+        fused_jobs_ = 0;
+        {
+            std::vector<SweepJob> jobs;
+            size_t lds = 0;
+            bool all = opt.fuse != 0;
+            for (int s = 0; all && s < S; ++s) {
+                Solver& q = *sys[s];
+                all = q.pf ? q.pf->append_sweep_jobs(jobs, lds, base_[s], base_[s], nr_[s], col0[s])
+                           : q.pd->append_sweep_jobs(jobs, lds, base_[s], base_[s], nr_[s], col0[s]);
+            }
+            if (all && !jobs.empty()) {
+                fused_.resize(jobs.size());
+                SAP_HIP_CHECK(hipMemcpyAsync(fused_.get(), jobs.data(), sizeof(SweepJob) * jobs.size(),
+                                             hipMemcpyHostToDevice, stream));
+                SAP_HIP_CHECK(hipStreamSynchronize(stream));
+                fused_jobs_ = (int)jobs.size();
+                fused_lds_ = lds;
+            }
+        }
         const double t0 = wall_ms();
         // batch_ is (re)made on the stream the solve runs on
         for (int s = 0; s < S; ++s) {
@@ -1142,6 +1179,9 @@ public:
 private:
     hipStream_t own_stream_ = nullptr;
     bool null_user_ = false;   // the caller's stream is the null stream; the solves run on own_stream_
+    DevBuf<SweepJob> fused_;   // the fused sweep's job table (fused_jobs_ > 0)
+    int fused_jobs_ = 0;
+    size_t fused_lds_ = 0;
     std::unique_ptr<BatchKrylov> kry_;
     std::vector<int> nr_;
     std::vector<int64_t> n_, base_;
@@ -1305,6 +1345,7 @@ void sap_default_options(sap_options* o)
     o->profile = 0;
     o->batch = 1;
     o->graph = 1;
+    o->fuse = 1;
     o->autotune = 0;
 }
 
@@ -1369,7 +1410,6 @@ int sap_setup_csr(sap_solver* h, int n, int nnz, const int* d_row_offsets, const
         return SAP_ERR_BADARG;
     }
     const sap_options& o = h->s->opt;
-    if (o.testDB) { h->err = "testDB is not supported"; return SAP_ERR_UNSUPPORTED; }
     return guarded(h, [&] { return h->s->setup_csr(n, nnz, d_row_offsets, d_col_indices, d_values); });
 }
 
@@ -1657,6 +1697,7 @@ sap_reordering* sap_reorder_csr(int n, const int* h_row_offsets, const int* h_co
     go.doRCM = o.maxBandwidth > 64;
     go.maxBandwidth = o.maxBandwidth;
     go.dropOffFrac = o.dropOffFraction;
+    go.testDB = o.testDB != 0;
     auto* r = new (std::nothrow) sap_reordering();
     if (!r) return fail(SAP_ERR_OOM);
     r->precision = o.precValueType;
@@ -1684,6 +1725,20 @@ int sap_reordering_info(const sap_reordering* r, int* k, int* k_db, int* k_reord
         if (k_reorder) *k_reorder = R.k_reorder;
         if (drop_off) *drop_off = R.dropoff_actual;
         if (nnz) *nnz = (int)R.ci.size();
+    };
+    if (r->precision == SAP_PREC_FLOAT) get(r->f); else get(r->d);
+    return SAP_OK;
+}
+
+int sap_reordering_dominance(const sap_reordering* r, double* diag_dom_ori, double* d_p1_ori, double* diag_dom,
+                             double* d_p1)
+{
+    if (!r) return SAP_ERR_BADARG;
+    auto get = [&](const auto& R) {
+        if (diag_dom_ori) *diag_dom_ori = R.diag_dom_ori;
+        if (d_p1_ori) *d_p1_ori = R.d_p1_ori;
+        if (diag_dom) *diag_dom = R.diag_dom;
+        if (d_p1) *d_p1 = R.d_p1;
     };
     if (r->precision == SAP_PREC_FLOAT) get(r->f); else get(r->d);
     return SAP_OK;

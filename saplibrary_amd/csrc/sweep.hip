@@ -676,24 +676,15 @@ __global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_interface_
 // UL-factored last partition the mirror order.  corr (2k per boundary) may
 // be null.  Output FP64 (outd) or preconditioner precision (outt).
 // ---------------------------------------------------------------------------
+// The full sweep of partition p for one right-hand side (pointers already
+// at that column), shared by the per-system and the job-table kernels.
 template <class T, int MODE>
-__global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_full_kernel(
-    const T* __restrict__ B, const T* __restrict__ Bul, int k, PartInfo pi, int last_is_ul,
-    const double* __restrict__ v, const T* __restrict__ corr, double* __restrict__ outd, T* __restrict__ outt,
-    T* __restrict__ scr, int ring_mask, int nbuf, int nr, int64_t ldv, int64_t ldo, const int* __restrict__ gate)
+__device__ __forceinline__ void sweep_full_body(const T* __restrict__ B, const T* __restrict__ Bul, int k, PartInfo pi,
+                                                int last_is_ul, int p, const double* __restrict__ v,
+                                                const T* __restrict__ corr, double* __restrict__ outd,
+                                                T* __restrict__ outt, T* __restrict__ scr, int ring_mask, int nbuf,
+                                                T* pend, T* xblk, T* xblk2, T* acc2)
 {
-    T* pend = lds_ring<T>();
-    __shared__ T xblk[NBS];
-    __shared__ T xblk2[2 * NBS];
-    __shared__ T acc2[2 * NBS];
-    int p, rh;
-    if (!job_and_rhs(pi.P, nr, p, rh)) return;
-    if (gate && !gate[rh]) return;
-    v += rh * ldv;
-    if (corr) corr += rh * (int64_t)2 * k * (pi.P - 1);
-    if (outd) outd += rh * ldo;
-    if (outt) outt += rh * pi.n;
-    scr += rh * pi.n;
     int64_t s, len;
     pi.bounds(p, s, len);
     const bool ul = last_is_ul && p == pi.P - 1;
@@ -754,6 +745,51 @@ __global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_full_kerne
         else
             lower_sweep<T>(V2, len, in2, [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, pend, ring_mask, xblk);
     }
+}
+
+template <class T, int MODE>
+__global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_full_kernel(
+    const T* __restrict__ B, const T* __restrict__ Bul, int k, PartInfo pi, int last_is_ul,
+    const double* __restrict__ v, const T* __restrict__ corr, double* __restrict__ outd, T* __restrict__ outt,
+    T* __restrict__ scr, int ring_mask, int nbuf, int nr, int64_t ldv, int64_t ldo, const int* __restrict__ gate)
+{
+    T* pend = lds_ring<T>();
+    __shared__ T xblk[NBS];
+    __shared__ T xblk2[2 * NBS];
+    __shared__ T acc2[2 * NBS];
+    int p, rh;
+    if (!job_and_rhs(pi.P, nr, p, rh)) return;
+    if (gate && !gate[rh]) return;
+    v += rh * ldv;
+    if (corr) corr += rh * (int64_t)2 * k * (pi.P - 1);
+    if (outd) outd += rh * ldo;
+    if (outt) outt += rh * pi.n;
+    scr += rh * pi.n;
+    sweep_full_body<T, MODE>(B, Bul, k, pi, last_is_ul, p, v, corr, outd, outt, scr, ring_mask, nbuf, pend, xblk,
+                             xblk2, acc2);
+}
+
+// Full sweeps of several systems in one launch: workgroup b takes job b of
+// a table (one partition of one system for one right-hand side; null B:
+// padding), with that system's band, width, staging depth and LDS ring;
+// the input column is V + voff and the output Z + ooff.
+template <class T>
+__global__ __launch_bounds__(SW_THREADS) void sweep_full_jobs_kernel(const SweepJob* __restrict__ jobs,
+                                                                     const double* __restrict__ V,
+                                                                     double* __restrict__ Z,
+                                                                     const int* __restrict__ gate)
+{
+    T* pend = lds_ring<T>();
+    __shared__ T xblk[NBS];
+    __shared__ T xblk2[2 * NBS];
+    __shared__ T acc2[2 * NBS];
+    const SweepJob J = jobs[blockIdx.x];
+    if (!J.B) return;
+    if (gate && !gate[J.gate]) return;
+    const PartInfo pi{J.n, J.P};
+    sweep_full_body<T, 2>((const T*)J.B, (const T*)J.Bul, J.k, pi, J.last_is_ul, J.part, V + J.voff,
+                          (const T*)J.corr, Z + J.ooff, nullptr, (T*)J.scr, J.ring_mask, J.nbuf, pend, xblk, xblk2,
+                          acc2);
 }
 
 // ---------------------------------------------------------------------------
@@ -1014,6 +1050,52 @@ void launch_diag_divide(const T* B, int64_t n, const double* v, double* out, hip
 }
 
 template <class T>
+bool sweep_jobs_append(std::vector<SweepJob>& jobs, size_t& lds, const T* B, const T* Bul, int k, PartInfo pi,
+                       int last_is_ul, const T* corr, T* scr, int64_t voff, int64_t ldv, int64_t ooff, int64_t ldo,
+                       int nr, int gate0, int nbuf_req)
+{
+    const SweepCfg c = sweep_cfg<T>(k, nbuf_req);
+    if (c.mode != 2 || k == 0) return false;
+    lds = std::max(lds, c.lds);
+    // the kernel's (job, right-hand side) order per system: the nr
+    // workgroups of a partition 8 ids apart (one XCD), padded to whole
+    // groups of 8 partitions
+    const int groups = (pi.P + 7) / 8;
+    for (int q = 0; q < groups * 8 * nr; ++q) {
+        const int r = (q >> 3) % nr;
+        const int part = ((q >> 3) / nr) * 8 + (q & 7);
+        SweepJob J{};
+        if (part < pi.P) {
+            J.B = B;
+            J.Bul = Bul;
+            J.corr = corr ? corr + (int64_t)r * 2 * k * (pi.P - 1) : nullptr;
+            J.scr = scr + (int64_t)r * pi.n;
+            J.voff = voff + r * ldv;
+            J.ooff = ooff + r * ldo;
+            J.n = pi.n;
+            J.P = pi.P;
+            J.k = k;
+            J.part = part;
+            J.gate = gate0 + r;
+            J.last_is_ul = last_is_ul;
+            J.ring_mask = ring_size(k) - 1;
+            J.nbuf = c.nbuf;
+        }
+        jobs.push_back(J);
+    }
+    return true;
+}
+
+template <class T>
+void launch_sweep_full_jobs(const SweepJob* d_jobs, int njobs, size_t lds, const double* V, double* Z,
+                            const int* gate, hipStream_t st)
+{
+    set_lds_limit((const void*)sweep_full_jobs_kernel<T>, lds);
+    hipLaunchKernelGGL(sweep_full_jobs_kernel<T>, dim3(njobs), dim3(SW_THREADS), lds, st, d_jobs, V, Z, gate);
+    SAP_LAUNCH_CHECK();
+}
+
+template <class T>
 int sweep_max_nbuf(int k)
 {
     const SweepCfg c = sweep_cfg<T>(k);
@@ -1022,6 +1104,10 @@ int sweep_max_nbuf(int k)
 
 #define SAP_INST(T)                                                                                               \
     template int sweep_max_nbuf<T>(int);                                                                           \
+    template bool sweep_jobs_append<T>(std::vector<SweepJob>&, size_t&, const T*, const T*, int, PartInfo, int,    \
+                                       const T*, T*, int64_t, int64_t, int64_t, int64_t, int, int, int);            \
+    template void launch_sweep_full_jobs<T>(const SweepJob*, int, size_t, const double*, double*, const int*,      \
+                                            hipStream_t);                                                          \
     template void launch_sweep_interface<T>(const T*, const T*, int, PartInfo, const double*, T*, T*, T*,          \
                                             hipStream_t, const Rhs&);                                              \
     template void launch_interface_solve<T>(const T*, const T*, const T*, const T*, int, PartInfo, const T*, T*,   \

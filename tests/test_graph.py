@@ -376,3 +376,42 @@ def test_product_var_assembly_is_bit_exact(sap, case, prec):
     assert got["P"] == Pc
     for f, x in v.items():
         assert np.array_equal(got["var"][f], x), f
+
+
+# ------------------------------------------------------------------- testDB
+def _dominance(n, rp, ci, val):
+    d = np.empty(n)
+    for i in range(n):
+        cols, v = ci[rp[i]:rp[i + 1]], np.abs(val[rp[i]:rp[i + 1]].astype(np.float64))
+        diag = v[cols == i].sum()
+        d[i] = diag / v[cols != i].sum()
+    d.sort()
+    return d[0], d[n // 100]
+
+
+@pytest.mark.parametrize("prec", ["float", "double"])
+def test_testdb_statistics(sap, prec):
+    """testDB (sap/graph.h:468-490, :537-561): the dominance statistics of
+    the input and of the DB-permuted and scaled matrix, computed directly;
+    the test stops after DB (no RCM: the bandwidth is the one after DB), and
+    after DB scaling every matched entry is 1 in magnitude and the others at
+    most 1, so the minimum ratio cannot drop below that of |a_ii| = 1 over
+    the row's other entries."""
+    n = 3000
+    rp, ci, val = sparse_band(n, 60, 6, 81)
+    T = np.float32 if prec == "float" else np.float64
+    vT = val.astype(T)
+    opts = sap.Options(testDB=1, precValueType=sap.PREC_FLOAT if prec == "float" else sap.PREC_DOUBLE)
+    r = sap.reorder_csr(rp, ci, val, opts)
+    dmin, dp1 = _dominance(n, rp, ci, vT)
+    assert r["diag_dom_ori"] == dmin and r["d_p1_ori"] == dp1
+    dmin2, dp12 = _dominance(n, r["rp"], r["ci"], r["val"].astype(T))
+    assert r["diag_dom"] == dmin2 and r["d_p1"] == dp12
+    assert r["k"] == r["k_db"] == r["k_reorder"]
+    assert np.array_equal(r["col_reord"], np.arange(n))
+    rows = np.repeat(np.arange(n), np.diff(r["rp"]))
+    assert np.abs(r["val"][rows == r["ci"]]).min() > 0.999
+    # |a_ii| = 1 and the others at most 1: d_i >= 1 / (entries of row i - 1)
+    others = np.diff(r["rp"]) - 1
+    assert r["diag_dom"] >= (1.0 / others.max()) * (1 - 1e-5)
+    assert r["d_p1"] > r["d_p1_ori"]
