@@ -100,7 +100,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS) + ["c2", "c3"])
+    ap.add_argument("--workload", default="c1", choices=sorted(WORKLOADS) + ["c2", "c3", "c4"])
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -119,6 +119,8 @@ def main():
         return bench_c3(args, torch, sap)
     if args.workload == "c2":
         return bench_c2(args, torch, sap)
+    if args.workload == "c4":
+        return bench_c4(args, torch, sap)
 
     if args.copy_calibration:
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
@@ -140,11 +142,13 @@ def main():
     torch.cuda.synchronize()
 
     def options(plain):
+        # autotune: the warm-up step times the conversion tile widths and
+        # sweep staging depths for this k (the results do not change)
         if plain:
             return sap.Options(precondType=sap.Block, factMethod=sap.LU_UL, precValueType=sap.PREC_DOUBLE,
-                               relTol=1e-10, profile=1)
+                               relTol=1e-10, profile=1, autotune=1)
         return sap.Options(precondType=sap.Spike if spike else sap.Block, factMethod=sap.LU_UL,
-                           precValueType=sap.PREC_FLOAT, relTol=1e-10, refine=8, profile=1)
+                           precValueType=sap.PREC_FLOAT, relTol=1e-10, refine=8, profile=1, autotune=1)
 
     def run(plain, steps, warmup, label):
         # one solver object: buffers allocated by the warm-up setup are reused
@@ -239,6 +243,10 @@ def main():
         "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "autotune": [{"family": ["conversion_tile", "sweep_nbuf"][e["family"]], "k": e["k"],
+                      "elem_bytes": e["elem_bytes"], "value": e["value"], "default": e["default_value"],
+                      "best_ms": round(e["best_ms"], 4), "default_ms": round(e["default_ms"], 4)}
+                     for e in sap.autotune_entries()],
     }
     print(json.dumps(out), flush=True)
 
@@ -275,7 +283,8 @@ def bench_c3(args, torch, sap):
     banded pattern, values x (1 + U[-0.05, 0.05]) per step, each followed by
     a solve."""
     P = 16
-    opts = dict(precondType=sap.Block, factMethod=sap.LU_UL, precValueType=sap.PREC_FLOAT, relTol=1e-10, refine=8)
+    opts = dict(precondType=sap.Block, factMethod=sap.LU_UL, precValueType=sap.PREC_FLOAT, relTol=1e-10, refine=8,
+                autotune=1)
     specs = c3_systems()
     mats, B = [], []
     for n, k, seed in specs:
@@ -413,6 +422,58 @@ def bench_c2(args, torch, sap):
           {"workload": "CSR N=500000 ~12 nnz/row, band width 300 under a symmetric permutation, 20% weak "
                        "diagonals; DB + scaling + RCM on the host, then SaP (LU_only, variable-bandwidth path)",
            "n": n, "nnz": int(rp[-1])}, specified=out["specified"], converging=conv)
+
+
+def bench_c4(args, torch, sap):
+    """Learned option selector: the network is trained on the GPU (the
+    library's own kernels) on the committed corpus of solver-measured
+    timings (tests/golden/selector_corpus.npz), then fresh matrices of the
+    corpus generators (never seen in training) are solved with the default
+    options and with the chosen ones (falling back to the defaults when the
+    choice does not reach the tolerance), both timed live."""
+    import numpy as np
+    from saplibrary_amd import selector as S
+    z = np.load(os.path.join(ROOT, "tests", "golden", "selector_corpus.npz"))
+    F, T, OK = z["features"], z["times"], z["ok"]
+    keep = OK.any(axis=1)
+    t0 = time.perf_counter()
+    sel = S.Selector(sap, torch, seed=1)
+    losses = sel.fit(F[keep], T[keep], OK[keep], epochs=3000)
+    train_ms = (time.perf_counter() - t0) * 1e3
+    print(f"[bench] c4: trained on {int(keep.sum())} matrices in {train_ms:.0f} ms, loss {losses[0]:.3g} -> "
+          f"{losses[-1]:.3g}", flush=True)
+    count = 24
+    rows = []
+    for q in range(count):
+        i = 5000 + q
+        A, b, f, is_csr, desc = S.corpus_matrix(sap, torch, i, np.random.default_rng((4, i)))
+        g = int(sel.choose(f[None, :])[0])
+        S.time_config(sap, torch, A, b, S.DEFAULT, is_csr)      # warm-up of this size
+        td, okd = S.time_config(sap, torch, A, b, S.DEFAULT, is_csr, reps=2)
+        tl, okl = S.time_config(sap, torch, A, b, S.GRID[g], is_csr, reps=2)
+        if not okl:
+            tl += td
+        rows.append((td, okd, tl, okl or okd, g, desc))
+        print(f"[bench] c4 matrix {q}: {desc} default {td:.1f} ms ok={okd}, learned {S.GRID[g]} {tl:.1f} ms",
+              flush=True)
+    both = [(td, tl) for td, okd, tl, ok, g, d in rows if okd]
+    ratio = np.array([tl / td for td, tl in both])
+    frac = float(np.mean(ratio <= 1.0)) if len(ratio) else None
+    gm = float(np.exp(np.mean(np.log(ratio)))) if len(ratio) else None
+    solved_default = sum(1 for r in rows if r[1])
+    solved_learned = sum(1 for r in rows if r[3])
+    total_d = sum(r[0] for r in rows)
+    total_l = sum(r[2] for r in rows)
+    _line("learned options vs defaults, time-to-solution ratio (geometric mean) @ rel. residual 1e-10", gm,
+          "ratio", args, total_l / count, "f32 preconditioner + f64 Krylov; f32 network",
+          "synthetic (corpus generators, matrices 5000..5023 not in the training corpus)",
+          {"workload": "24 fresh matrices (60% banded N 20k-100k K 10-150 dominance 0.5-2, 40% CSR N 10k-30k "
+                       "width 40-300 with weak diagonals), default options (Spike LU_only DB P=16) vs the "
+                       "network's choice from 24 features over a 60-point option grid",
+           "grid": len(S.GRID), "features": S.N_FEATURES},
+          no_slower_fraction=frac, matrices_default_converged=solved_default,
+          matrices_learned_converged=solved_learned, total_default_ms=round(total_d, 1),
+          total_learned_ms=round(total_l, 1), train_ms=round(train_ms, 1), train_matrices=int(keep.sum()))
 
 
 def cpu_baseline(workload, threads):

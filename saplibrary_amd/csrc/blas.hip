@@ -97,7 +97,10 @@ __global__ void axpby_ratio_kernel(int64_t n, double num, const double* __restri
                                    const double* __restrict__ x, double beta, const double* __restrict__ y,
                                    double* __restrict__ z)
 {
-    const double alpha = sign * (num / *den);
+    // a zero gamma stops the solve (code -11) right after this update; the
+    // updated vectors are then dead, and alpha = 0 keeps them finite
+    const double d = *den;
+    const double alpha = d == 0.0 ? 0.0 : sign * (num / d);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
         z[i] = axpby_elem(alpha, x[i], beta, y[i]);

@@ -413,6 +413,7 @@ int Precond<T>::factor_band_steps()
     cnt_.resize(1);
     g_.resize(n);
     scr0_.resize(n);
+    split_ = false;
 
     if (!spike_active()) {
         // buffers of the SaP-C steps are not needed (and not reported)
@@ -496,8 +497,21 @@ int Precond<T>::factor_band_steps()
             pi.bounds(p, s, len);
             jobs.push_back({s, len, -1, boost, boost_last, 1});
         }
+        // With every pivot boosted (no factorization can fail), also the LU
+        // of the last partition and the UL of the first (two more jobs
+        // beside P - 1 + P - 1 on a chip of 256 CUs): the full sweep then
+        // gives each partition two workgroups, rows below its middle from
+        // the LU factors and rows above from the UL factors.
+        split_ = boost && boost_last;
+        if (split_) {
+            int64_t s0, l0;
+            pi.bounds(0, s0, l0);
+            jobs.push_back({s_last, l_last, 1, boost, boost_last, 0});
+            jobs.push_back({s0, l0, -1, boost, boost_last, 1});
+        }
         scr1_.resize(n);
     } else {
+        split_ = false;
         scr1_.release();
         for (int p = 0; p < P; ++p) {
             int64_t s, len;
@@ -605,8 +619,10 @@ void Precond<T>::solve_band(const double* v, double* z)
                                       stream);
         }
         ProfScope ps(prof, "sweep_full");
+        Rhs r2 = rh;
+        if (split_ && lu_ul()) r2.split_scr = scr1_.get();
         launch_sweep_full<T>(B_.get(), Bul_.get(), k, pi, lu_ul() ? 1 : 0, v, corr_.get(), z, nullptr, scr0_.get(),
-                             stream, rh);
+                             stream, r2);
     } else {
         ProfScope ps(prof, "sweep_full");
         launch_sweep_full<T>(B_.get(), B_.get(), k, pi, 0, v, nullptr, z, nullptr, scr0_.get(), stream, rh);
@@ -695,8 +711,10 @@ void Precond<T>::solve_band_multi(const double* V, int64_t ldv, double* Z, int64
                                       stream, rh);
         }
         ProfScope ps(prof, "sweep_full");
+        Rhs r2 = rh;
+        if (split_ && lu_ul()) r2.split_scr = scr1m_.get();
         launch_sweep_full<T>(B_.get(), Bul_.get(), k, pi, lu_ul() ? 1 : 0, V, corrm_.get(), Z, nullptr, scr0m_.get(),
-                             stream, rh);
+                             stream, r2);
     } else {
         ProfScope ps(prof, "sweep_full");
         launch_sweep_full<T>(B_.get(), B_.get(), k, pi, 0, V, nullptr, Z, nullptr, scr0m_.get(), stream, rh);

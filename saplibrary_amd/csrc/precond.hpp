@@ -215,6 +215,9 @@ private:
     void solve_band(const double* v, double* z);
     // tuned launch parameters (0: the untuned defaults)
     int tile_ = 0, nbuf_ = 0;
+    // LU_UL with every pivot boosted: LU and UL factors of every partition,
+    // two workgroups per partition in the full sweep
+    bool split_ = false;
     void tune_sweeps();
     void solve_band_multi(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate);
     // apply_multi's work buffers (nr_cap_ columns)

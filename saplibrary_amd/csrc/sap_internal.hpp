@@ -257,6 +257,11 @@ struct Rhs {
     int64_t ldv = 0, ldo = 0;
     const int* gate = nullptr;
     int nbuf = 0;   // LDS block buffers of the staged sweeps (0: as many as fit)
+    // full sweeps only: when set, every partition gets two workgroups, the
+    // bottom half of its rows from the LU band and the top half from the UL
+    // band (both must hold factors of every partition); the top half's
+    // stage-1 values go to this second scratch (same layout as scr)
+    void* split_scr = nullptr;
 };
 // the staged sweeps' largest LDS block-buffer count for k (0: not staged)
 template <class T>

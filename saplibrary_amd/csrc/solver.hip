@@ -441,6 +441,13 @@ public:
                             last_res_dot_ = rr0_dot;
                         }
                         if (done) break;
+                        // Not reached: on a first step the monitor has no
+                        // stagnation count >= 3 and no extra steps, so it asks
+                        // for a check only when r_norm <= tol, and finished()
+                        // then converges (a NaN norm stops in
+                        // needCheckConvergence).  The reference would hold
+                        // b - A M^{-1} xx in rr(0) here; the recursive rr(0)
+                        // is kept.
                         prec(rr[0], Pv);
                     } else {
                         const double* src = rr[j];

@@ -683,11 +683,16 @@ __device__ __forceinline__ void sweep_full_body(const T* __restrict__ B, const T
                                                 int last_is_ul, int p, const double* __restrict__ v,
                                                 const T* __restrict__ corr, double* __restrict__ outd,
                                                 T* __restrict__ outt, T* __restrict__ scr, int ring_mask, int nbuf,
-                                                T* pend, T* xblk, T* xblk2, T* acc2)
+                                                T* pend, T* xblk, T* xblk2, T* acc2, int half = -1)
 {
     int64_t s, len;
     pi.bounds(p, s, len);
-    const bool ul = last_is_ul && p == pi.P - 1;
+    // half 0: rows [s + len/2, s + len) from the LU factors (stage 2 from
+    // the bottom up to the middle); half 1: rows [s, s + len/2) from the UL
+    // factors (stage 2 from the top down to the middle); -1: all rows
+    const bool ul = half >= 0 ? half == 1 : (last_is_ul && p == pi.P - 1);
+    const int64_t mid = len / 2;
+    const int64_t len2 = half < 0 ? len : (half == 1 ? mid : len - mid);
     const T* F = ul ? Bul : B;
     const int d1 = ul ? -1 : 1;
     const BandView<T> V1 = make_view(F, s, len, k, d1);
@@ -718,11 +723,11 @@ __device__ __forceinline__ void sweep_full_body(const T* __restrict__ B, const T
                            pend, ring_mask, xblk2, acc2, dyn, nbuf);
         __syncthreads();
         if (outd)
-            lower_sweep_lds<T>(V2, len, (const T*)scr, grow(-d1, 0), -d1, same,
+            lower_sweep_lds<T>(V2, len2, (const T*)scr, grow(-d1, 0), -d1, same,
                                [&](int64_t r, T y) { outd[grow(-d1, r)] = (double)y; }, false, pend, ring_mask, xblk2,
                                acc2, dyn, nbuf);
         else
-            lower_sweep_lds<T>(V2, len, (const T*)scr, grow(-d1, 0), -d1, same,
+            lower_sweep_lds<T>(V2, len2, (const T*)scr, grow(-d1, 0), -d1, same,
                                [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, false, pend, ring_mask, xblk2, acc2,
                                dyn, nbuf);
     } else if constexpr (MODE == 1) {
@@ -730,20 +735,20 @@ __device__ __forceinline__ void sweep_full_body(const T* __restrict__ B, const T
                             acc2);
         __syncthreads();
         if (outd)
-            lower_sweep_fast<T>(V2, len, in2, [&](int64_t r, T y) { outd[grow(-d1, r)] = (double)y; }, false, pend,
+            lower_sweep_fast<T>(V2, len2, in2, [&](int64_t r, T y) { outd[grow(-d1, r)] = (double)y; }, false, pend,
                                 ring_mask, xblk2, acc2);
         else
-            lower_sweep_fast<T>(V2, len, in2, [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, false, pend, ring_mask,
+            lower_sweep_fast<T>(V2, len2, in2, [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, false, pend, ring_mask,
                                 xblk2, acc2);
     } else {
         lower_sweep<T>(V1, len, rhs, [&](int64_t r, T y) { scr[grow(d1, r)] = y / V1.at(r, r); }, pend, ring_mask,
                        xblk);
         __syncthreads();
         if (outd)
-            lower_sweep<T>(V2, len, in2, [&](int64_t r, T y) { outd[grow(-d1, r)] = (double)y; }, pend, ring_mask,
+            lower_sweep<T>(V2, len2, in2, [&](int64_t r, T y) { outd[grow(-d1, r)] = (double)y; }, pend, ring_mask,
                            xblk);
         else
-            lower_sweep<T>(V2, len, in2, [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, pend, ring_mask, xblk);
+            lower_sweep<T>(V2, len2, in2, [&](int64_t r, T y) { outt[grow(-d1, r)] = y; }, pend, ring_mask, xblk);
     }
 }
 
@@ -751,22 +756,27 @@ template <class T, int MODE>
 __global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_full_kernel(
     const T* __restrict__ B, const T* __restrict__ Bul, int k, PartInfo pi, int last_is_ul,
     const double* __restrict__ v, const T* __restrict__ corr, double* __restrict__ outd, T* __restrict__ outt,
-    T* __restrict__ scr, int ring_mask, int nbuf, int nr, int64_t ldv, int64_t ldo, const int* __restrict__ gate)
+    T* __restrict__ scr, int ring_mask, int nbuf, int nr, int64_t ldv, int64_t ldo, const int* __restrict__ gate,
+    int split, T* __restrict__ scr_alt)
 {
     T* pend = lds_ring<T>();
     __shared__ T xblk[NBS];
     __shared__ T xblk2[2 * NBS];
     __shared__ T acc2[2 * NBS];
-    int p, rh;
-    if (!job_and_rhs(pi.P, nr, p, rh)) return;
+    // split: two workgroups per partition, the bottom half of its rows from
+    // the LU factors and the top half from the UL factors (scratch scr_alt)
+    int job, rh;
+    if (!job_and_rhs(split ? 2 * pi.P : pi.P, nr, job, rh)) return;
     if (gate && !gate[rh]) return;
+    const int p = split ? job >> 1 : job;
+    const int half = split ? (job & 1) : -1;
     v += rh * ldv;
     if (corr) corr += rh * (int64_t)2 * k * (pi.P - 1);
     if (outd) outd += rh * ldo;
     if (outt) outt += rh * pi.n;
-    scr += rh * pi.n;
-    sweep_full_body<T, MODE>(B, Bul, k, pi, last_is_ul, p, v, corr, outd, outt, scr, ring_mask, nbuf, pend, xblk,
-                             xblk2, acc2);
+    T* sc = (half == 1 ? scr_alt : scr) + rh * pi.n;
+    sweep_full_body<T, MODE>(B, Bul, k, pi, last_is_ul, p, v, corr, outd, outt, sc, ring_mask, nbuf, pend, xblk,
+                             xblk2, acc2, half);
 }
 
 // Full sweeps of several systems in one launch: workgroup b takes job b of
@@ -1023,9 +1033,10 @@ void launch_sweep_full_t(const T* B, const T* Bul, int k, PartInfo pi, int last_
                          double* outd, T* outt, T* scr, const SweepCfg& c, const Rhs& rh, hipStream_t st)
 {
     set_lds_limit((const void*)sweep_full_kernel<T, MODE>, c.lds);
-    hipLaunchKernelGGL((sweep_full_kernel<T, MODE>), dim3(job_rhs_grid(pi.P, rh.nr)), dim3(c.threads), c.lds, st, B,
-                       Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, ring_size(k) - 1, c.nbuf, rh.nr, rh.ldv,
-                       rh.ldo, rh.gate);
+    const int split = rh.split_scr != nullptr;
+    hipLaunchKernelGGL((sweep_full_kernel<T, MODE>), dim3(job_rhs_grid(split ? 2 * pi.P : pi.P, rh.nr)),
+                       dim3(c.threads), c.lds, st, B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr,
+                       ring_size(k) - 1, c.nbuf, rh.nr, rh.ldv, rh.ldo, rh.gate, split, (T*)rh.split_scr);
     SAP_LAUNCH_CHECK();
 }
 

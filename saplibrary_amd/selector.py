@@ -207,7 +207,7 @@ def corpus_matrix(sap, torch, i, rng):
     return A, b, f, True, desc
 
 
-def generate_corpus(sap, torch, count, start=0, seed=4, log=print):
+def generate_corpus(sap, torch, count, start=0, seed=4, log=print, on_row=None):
     """Features (count x 24), times (count x |GRID|, ms) and convergence
     flags of corpus matrices start .. start + count - 1 (matrix i drawn from
     its own generator, seeded by (seed, i)), each option set timed by the
@@ -231,6 +231,8 @@ def generate_corpus(sap, torch, count, start=0, seed=4, log=print):
         kinds.append(1 if is_csr else 0)
         log(f"[corpus] {i} ({q + 1}/{count}) {desc} default {T[q, DEFAULT_INDEX]:.1f} ms "
             f"ok={OK[q, DEFAULT_INDEX]} best {T[q][OK[q]].min() if OK[q].any() else float('nan'):.1f} ms")
+        if on_row is not None:
+            on_row(q + 1, F[:q + 1], T[:q + 1], OK[:q + 1], np.array(kinds))
     return F, T, OK, np.array(kinds)
 
 

@@ -24,9 +24,13 @@ def main():
     count = int(sys.argv[1]) if len(sys.argv) > 1 else 240
     out = sys.argv[2] if len(sys.argv) > 2 else os.path.join(ROOT, "tests", "golden", "selector_corpus.npz")
     start = int(sys.argv[3]) if len(sys.argv) > 3 else 0
-    F, T, OK, kinds = S.generate_corpus(sap, torch, count, start=start, log=lambda m: print(m, flush=True))
-    np.savez_compressed(out, features=F, times=T, ok=OK, kinds=kinds, grid=np.array(S.GRID),
-                        default_index=S.DEFAULT_INDEX)
+
+    def save(rows, F, T, OK, kinds):
+        # written after every matrix, so a run cut short keeps what it measured
+        np.savez_compressed(out, features=F, times=T, ok=OK, kinds=kinds, grid=np.array(S.GRID),
+                            default_index=S.DEFAULT_INDEX)
+
+    S.generate_corpus(sap, torch, count, start=start, log=lambda m: print(m, flush=True), on_row=save)
     print(f"wrote {out}", flush=True)
 
 

@@ -171,24 +171,34 @@ def test_batched_graph_replays_match_eager(sap, oracle):
         np.testing.assert_array_equal(host(X), eager)
 
 
-def test_system_batch_equals_single_systems_bitwise(sap, oracle):
+SYSTEM_SETS = [
+    # SaP-C systems, one diagonal system: per-system launches
+    ([(9000, 20, 4), (5000, 7, 2), (12000, 45, 6), (3000, 0, 1)], 0),
+    # Block (SaP-D) systems: one fused sweep launch for all of them
+    ([(9000, 20, 4), (5000, 7, 2), (12000, 45, 6), (7000, 130, 3), (4000, 230, 2)], 1),
+]
+
+
+@pytest.mark.parametrize("specs,block", SYSTEM_SETS)
+@pytest.mark.parametrize("fuse", [1, 0])
+def test_system_batch_equals_single_systems_bitwise(sap, oracle, specs, block, fuse):
     """Several independent banded systems of different sizes and widths
     (SystemBatch, sap_batch_*): every column equals solving it alone with
-    its own system's solver, bit for bit (no refinement)."""
-    specs = [(9000, 20, 4), (5000, 7, 2), (12000, 45, 6), (3000, 0, 1)]
+    its own system's solver, bit for bit (no refinement), with the systems'
+    sweeps in one launch (fuse, Block systems) or one launch per system."""
     mats, bands, Bs = [], [], []
     for i, (n, k, P) in enumerate(specs):
         band = gen_band(oracle, n, k, 1.2, 400 + i)
         bands.append(band)
         mats.append(sap.BandedMatrix(dev(band)))
         Bs.append(np.stack([oracle.gen_vector(n, 410 + 10 * i + c) for c in range(3)], axis=1))
-    opts = dict(relTol=1e-6, precondType=sap.Spike, factMethod=sap.LU_UL)   # FP32 factors, no refinement
-    sb = sap.SystemBatch([P for _, _, P in specs], sap.Options(**opts))
+    opts = dict(relTol=1e-6, precondType=sap.Block if block else sap.Spike, factMethod=sap.LU_UL)   # FP32, no refinement
+    sb = sap.SystemBatch([P for _, _, P in specs], sap.Options(fuse=fuse, **opts))
     sb.setup(mats)
     Xd = [torch.zeros((n, 3), dtype=torch.float64, device="cuda") for n, _, _ in specs]
-    assert sb.solve([dev(B) for B in Bs], Xd)
+    ok = sb.solve([dev(B) for B in Bs], Xd)
     res = sb.results()
-    assert len(res) == 3 * len(specs) and all(r[0] == 1 for r in res)
+    assert len(res) == 3 * len(specs) and ok == all(r[0] == 1 for r in res)
     for i, (n, k, P) in enumerate(specs):
         Xs, its, codes = one_by_one(sap, mats[i], opts, P, Bs[i], np.zeros_like(Bs[i]))
         np.testing.assert_array_equal(host(Xd[i]), Xs)

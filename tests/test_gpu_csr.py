@@ -260,12 +260,23 @@ def test_reordering_path_partition_clamp(sap, oracle, variable):
     np.testing.assert_allclose(host(z), expect, rtol=0, atol=1e-10 * np.abs(expect).max())
 
 
-def test_csr_options_rejected(sap):
-    rp, ci, val = sparse_band(200, 10, 3, 70)
+def test_csr_testdb_setup(sap):
+    """testDB (sap/precond.h:1741-1756): the setup runs DB, reports the
+    dominance statistics the host reordering computes and the bandwidth
+    after DB, builds no preconditioner (a solve is then Illegal_solve)."""
+    rp, ci, val = sparse_band(2000, 40, 5, 70)
     A = sap.CsrMatrix.from_host(rp, ci, val)
+    g = sap.Solver(2, sap.Options(testDB=1))
+    g.setup(A)
+    st = g.getStats()
+    r = sap.reorder_csr(rp, ci, val, sap.Options(testDB=1))
+    assert (st.diag_dom_ori, st.d_p1_ori, st.diag_dom, st.d_p1) == (r["diag_dom_ori"], r["d_p1_ori"],
+                                                                    r["diag_dom"], r["d_p1"])
+    assert st.bandwidthDB == r["k_db"] and st.d_p1 > st.d_p1_ori > 0
+    b = torch.ones(2000, dtype=torch.float64, device="cuda")
     with pytest.raises(sap.SapError) as ei:
-        sap.Solver(2, sap.Options(variableBandwidth=0, testDB=1)).setup(A)
-    assert ei.value.reason == -9
+        g.solve(None, b, torch.zeros_like(b))
+    assert ei.value.reason == -4
 
 
 # ------------------------------------------------------------------- update
