@@ -405,6 +405,12 @@ int         sap_mlp_train(sap_mlp* m, const float* d_X, const float* d_T, const 
                           float lr, float* h_losses);
 const char* sap_mlp_last_error(const sap_mlp* m);
 
+/* Measurement: the best of reps timed launches (after one warm-up) of the
+ * library's streaming-read kernel over n doubles (n even, 16-byte aligned
+ * device pointer), in ms on the null stream: the device's practical read
+ * bandwidth, beside which the band-streaming kernels are quoted. */
+int         sap_stream_read_ms(const double* d_x, int64_t n, int reps, double* best_ms);
+
 /* Kernel-family timing since the last sap_reset_profile (profile = 1). */
 int         sap_get_profile(const sap_solver* s, sap_kernel_profile* out, int max_entries);
 void        sap_reset_profile(sap_solver* s);

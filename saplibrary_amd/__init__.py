@@ -143,7 +143,7 @@ EXPORTS = (
     "sap_reordering_assemble_var", "sap_reordering_var_get", "sap_mm_read", "sap_mm_info", "sap_mm_get_csr",
     "sap_mm_free", "sap_solve_noalloc", "sap_batch_create", "sap_batch_destroy", "sap_batch_set_stream",
     "sap_batch_setup_banded", "sap_batch_solve", "sap_batch_result", "sap_batch_times", "sap_batch_last_error",
-    "sap_autotune_entries", "sap_reordering_dominance", "sap_mlp_create", "sap_mlp_destroy", "sap_mlp_num_params", "sap_mlp_get_params",
+    "sap_autotune_entries", "sap_stream_read_ms", "sap_reordering_dominance", "sap_mlp_create", "sap_mlp_destroy", "sap_mlp_num_params", "sap_mlp_get_params",
     "sap_mlp_set_params", "sap_mlp_forward", "sap_mlp_gradient", "sap_mlp_train", "sap_mlp_last_error",
 )
 
@@ -153,6 +153,15 @@ class TuneEntry(ctypes.Structure):
     _fields_ = [("family", ctypes.c_int), ("k", ctypes.c_int), ("elem_bytes", ctypes.c_int), ("value", ctypes.c_int),
                 ("default_value", ctypes.c_int), ("candidates", ctypes.c_int), ("best_ms", ctypes.c_double),
                 ("default_ms", ctypes.c_double)]
+
+
+def stream_read_gbs(nbytes=1 << 31, reps=5):
+    """The practical streaming-read bandwidth (GB/s) of the library's own
+    read kernel over a device buffer of nbytes."""
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    ms = ctypes.c_double()
+    _check(lib().sap_stream_read_ms(ctypes.c_void_p(x.data_ptr()), x.numel(), reps, ctypes.byref(ms)))
+    return nbytes / (ms.value * 1e-3) / 1e9
 
 
 def autotune_entries():
@@ -232,6 +241,7 @@ def lib():
     L.sap_batch_result.argtypes = [vp, i, ip, ctypes.POINTER(ctypes.c_float), dp]
     L.sap_batch_times.argtypes = [vp, dp, dp]
     L.sap_autotune_entries.argtypes = [ctypes.POINTER(TuneEntry), i]
+    L.sap_stream_read_ms.argtypes = [vp, i64, i, dp]
     fp = ctypes.POINTER(ctypes.c_float)
     L.sap_mlp_create.argtypes = [i, i, i, ctypes.c_uint64]
     L.sap_mlp_create.restype = vp

@@ -236,6 +236,24 @@ __global__ void spread_kernel(int64_t n, SpreadArgs a)
     }
 }
 
+// Streaming read of n doubles (16-byte non-temporal loads, a per-block sum
+// kept so the loads are not dead): the device's practical read bandwidth
+// for the kernels that stream a band once.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(256) void stream_read_kernel(int64_t n2, const dvec2* __restrict__ x,
+                                                          double* __restrict__ partials)
+{
+    double s = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) {
+        const dvec2 v = __builtin_nontemporal_load(x + i);
+        s += v.x + v.y;
+    }
+    s = block_reduce_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 inline unsigned grid_for(int64_t n)
 {
     int64_t g = (n + 255) / 256;
@@ -249,6 +267,29 @@ inline unsigned grid_for(int64_t n)
 int dot_partials_size() { return 3 * RED_BLOCKS; }
 
 int bdot_partials_size(int cols) { return 3 * RED_BLOCKS * cols; }
+
+double stream_read_best_ms(const double* x, int64_t n, int reps, hipStream_t st)
+{
+    const int blocks = 256 * 8;
+    DevBuf<double> part(blocks);
+    hipEvent_t a, b;
+    SAP_HIP_CHECK(hipEventCreate(&a));
+    SAP_HIP_CHECK(hipEventCreate(&b));
+    double best = -1.0;
+    for (int r = 0; r < reps + 1; ++r) {   // the first launch is a warm-up
+        SAP_HIP_CHECK(hipEventRecord(a, st));
+        hipLaunchKernelGGL(stream_read_kernel, dim3(blocks), dim3(256), 0, st, n / 2,
+                           reinterpret_cast<const dvec2*>(x), part.get());
+        SAP_HIP_CHECK(hipEventRecord(b, st));
+        SAP_HIP_CHECK(hipEventSynchronize(b));
+        float ms = 0.f;
+        SAP_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+        if (r > 0 && (best < 0 || ms < best)) best = ms;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return best;
+}
 
 // the batched grids: up to 8192 / cols workgroups per column (at least 64)
 inline unsigned bgrid_for(int64_t maxlen, int cols)

@@ -102,18 +102,19 @@ void Precond<T>::begin_setup(int64_t n_, int k_)
     flag_.resize(1);
     // 8 elements of slack: the sweeps copy 16-byte chunks covering each
     // column segment, which may reach a few elements past the last column
-    B_.resize((size_t)w * n + 8);
+    auto alloc_band = [&](DevBuf<T>& b) { b.resize((size_t)w * n + 8); };
+    alloc_band(B_);
     // LU_UL factors the UL of partitions 1..P-1 on a second copy of the band
     // (kept across setups of the same size)
     const bool need_ul = lu_ul() && cfg.precondType == SAP_Spike && P > 1 && k > 0;
-    if (need_ul) Bul_.resize((size_t)w * n + 8);
+    if (need_ul) alloc_band(Bul_);
     else Bul_.release();
     // the first partition is the longest (PartInfo gives the n mod P extra
     // rows to the first partitions)
     int64_t s0 = 0, len0 = n;
     PartInfo{n, P}.bounds(0, s0, len0);
     sep_in_ = need_ul && band_factor_resident_fits<T>(k, len0);
-    if (sep_in_) Bin_.resize((size_t)w * n + 8);
+    if (sep_in_) alloc_band(Bin_);
     else Bin_.release();
 }
 

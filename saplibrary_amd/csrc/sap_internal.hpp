@@ -204,6 +204,8 @@ struct BCoef {
 inline BCoef bconst(double v) { BCoef k; k.cst = v; return k; }
 inline BCoef bfield(int f, double sign = 1.0) { BCoef k; k.field = f; k.sign = sign; return k; }
 int  bdot_partials_size(int cols);
+// best of reps timed streaming reads of n doubles (n even, 16-byte aligned)
+double stream_read_best_ms(const double* x, int64_t n, int reps, hipStream_t st);
 // out[c * out_stride + q] = a[q](:, c) . b[q](:, c) for q < nd (1..3)
 void launch_bdots(const BatchCols& bc, int nd, const double* const* a, const double* const* b, const int* gate,
                   double* d_partials, double* out, int out_stride, hipStream_t st);

@@ -1356,6 +1356,15 @@ void sap_default_options(sap_options* o)
     o->autotune = 0;
 }
 
+int sap_stream_read_ms(const double* d_x, int64_t n, int reps, double* best_ms)
+{
+    if (!d_x || n < 2 || (n & 1) || reps < 1 || !best_ms || ((uintptr_t)d_x & 15)) return SAP_ERR_BADARG;
+    return guarded(nullptr, [&] {
+        *best_ms = sap_amd::stream_read_best_ms(d_x, n, reps, nullptr);
+        return 0;
+    });
+}
+
 int sap_autotune_entries(sap_tune_entry* out, int max_entries)
 {
     const auto e = sap_amd::Autotuner::instance().entries();

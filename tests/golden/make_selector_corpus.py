@@ -1,4 +1,5 @@
-"""Writes tests/golden/selector_corpus.npz: features and measured
+"""Writes tests/golden/selector_corpus.npz (the committed file holds matrices
+0-199: two runs, 100 0 and 100 100, concatenated): features and measured
 time-to-solution of every option set of the selector's grid for the
 corpus matrices (saplibrary_amd.selector.generate_corpus), measured by the
 solver on the GPU this script runs on.
