@@ -139,9 +139,12 @@ int Precond<T>::setup(int64_t n_, int k_, const double* d_band)
         // conversion tile width (a copy: every width writes the same band)
         const int tmax = band_to_colmajor_max_tile<T>(k);
         if (tmax >= 8) {
+            // multiples of 8 from 16 up: narrower tiles let two or three
+            // workgroups share a CU (read and write phases overlap), wider
+            // ones read longer row segments (less overfetch at their ends)
             std::vector<int> cand;
-            for (int t = 8; t <= tmax; t *= 2) cand.push_back(t);
-            if (cand.back() != tmax) cand.push_back(tmax);
+            for (int t = 16; t <= tmax; t += 8) cand.push_back(t);
+            if (cand.empty() || cand.back() != tmax) cand.push_back(tmax);
             T* b2 = sep_in_ ? nullptr : Bul_.get();
             tile_ = Autotuner::instance().tune(
                 TUNE_B2C_TILE, k, (int)sizeof(T), tmax, cand,

@@ -1005,7 +1005,8 @@ void launch_sweep_interface(const T* B, const T* Bul, int k, PartInfo pi, const 
                             hipStream_t st, const Rhs& rh)
 {
     if (pi.P < 2) return;
-    const SweepCfg c = sweep_cfg<T>(k, rh.nbuf);
+    // several columns: the fewest staging buffers, so more workgroups fit on a CU
+    const SweepCfg c = sweep_cfg<T>(k, rh.nr > 1 ? 4 : rh.nbuf);
     if (c.mode == 2)      launch_sweep_interface_t<T, 2>(B, Bul, k, pi, v, g, scr0, scr1, c, rh, st);
     else if (c.mode == 1) launch_sweep_interface_t<T, 1>(B, Bul, k, pi, v, g, scr0, scr1, c, rh, st);
     else                  launch_sweep_interface_t<T, 0>(B, Bul, k, pi, v, g, scr0, scr1, c, rh, st);
@@ -1044,7 +1045,7 @@ template <class T>
 void launch_sweep_full(const T* B, const T* Bul, int k, PartInfo pi, int last_is_ul, const double* v, const T* corr,
                        double* outd, T* outt, T* scr, hipStream_t st, const Rhs& rh)
 {
-    const SweepCfg c = sweep_cfg<T>(k, rh.nbuf);
+    const SweepCfg c = sweep_cfg<T>(k, rh.nr > 1 ? 4 : rh.nbuf);
     if (c.mode == 2)      launch_sweep_full_t<T, 2>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, c, rh, st);
     else if (c.mode == 1) launch_sweep_full_t<T, 1>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, c, rh, st);
     else                  launch_sweep_full_t<T, 0>(B, Bul, k, pi, last_is_ul, v, corr, outd, outt, scr, c, rh, st);
@@ -1065,7 +1066,9 @@ bool sweep_jobs_append(std::vector<SweepJob>& jobs, size_t& lds, const T* B, con
                        int last_is_ul, const T* corr, T* scr, int64_t voff, int64_t ldv, int64_t ooff, int64_t ldo,
                        int nr, int gate0, int nbuf_req)
 {
-    const SweepCfg c = sweep_cfg<T>(k, nbuf_req);
+    // many workgroups: the fewest staging buffers, so more fit on a CU
+    (void)nbuf_req;
+    const SweepCfg c = sweep_cfg<T>(k, 4);
     if (c.mode != 2 || k == 0) return false;
     lds = std::max(lds, c.lds);
     // the kernel's (job, right-hand side) order per system: the nr
