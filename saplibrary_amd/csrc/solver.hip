@@ -1425,7 +1425,6 @@ int sap_setup_csr(sap_solver* h, int n, int nnz, const int* d_row_offsets, const
         h->err = "bad matrix arguments";
         return SAP_ERR_BADARG;
     }
-    const sap_options& o = h->s->opt;
     return guarded(h, [&] { return h->s->setup_csr(n, nnz, d_row_offsets, d_col_indices, d_values); });
 }
 

@@ -206,19 +206,21 @@ def test_system_batch_equals_single_systems_bitwise(sap, oracle, specs, block, f
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_autotuned_launches_give_the_same_bits(sap, oracle, prec):
+@pytest.mark.parametrize("block", [0, 1])
+def test_autotuned_launches_give_the_same_bits(sap, oracle, prec, block):
     """options.autotune = 1 times the conversion tile widths and the sweep
     staging depths on the first setup of a half-bandwidth and keeps the
     fastest; neither changes the arithmetic, so factors, M^{-1} v and the
-    solution equal the untuned run bit for bit, and the choices are listed."""
-    n, k, P = 20000, 100 + prec, 8
-    band = oracle.gen_banded(n, k, 0.8, 501)
+    solution equal the untuned run bit for bit, and the choices are listed
+    (for the SaP-C and the block-Jacobi preconditioners)."""
+    n, k, P = 20000, 100 + prec - 4 * block, 8
+    band = oracle.gen_banded(n, k, 0.8 + block, 501)
     A = sap.BandedMatrix(dev(band))
     b = dev(oracle.gen_vector(n, 502))
     out = []
     for tune in (0, 1):
         g = sap.Solver(P, sap.Options(factMethod=sap.LU_UL, relTol=1e-10, refine=6, autotune=tune,
-                                      precValueType=prec))
+                                      precValueType=prec, precondType=sap.Block if block else sap.Spike))
         g.setup(A)
         z = torch.empty(n, dtype=torch.float64, device="cuda")
         g.precond_apply(b, z)
@@ -229,6 +231,6 @@ def test_autotuned_launches_give_the_same_bits(sap, oracle, prec):
         np.testing.assert_array_equal(a, c)
     ent = [e for e in sap.autotune_entries() if e["k"] == k and e["elem_bytes"] == (8 if prec else 4)]
     fams = {e["family"] for e in ent}
-    assert 0 in fams and 1 in fams
+    assert fams == {0, 1}
     for e in ent:
         assert e["candidates"] >= 2 and e["best_ms"] > 0 and e["best_ms"] <= e["default_ms"] + 1e-9

@@ -69,7 +69,8 @@ def test_csr_spmv(sap, n, per_row):
     np.testing.assert_allclose(host(y), ref, rtol=0, atol=1e-13 * np.abs(ref).max())
 
 
-@pytest.mark.parametrize("n,k,P,fact", [(3000, 20, 4, "LU_UL"), (4000, 50, 5, "LU_only"), (500, 3, 1, "LU_UL")])
+@pytest.mark.parametrize("n,k,P,fact", [(3000, 20, 4, "LU_UL"), (6000, 80, 4, "LU_UL"), (4000, 50, 5, "LU_only"),
+                                         (500, 3, 1, "LU_UL")])
 def test_csr_without_reordering_is_the_banded_setup(sap, oracle, n, k, P, fact):
     band = oracle.gen_banded(n, k, 0.9, 41)
     rp, ci, val = csr_of_band(band)
