@@ -2,6 +2,8 @@
 // row-major FP64 band -> column-major preconditioner band conversion.
 #include "sap_internal.hpp"
 
+#include <atomic>
+
 #include <algorithm>
 
 namespace sap_amd {
@@ -269,6 +271,126 @@ __global__ __launch_bounds__(B2C_THREADS) void band_to_colmajor_cols_kernel(int6
     }
 }
 
+// Persistent, software-pipelined form of the column-tile conversion (tiles
+// of tc <= 64 columns, k small enough that a tile's tc + 2k input rows fit
+// RPW rows per wavefront): each workgroup converts a run of tiles, and the
+// row segments of its next tile are loaded into registers while the current
+// tile goes from LDS to memory, so the loads of one tile and the stores of
+// the previous one overlap (the non-persistent kernel's workgroups, one per
+// CU for its LDS, alternate between the two).  Tiles are dealt to the
+// workgroups XCD by XCD (workgroup b runs on XCD b mod 8): each XCD converts
+// one contiguous eighth of the band, so the cache lines two neighbouring
+// tiles share at their segment ends come through one L2.  The stores are
+// 16-byte vectors of the contiguous output run.  Same values as
+// band_to_colmajor_cols_kernel, element for element.
+// ---------------------------------------------------------------------------
+template <class T, int RPW>
+__global__ __launch_bounds__(B2C_THREADS) void band_to_colmajor_pipe_kernel(int64_t n, int k, int tc, int64_t nchunks,
+                                                                        const double* __restrict__ in,
+                                                                        T* __restrict__ out, T* __restrict__ out2)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T* const lds = reinterpret_cast<T*>(smem_raw);
+    constexpr int NWV = B2C_THREADS / 64;
+    constexpr int V = 16 / (int)sizeof(T);   // elements per 16-byte store
+    const int w = 2 * k + 1, ls = w + 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // tiles of this workgroup: XCD x = b mod 8 owns tiles [lo, hi), its
+    // per = G / 8 workgroups take every per-th of them
+    const int G = gridDim.x, b = blockIdx.x;
+    const int x = b & 7, slot = b >> 3, per = G >> 3;
+    const int64_t lo = nchunks * x / 8, hi = nchunks * (x + 1) / 8;
+    int64_t ch = lo + slot;
+    if (ch >= hi) return;   // uniform over the workgroup
+
+    // Row rr of a tile (input row c0 - k + rr) gives the tile the segment
+    // o in [ob, ob + len) with ob = max(0, 2k - rr): lane l loads offset
+    // ob + l, so every per-row quantity but the lane is uniform, and the
+    // loads are buffer loads relative to the tile's first input row (lanes
+    // past the segment, and rows outside the matrix, read zero through an
+    // offset past the buffer's end).
+    double v[RPW];
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    auto load = [&](int64_t c) {
+        const int64_t c0 = c * tc;
+        const int ncol = (int)min((int64_t)tc, n - c0);
+        const int nrow = ncol + 2 * k;
+        const int64_t rlo = max((int64_t)0, c0 - k), rhi = min(n, c0 + ncol + k);
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(in + rlo * w), 0, (int)((rhi - rlo) * w * (int64_t)sizeof(double)), 0x00020000);
+#pragma unroll
+        for (int u = 0; u < RPW; ++u) {
+            const int rr = wv + NWV * u;
+            const int64_t r = c0 - k + rr;
+            const int ob = max(0, 2 * k - rr);
+            const int len = min(w, 2 * k + ncol - rr) - ob;
+            const bool row_ok = rr < nrow && r >= 0 && r < n;
+            const int base = row_ok ? (int)((r - rlo) * w + ob) : 0;
+            const int off = (row_ok && lane < len) ? (base + lane) * (int)sizeof(double) : (int)0x80000000;
+            const auto q = __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0);
+            v[u] = __longlong_as_double((long long)(((unsigned long long)q[1] << 32) | q[0]));
+        }
+    };
+    load(ch);
+    for (;;) {
+        const int64_t c0 = ch * tc;
+        const int ncol = (int)min((int64_t)tc, n - c0);
+        const int nrow = ncol + 2 * k;
+        // segment element ob + l of row rr is column rr - 2k + ob + l,
+        // diagonal 2k - ob - l: LDS index (rr - 2k + ob) ls + 2k - ob + l w
+#pragma unroll
+        for (int u = 0; u < RPW; ++u) {
+            const int rr = wv + NWV * u;
+            const int ob = max(0, 2 * k - rr);
+            const int len = min(w, 2 * k + ncol - rr) - ob;
+            if (rr < nrow && lane < len) lds[(rr - 2 * k + ob) * ls + 2 * k - ob + lane * w] = (T)v[u];
+        }
+        __syncthreads();
+        const int64_t nx = ch + per;
+        if (nx < hi) load(nx);
+        // the run of ncol * w elements: element e is column e / w, diagonal
+        // e % w, at LDS index e + e / w (column stride w + 1)
+        const int tot = ncol * w;
+        T* const dst = out + c0 * w;
+        T* const dst2 = out2 ? out2 + c0 * w : nullptr;
+        const int nvec = tot / V;
+        constexpr int STEP = B2C_THREADS * V;
+        const int qs = STEP / w, rs = STEP - qs * w;
+        int e = (int)threadIdx.x * V;
+        int cc = e / w, d = e - cc * w;
+        for (int q = threadIdx.x; q < nvec; q += B2C_THREADS) {
+            T val[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                const int di = d + i;
+                val[i] = lds[cc * ls + di + (di >= w ? 1 : 0)];
+            }
+            if constexpr (V == 4) {
+                const float4 f = make_float4(val[0], val[1], val[2], val[3]);
+                *reinterpret_cast<float4*>(dst + e) = f;
+                if (dst2) *reinterpret_cast<float4*>(dst2 + e) = f;
+            } else {
+                const double2 f = make_double2(val[0], val[1]);
+                *reinterpret_cast<double2*>(dst + e) = f;
+                if (dst2) *reinterpret_cast<double2*>(dst2 + e) = f;
+            }
+            e += STEP;
+            cc += qs;
+            d += rs;
+            if (d >= w) { d -= w; ++cc; }
+        }
+        for (int t = nvec * V + (int)threadIdx.x; t < tot; t += B2C_THREADS) {
+            const int tc_ = t / w;
+            const T val = lds[t + tc_];
+            dst[t] = val;
+            if (dst2) dst2[t] = val;
+        }
+        __syncthreads();   // the LDS tile is free for the next scatter
+        if (nx >= hi) break;
+        ch = nx;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // CSR entries into the zero-filled column-major band, cast to the band type
 // (the COO scatter copyFromCOOMatrixToBandedMatrix,
@@ -476,6 +598,22 @@ void launch_csr_spmm(int64_t n, int64_t nnz, const int* rp, const int* ci, const
     SAP_LAUNCH_CHECK();
 }
 
+// compute units of the current device (cached per device)
+int device_cus()
+{
+    static std::atomic<int> cus[64] = {};
+    int dev = 0;
+    SAP_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) dev = 0;
+    int v = cus[dev].load(std::memory_order_relaxed);
+    if (v == 0) {
+        SAP_HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+        if (v <= 0) v = 256;
+        cus[dev].store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
+
 template <class T>
 int band_to_colmajor_max_tile(int k)
 {
@@ -491,7 +629,22 @@ void launch_band_to_colmajor(int64_t n, int k, const double* band_rm, T* B, hipS
     // bands too wide for 8 columns of LDS use the diagonal-box kernel
     int tc = band_to_colmajor_max_tile<T>(k);
     if (tile >= 8 && tile < tc) tc = tile;
-    if (tc >= 8) {
+    const int rows = tc + 2 * k;   // input rows of one tile
+    constexpr int V = 16 / (int)sizeof(T);
+    const bool aligned = ((uintptr_t)B % 16 == 0) && ((uintptr_t)B2 % 16 == 0);
+    if (tc >= 8 && rows <= 32 * (B2C_THREADS / 64) && tc % V == 0 && w >= V && aligned) {
+        const size_t lds = (size_t)tc * (w + 1) * sizeof(T);
+        const int64_t nchunks = (n + tc - 1) / tc;
+        const int G = 8 * std::max(1, std::min(device_cus() / 8, (int)((nchunks + 7) / 8)));
+        const int rpw = (rows + B2C_THREADS / 64 - 1) / (B2C_THREADS / 64);
+        auto go = [&](auto kern) {
+            SAP_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(kern, dim3((unsigned)G), dim3(B2C_THREADS), lds, st, n, k, tc, nchunks, band_rm, B, B2);
+        };
+        if (rpw <= 8)       go(band_to_colmajor_pipe_kernel<T, 8>);
+        else if (rpw <= 16) go(band_to_colmajor_pipe_kernel<T, 16>);
+        else                go(band_to_colmajor_pipe_kernel<T, 32>);
+    } else if (tc >= 8) {
         const size_t lds = (size_t)tc * (w + 1) * sizeof(T);
         SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_to_colmajor_cols_kernel<T>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

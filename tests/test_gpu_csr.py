@@ -69,16 +69,21 @@ def test_csr_spmv(sap, n, per_row):
     np.testing.assert_allclose(host(y), ref, rtol=0, atol=1e-13 * np.abs(ref).max())
 
 
-@pytest.mark.parametrize("n,k,P,fact", [(3000, 20, 4, "LU_UL"), (6000, 80, 4, "LU_UL"), (4000, 50, 5, "LU_only"),
-                                         (500, 3, 1, "LU_UL")])
-def test_csr_without_reordering_is_the_banded_setup(sap, oracle, n, k, P, fact):
+@pytest.mark.parametrize("n,k,P,fact,prec", [(3000, 20, 4, "LU_UL", 0), (6000, 80, 4, "LU_UL", 0),
+                                              (4000, 50, 5, "LU_only", 0), (500, 3, 1, "LU_UL", 0),
+                                              (5000, 60, 4, "LU_UL", 1), (3000, 25, 3, "LU_only", 1)])
+def test_csr_without_reordering_is_the_banded_setup(sap, oracle, n, k, P, fact, prec):
+    """The band converted from the row-major input (column tiles through
+    LDS) and the band scattered from the CSR entries are the same band: the
+    factors and M^{-1} v agree bit for bit."""
     band = oracle.gen_banded(n, k, 0.9, 41)
     rp, ci, val = csr_of_band(band)
     f = sap.LU_UL if fact == "LU_UL" else sap.LU_only
-    opts = sap.Options(factMethod=f, performReorder=0, performDB=0, applyScaling=0, variableBandwidth=0)
+    opts = sap.Options(factMethod=f, performReorder=0, performDB=0, applyScaling=0, variableBandwidth=0,
+                       precValueType=prec)
     g_csr = sap.Solver(P, opts)
     g_csr.setup(sap.CsrMatrix.from_host(rp, ci, val))
-    g_band = sap.Solver(P, sap.Options(factMethod=f))
+    g_band = sap.Solver(P, sap.Options(factMethod=f, precValueType=prec))
     g_band.setup(sap.BandedMatrix(dev(band)))
     st = g_csr.getStats()
     assert st.bandwidth == k and st.bandwidthDB == 0 and st.bandwidthReorder == 0
