@@ -416,8 +416,11 @@ int         sap_get_profile(const sap_solver* s, sap_kernel_profile* out, int ma
 void        sap_reset_profile(sap_solver* s);
 
 /* Diagnostics for tests: copies of the factored band (column-major,
- * (2k+1) x n, as FP64), of the K x K spike tips (2 k^2 (P-1)) and of the
- * Schur blocks S_i = I - W_i V_i after factorization (k^2 (P-1)); host out. */
+ * (2k+1) x n, as FP64; after a variable-bandwidth setup the partitions' own
+ * bands one after the other, partition p (2 K_p + 1) x n_p in second-level
+ * order, K_p = max(1, the partition's half-bandwidth)), of the K x K spike
+ * tips (2 k^2 (P-1)) and of the Schur blocks S_i = I - W_i V_i after
+ * factorization (k^2 (P-1)); host out. */
 int         sap_get_factors(const sap_solver* s, double* h_out, int64_t count);
 int         sap_get_spikes(const sap_solver* s, double* h_wv, double* h_schur, int64_t count);
 int         sap_num_partitions(const sap_solver* s);

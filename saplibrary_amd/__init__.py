@@ -615,6 +615,16 @@ class Solver:
         _check(lib().sap_get_factors(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size), self.h)
         return out
 
+    def factors_flat(self, count):
+        """The factored storage as a flat array of count values: on the
+        variable-bandwidth path the partitions' own bands one after the other
+        (partition p in a (2 K_p + 1) x n_p column-major band, second-level
+        order)."""
+        import numpy as np
+        out = np.empty(count)
+        _check(lib().sap_get_factors(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size), self.h)
+        return out
+
     def spikes(self):
         """(WV, S): spike tips (P-1, 2, k, k) stored RHS-major and the factored
         Schur blocks (P-1, k, k) stored column-major."""

@@ -23,11 +23,12 @@ namespace {
 // the rank-NB product from LDS.
 // ---------------------------------------------------------------------------
 template <class T, int NB>
-__global__ __launch_bounds__(256) void band_factor_kernel(T* __restrict__ B0, T* __restrict__ B1, int k,
+__global__ __launch_bounds__(256) void band_factor_kernel(T* __restrict__ B0, T* __restrict__ B1, int k_all,
                                                           const FactorJob* __restrict__ jobs)
 {
     const FactorJob job = jobs[blockIdx.x];
-    T* __restrict__ base = job.buf ? B1 : B0;
+    const int k = job.kb > 0 ? job.kb : k_all;
+    T* __restrict__ base = (job.buf ? B1 : B0) + job.boff;
     const int64_t w = 2 * k + 1;
     const int64_t len = job.len;
     const int dir = job.dir;
@@ -196,13 +197,14 @@ struct MfmaLds {
 };
 
 template <class T, int RPT>
-__global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B0, T* __restrict__ B1, int k,
+__global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B0, T* __restrict__ B1, int k_all,
                                                                const FactorJob* __restrict__ jobs)
 {
     using MF = Mfma<T>;
     constexpr int TILE = MF::TILE;
     const FactorJob job = jobs[blockIdx.x];
-    T* __restrict__ base = job.buf ? B1 : B0;
+    const int k = job.kb > 0 ? job.kb : k_all;
+    T* __restrict__ base = (job.buf ? B1 : B0) + job.boff;
     const int64_t w = 2 * k + 1;
     const int64_t len = job.len;
     const int dir = job.dir;
@@ -1159,6 +1161,17 @@ void launch_factor_mfma(T* B0, T* B1, int k, const FactorJob* jobs, int njobs, h
 }
 
 template <class T>
+int band_factor_kernel_class(int k, int64_t max_len)
+{
+    if (band_factor_resident_fits<T>(k, max_len)) return (k + 32 + 15) / 16;   // resident, by window tiles
+    if (MfmaLds<T>(k).bytes <= LDS_LIMIT && k + MNB <= 256) return 100;
+    if (factor_lds_bytes<T>(k, 32) <= LDS_LIMIT) return 132;
+    if (factor_lds_bytes<T>(k, 16) <= LDS_LIMIT) return 116;
+    if (factor_lds_bytes<T>(k, 8) <= LDS_LIMIT) return 108;
+    return -1;
+}
+
+template <class T>
 void launch_band_factor(T* B0, T* B1, const T* Bin, int k, const FactorJob* d_jobs, int njobs, int64_t max_len,
                         hipStream_t st)
 {
@@ -1295,6 +1308,7 @@ int64_t count_band_zeros(const T* A, int64_t na, const T* B, int64_t nb, unsigne
     template void count_band_zeros_async<T>(const T*, int64_t, const T*, int64_t, unsigned long long*,        \
                                             unsigned long long*, hipStream_t, int);                           \
     template void launch_band_factor<T>(T*, T*, const T*, int, const FactorJob*, int, int64_t, hipStream_t);                     \
+    template int band_factor_kernel_class<T>(int, int64_t);                                                   \
     template void launch_zero_pivot_check<T>(const T*, int, int64_t, int64_t, int*, hipStream_t);             \
     template void launch_extract_coupling<T>(const T*, int, PartInfo, T*, T*, hipStream_t);                   \
     template void launch_spike_tips<T>(const T*, const T*, int, PartInfo, int, const T*, const T*, T*,        \

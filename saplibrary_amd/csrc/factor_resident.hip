@@ -174,7 +174,7 @@ __device__ inline double pick(double a, double b, int m)
 }
 template <class T, int KT, int NW>
 __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __restrict__ B0, T* __restrict__ B1,
-                                                                   const T* __restrict__ Bin, int k,
+                                                                   const T* __restrict__ Bin, int k_all,
                                                                    const FactorJob* __restrict__ jobs)
 {
     using MF = Mfma16<T>;
@@ -186,7 +186,8 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
     static_assert(NTR <= UOFF && UOFF + NTR <= NT, "B2 thread ranges");
     static_assert(SPW <= 32, "slot class masks are 32-bit");
     const FactorJob job = jobs[blockIdx.x];
-    T* __restrict__ base = job.buf ? B1 : B0;
+    const int k = job.kb > 0 ? job.kb : k_all;
+    T* __restrict__ base = (job.buf ? B1 : B0) + job.boff;
     const int64_t w = 2 * k + 1;
     const int64_t len = job.len;
     const int dir = job.dir;
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(64 * NW) void band_factor_resident_kernel(T* __rest
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(base + job.start * w), 0, (int)(len * w * (int64_t)sizeof(T)), 0x00020000);
     const __amdgpu_buffer_rsrc_t rsrc_in = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)((Bin ? Bin : base) + job.start * w), 0, (int)(len * w * (int64_t)sizeof(T)), 0x00020000);
+        (void*)((Bin ? Bin + job.boff : base) + job.start * w), 0, (int)(len * w * (int64_t)sizeof(T)), 0x00020000);
     const int rel0 = dir > 0 ? 0 : (int)(len * w - 1);
     // Tiles are held transposed: accumulator register q of lane l holds
     // A(dr0 + l % 16, dc0 + row(l, q)), so for each register the 16 lanes of

@@ -25,7 +25,7 @@ void Precond<T>::factor(const std::vector<FactorJob>& jobs)
     const bool same = jobs.size() == jobs_host_.size() &&
                       std::equal(jobs.begin(), jobs.end(), jobs_host_.begin(), [](const FactorJob& a, const FactorJob& b) {
                           return a.start == b.start && a.len == b.len && a.dir == b.dir && a.boost == b.boost &&
-                                 a.boost_last == b.boost_last && a.buf == b.buf;
+                                 a.boost_last == b.boost_last && a.buf == b.buf && a.kb == b.kb && a.boff == b.boff;
                       });
     if (!same) {
         jobs_.resize(jobs.size());
@@ -81,8 +81,19 @@ void upload(DevBuf<V>& d, const std::vector<V>& h, hipStream_t st)
 
 }  // namespace
 
+namespace {
+
+int clamp_partitions(int64_t n, int k, int requested)
+{
+    const int64_t rows = std::max<int64_t>(k + 1, 2 * (int64_t)k);
+    const int64_t maxP = std::max<int64_t>(1, n / rows);
+    return std::max(1, (int)std::min<int64_t>(requested, maxP));
+}
+
+}  // namespace
+
 template <class T>
-void Precond<T>::begin_setup(int64_t n_, int k_)
+void Precond<T>::begin_setup(int64_t n_, int k_, bool const_band)
 {
     n = n_;
     k = k_;
@@ -94,19 +105,21 @@ void Precond<T>::begin_setup(int64_t n_, int k_)
     // path uses it: with shorter partitions the 2k-row interface windows of
     // neighbouring boundaries overlap, and the reduced solve and
     // purification of one boundary would change the other's values.
-    const int64_t rows = std::max<int64_t>(k + 1, 2 * (int64_t)k);
-    const int64_t maxP = std::max<int64_t>(1, n / rows);
-    P = (int)std::min<int64_t>(cfg.numPartitions, maxP);
-    if (P < 1) P = 1;
+    P = clamp_partitions(n, k, cfg.numPartitions);
     const int64_t w = 2 * k + 1;
     flag_.resize(1);
     // 8 elements of slack: the sweeps copy 16-byte chunks covering each
     // column segment, which may reach a few elements past the last column
     auto alloc_band = [&](DevBuf<T>& b) { b.resize((size_t)w * n + 8); };
-    alloc_band(B_);
+    if (const_band) {
+        alloc_band(B_);
+        Bv_.release();
+    } else {
+        B_.release();   // the variable-bandwidth bands replace it
+    }
     // LU_UL factors the UL of partitions 1..P-1 on a second copy of the band
     // (kept across setups of the same size)
-    const bool need_ul = lu_ul() && cfg.precondType == SAP_Spike && P > 1 && k > 0;
+    const bool need_ul = const_band && cfg.factMethod == SAP_LU_UL && cfg.precondType == SAP_Spike && P > 1 && k > 0;
     if (need_ul) alloc_band(Bul_);
     else Bul_.release();
     // the first partition is the longest (PartInfo gives the n mod P extra
@@ -294,32 +307,105 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
     //     in-place reduced sweeps, sap/device/sweep_band_var.cuh:21-191, then
     //     race), so the banded path's clamp n / max(k+1, 2k) is used here
     //     too, and the assembly below uses the same partition count.
-    begin_setup(n_, R.k);
-    const int Pvar = P;
-    var_mode_ = rc.variable && Pvar > 1 && R.k > 0;
+    var_mode_ = rc.variable && R.k > 0 && clamp_partitions(n_, R.k, cfg.numPartitions) > 1;
+    begin_setup(n_, R.k, !var_mode_);
+    const PartInfo pi{n, P};
 
     // (4) band of the reordered matrix, vectors of the permutations
     t = wall_now_ms();
+    VarBand<T> VB;
     if (var_mode_) {
-        const VarBand<T> VB = assemble_var((int)n, R.k, Pvar, R.rp, R.ci, R.val);
+        VB = assemble_var((int)n, R.k, P, R.rp, R.ci, R.val);
         profile = BandProfile{};
         for (int i = 0; i < (int)n; ++i) {
             profile.sum_row += VB.ks_row[i];
             profile.sum_col += VB.ks_col[i];
             profile.sum_prod += (double)VB.ks_row[i] * VB.ks_col[i];
         }
+        // storage widths: a partition without off-diagonal entries keeps a
+        // band of width 3 (zeros beside the diagonal) so every kernel sees
+        // k >= 1.  Each band starts 256-byte aligned or better (the staged
+        // sweeps copy 16-byte chunks aligned relative to the band's start).
+        vks_.resize(P);
+        vboff_.assign(P + 1, 0);
+        for (int p = 0; p < P; ++p) {
+            int64_t s, len;
+            pi.bounds(p, s, len);
+            vks_[p] = std::max(VB.ks[p], 1);
+            vboff_[p + 1] = (vboff_[p] + len * (2 * (int64_t)vks_[p] + 1) + 63) / 64 * 64;
+        }
     } else {
         profile = band_profile((int)n, R.rp.data(), R.ci.data(), P);
+        vks_.clear();
+        vboff_.clear();
     }
     time_cpu_assemble = wall_now_ms() - t;
     t = wall_now_ms();
     DevBuf<int> rp_d, ci_d;
     DevBuf<T> val_d;
-    upload(rp_d, R.rp, stream);
-    upload(ci_d, R.ci, stream);
-    upload(val_d, R.val, stream);
-    upload(row_perm_, R.row_perm, stream);
-    upload(col_reord_, R.col_reord, stream);
+    if (var_mode_) {
+        // the second-level permutation folded into the vector permutations
+        // (sap/precond.h:2070-2080): leftTrans lands in second-level order,
+        // rightTrans reads from it
+        std::vector<int> rperm(n), creord(n);
+        for (int64_t i = 0; i < n; ++i) rperm[i] = VB.second_perm[R.row_perm[i]];
+        for (int64_t i = 0; i < n; ++i) creord[i] = R.col_reord[VB.second_reorder[i]];
+        upload(row_perm_, rperm, stream);
+        upload(col_reord_, creord, stream);
+        upload(rp_d, VB.drp, stream);
+        upload(ci_d, VB.dci, stream);
+        upload(val_d, VB.dval, stream);
+        upload(vks_d_, vks_, stream);
+        upload(vboff_d_, vboff_, stream);
+        upload(sp_, VB.second_perm, stream);
+        upload(sr_, VB.second_reorder, stream);
+        upload(vperm_r_, VB.perms_right, stream);
+        upload(vperm_l_, VB.perms_left, stream);
+        // packed column counts and spike solve ranges per (boundary, side)
+        std::vector<int> widths(2 * (P - 1));
+        std::vector<VarSpikeRangeH> ranges(2 * (P - 1));
+        vmaxw_ = 0;
+        for (int i = 0; i < P - 1; ++i) {
+            int64_t s0, l0, s1, l1;
+            pi.bounds(i, s0, l0);
+            pi.bounds(i + 1, s1, l1);
+            widths[2 * i] = VB.widths_right[i];
+            widths[2 * i + 1] = VB.widths_left[i];
+            vmaxw_ = std::max({vmaxw_, VB.widths_right[i], VB.widths_left[i]});
+            // V_i: forward from the first second-level row holding a
+            // coupling entry (first_rows, sap/graph.h:994-1007), backward
+            // down to the lowest of the last k rows (sap/precond.h:5650)
+            int mn0 = (int)l0, mn1 = (int)l1;
+            for (int t2 = 0; t2 < k; ++t2) {
+                mn0 = std::min(mn0, VB.second_perm[s1 - k + t2] - (int)s0);
+                mn1 = std::min(mn1, VB.second_perm[s1 + t2] - (int)s1);
+            }
+            ranges[2 * i] = {VB.first_rows[i] - (int)s0, mn0};
+            // W_{i+1}: the coupling rows are its first k rows
+            ranges[2 * i + 1] = {mn1, mn1};
+        }
+        upload(vwidth_, widths, stream);
+        upload(vrange_, ranges, stream);
+        // the coupling blocks, column-major k x k after the bands
+        const size_t kk = (size_t)k * k;
+        Bv_.resize((size_t)vboff_[P] + 2 * kk * (P - 1) + 8);
+        std::vector<T> cb(2 * kk * (P - 1));
+        for (int i = 0; i < P - 1; ++i)
+            for (int a = 0; a < k; ++a)
+                for (int b = 0; b < k; ++b) {
+                    cb[i * kk + (size_t)b * k + a] = VB.offDiags[(size_t)2 * i * kk + (size_t)a * k + b];
+                    cb[(P - 1) * kk + i * kk + (size_t)b * k + a] =
+                        VB.offDiags[(size_t)(2 * i + 1) * kk + (size_t)a * k + b];
+                }
+        vcoupling_h_.swap(cb);
+    } else {
+        upload(rp_d, R.rp, stream);
+        upload(ci_d, R.ci, stream);
+        upload(val_d, R.val, stream);
+        upload(row_perm_, R.row_perm, stream);
+        upload(col_reord_, R.col_reord, stream);
+        vcoupling_h_.clear();
+    }
     upload(row_scale_, R.row_scale, stream);
     upload(col_scale_, R.col_scale, stream);
     src_ = Source::csr_reordered;
@@ -330,10 +416,33 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
         // was dropped or was an explicit zero)
         const int64_t w = 2 * (int64_t)k + 1;
         map.assign(nnz, -1);
+        const size_t kk = (size_t)k * k;
         for (int r = 0; r < (int)n; ++r)
             for (int e = R.rp[r]; e < R.rp[r + 1]; ++e) {
                 const int64_t c = R.ci[e];
-                map[R.entry_orig[e]] = c * w + k + r - c;
+                if (!var_mode_) {
+                    map[R.entry_orig[e]] = c * w + k + r - c;
+                    continue;
+                }
+                // variable bandwidth: the partition's own band (second-level
+                // order) or a coupling block (first-level order)
+                const int pr = pi.part_of(r), pc = pi.part_of(c);
+                int64_t slot;
+                if (pr == pc) {
+                    int64_t s, len;
+                    pi.bounds(pr, s, len);
+                    const int64_t r2 = VB.second_perm[r], c2 = VB.second_perm[c];
+                    slot = vboff_[pr] + (c2 - s) * (2 * (int64_t)vks_[pr] + 1) + vks_[pr] + (r2 - c2);
+                } else {
+                    const int i = std::min(pr, pc);
+                    int64_t s1, l1;
+                    pi.bounds(i + 1, s1, l1);
+                    if (pc > pr)   // B_i(a, b), a = r - (s1 - k), b = c - s1
+                        slot = vboff_[P] + (int64_t)(i * kk) + (c - s1) * k + (r - (s1 - k));
+                    else           // C_{i+1}(a, b), a = r - s1, b = c - (s1 - k)
+                        slot = vboff_[P] + (int64_t)((P - 1 + i) * kk) + (c - (s1 - k)) * k + (r - s1);
+                }
+                map[R.entry_orig[e]] = slot;
             }
         upload(upd_map_, map, stream);
         if (rc.scale && !R.entry_scale.empty()) upload(upd_scale_, R.entry_scale, stream);
@@ -352,10 +461,19 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
     {
         ProfScope ps(prof, "csr_to_band");
         SAP_HIP_CHECK(hipMemsetAsync(band_in(), 0, band_in_bytes(), stream));
-        launch_csr_to_band<T, T>(n, k, rp_d.get(), ci_d.get(), val_d.get(), band_in(), stream);
+        if (var_mode_) {
+            launch_csr_to_varband<T>(pi, rp_d.get(), ci_d.get(), val_d.get(), vks_d_.get(), vboff_d_.get(), Bv_.get(),
+                                     stream);
+            if (!vcoupling_h_.empty())
+                SAP_HIP_CHECK(hipMemcpyAsync(voffB(), vcoupling_h_.data(), vcoupling_h_.size() * sizeof(T),
+                                             hipMemcpyHostToDevice, stream));
+        } else {
+            launch_csr_to_band<T, T>(n, k, rp_d.get(), ci_d.get(), val_d.get(), band_in(), stream);
+        }
     }
     SAP_HIP_CHECK(hipEventRecord(ev.e[1], stream));
     time_toBanded = elapsed_ms(ev.e[0], ev.e[1]);   // waits for the fill before the uploads are freed
+    vcoupling_h_ = std::vector<T>();
     const int rc1 = factor_band();
     if (rc1 == 0) tune_sweeps();
     return rc1;
@@ -407,9 +525,198 @@ int Precond<T>::factor_band()
     return rc;
 }
 
+// Variable-bandwidth setup after the fill (sap/precond.h:4546-4631 LU of the
+// partitions in their own bands, :5540-5790 spikes, :2549-2611 reduced
+// blocks): every pivot boosted, as in the reference's variable-bandwidth
+// kernels.
+template <class T>
+int Precond<T>::var_factor_steps()
+{
+    const PartInfo pi{n, P};
+    cnt_.resize(1);
+    scr0_.resize(n);
+    split_ = false;
+    vjobs_[0].n = vjobs_[1].n = 0;   // the sweep job tables are built again for these bands
+    offB_.release(); offC_.release(); Bul_.release(); scr1_.release();
+    // one launch per factorization kernel; the jobs of a launch keep their
+    // own widths (FactorJob::kb) and band offsets
+    std::vector<std::pair<int, int>> order;   // (kernel class, partition)
+    for (int p = 0; p < P; ++p) {
+        int64_t s, len;
+        pi.bounds(p, s, len);
+        const int cls = band_factor_kernel_class<T>(vks_[p], len);
+        if (cls < 0) throw Error(SAP_ERR_UNSUPPORTED, "half-bandwidth too large for the banded factorization kernel");
+        order.push_back({cls, p});
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.first < b.first; });
+    std::vector<FactorJob> jobs;
+    for (const auto& o : order) {
+        int64_t s, len;
+        pi.bounds(o.second, s, len);
+        FactorJob J{0, len, 1, 1, 1, 0, vks_[o.second], vboff_[o.second]};
+        jobs.push_back(J);
+    }
+    const bool same = jobs.size() == jobs_host_.size() &&
+                      std::equal(jobs.begin(), jobs.end(), jobs_host_.begin(), [](const FactorJob& a, const FactorJob& b) {
+                          return a.start == b.start && a.len == b.len && a.dir == b.dir && a.boost == b.boost &&
+                                 a.boost_last == b.boost_last && a.buf == b.buf && a.kb == b.kb && a.boff == b.boff;
+                      });
+    if (!same) {
+        jobs_.resize(jobs.size());
+        SAP_HIP_CHECK(hipMemcpyAsync(jobs_.get(), jobs.data(), jobs.size() * sizeof(FactorJob), hipMemcpyHostToDevice,
+                                     stream));
+        jobs_host_ = jobs;
+    }
+    tm_.mark(3, stream);
+    {
+        ProfScope ps(prof, "band_factor");
+        size_t a = 0;
+        while (a < order.size()) {
+            size_t b = a;
+            int kmax = 0;
+            int64_t max_len = 0;
+            while (b < order.size() && order[b].first == order[a].first) {
+                kmax = std::max(kmax, jobs[b].kb);
+                max_len = std::max(max_len, jobs[b].len);
+                ++b;
+            }
+            launch_band_factor<T>(Bv_.get(), Bv_.get(), nullptr, kmax, jobs_.get() + a, (int)(b - a), max_len, stream);
+            a = b;
+        }
+    }
+    tm_.mark(4, stream);
+    tm_.span(&time_bandLU, 3, 4);
+    actual_nnz = vboff_[P] - count_band_zeros<T>(Bv_.get(), vboff_[P], (const T*)nullptr, 0, cnt_.get(), stream);
+
+    if (!spike_active()) {
+        WV_.release(); S_.release(); Sinv_.release(); corr_.release(); g_.release(); gd_.release(); vq_.release();
+        return 0;
+    }
+    const size_t kk = (size_t)k * k;
+    g_.resize(n);
+    gd_.resize(n);
+    vq_.resize(n);
+    WV_.resize(2 * kk * (P - 1));
+    tm_.mark(5, stream);
+    {
+        ProfScope ps(prof, "spike_tips");
+        SAP_HIP_CHECK(hipMemsetAsync(WV_.get(), 0, WV_.size() * sizeof(T), stream));
+        launch_var_spikes<T>(pi, k, Bv_.get(), vks_d_.get(), vboff_d_.get(), sp_.get(), voffB(), voffC(),
+                             vperm_r_.get(), vperm_l_.get(), vwidth_.get(), vrange_.get(), vmaxw_, WV_.get(), stream);
+    }
+    tm_.mark(6, stream);
+    S_.resize(kk * (P - 1));
+    Sinv_.resize(kk * (P - 1));
+    corr_.resize((size_t)2 * k * (P - 1));
+    {
+        ProfScope ps(prof, "reduced_factor");
+        launch_reduced_factor<T>(WV_.get(), k, P - 1, S_.get(), Sinv_.get(), cfg.safeFactorization, stream);
+    }
+    tm_.mark(7, stream);
+    tm_.span(&time_assembly, 5, 6);
+    tm_.span(&time_fullLU, 6, 7);
+    return 0;
+}
+
+// Full sweeps (L, D^{-1}, U') of every partition on its own band, second-
+// level order in and out, one launch over all partitions when every width
+// takes the staged sweep.
+template <class T>
+typename Precond<T>::VJobs& Precond<T>::var_jobs(int nr, int64_t ldv, int64_t ldz, T* scr)
+{
+    VJobs& J = vjobs_[nr == 1 ? 0 : 1];
+    const VJobKey key{nr, ldv, ldz, scr};
+    if (key.nr == J.key.nr && key.ldv == J.key.ldv && key.ldz == J.key.ldz && key.scr == J.key.scr && J.n > 0)
+        return J;
+    const PartInfo pi{n, P};
+    std::vector<SweepJob> jobs;
+    size_t lds = 0;
+    bool ok = true;
+    for (int p = 0; p < P && ok; ++p) {
+        int64_t s, len;
+        pi.bounds(p, s, len);
+        const T* Bp = Bv_.get() + vboff_[p];
+        ok = sweep_jobs_append<T>(jobs, lds, Bp, Bp, vks_[p], PartInfo{len, 1}, 0, nullptr, scr + nr * s, s, ldv, s,
+                                  ldz, nr, 0, 0);
+    }
+    J.ok = ok;
+    J.lds = lds;
+    J.n = ok ? jobs.size() : 1;
+    if (ok) {
+        J.d.resize(jobs.size());
+        SAP_HIP_CHECK(hipMemcpyAsync(J.d.get(), jobs.data(), jobs.size() * sizeof(SweepJob), hipMemcpyHostToDevice,
+                                     stream));
+        SAP_HIP_CHECK(hipStreamSynchronize(stream));   // the host table is released on return
+    }
+    J.key = key;
+    return J;
+}
+
+template <class T>
+void Precond<T>::var_sweeps(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate, T* scr)
+{
+    const PartInfo pi{n, P};
+    const VJobs& J = var_jobs(nr, ldv, ldz, scr);
+    if (J.ok) {
+        launch_sweep_full_jobs<T>(J.d.get(), (int)J.n, J.lds, V, Z, gate, stream);
+        return;
+    }
+    // a width the staged sweep does not take: one launch per partition
+    Rhs rh;
+    rh.nr = nr;
+    rh.ldv = ldv;
+    rh.ldo = ldz;
+    rh.gate = gate;
+    for (int p = 0; p < P; ++p) {
+        int64_t s, len;
+        pi.bounds(p, s, len);
+        const T* Bp = Bv_.get() + vboff_[p];
+        launch_sweep_full<T>(Bp, Bp, vks_[p], PartInfo{len, 1}, 0, V + s, nullptr, Z + s, nullptr, scr + nr * s,
+                             stream, rh);
+    }
+}
+
+// getSRev's variable-bandwidth branch (sap/precond.h:1457-1470, then the
+// final sweeps :1525-1526): sweeps of every partition (second-level order),
+// the interface values gathered in first-level order, the reduced solve and
+// purification, the purified right-hand side back in second-level order,
+// sweeps again.  Block: one sweep pass.
+template <class T>
+void Precond<T>::var_solve(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate)
+{
+    const PartInfo pi{n, P};
+    T* scr = nr == 1 ? scr0_.get() : scr0m_.get();
+    if (!spike_active()) {
+        ProfScope ps(prof, "sweep_full");
+        var_sweeps(V, ldv, Z, ldz, nr, gate, scr);
+        return;
+    }
+    double* gd = nr == 1 ? gd_.get() : gdm_.get();
+    double* vq = nr == 1 ? vq_.get() : vqm_.get();
+    T* g = nr == 1 ? g_.get() : gm_.get();
+    T* corr = nr == 1 ? corr_.get() : corrm_.get();
+    {
+        ProfScope ps(prof, "sweep_interface");
+        var_sweeps(V, ldv, gd, n, nr, gate, scr);
+    }
+    {
+        ProfScope ps(prof, "interface_solve");
+        launch_var_gather<T>(pi, k, sp_.get(), gd, n, g, nr, gate, stream);
+        Rhs rh;
+        rh.nr = nr;
+        rh.gate = gate;
+        launch_interface_solve<T>(WV_.get(), Sinv_.get(), voffB(), voffC(), k, pi, g, corr, stream, rh);
+        launch_var_purify<T>(pi, k, sr_.get(), V, ldv, corr, vq, n, nr, gate, stream);
+    }
+    ProfScope ps(prof, "sweep_full");
+    var_sweeps(vq, n, Z, ldz, nr, gate, scr);
+}
+
 template <class T>
 int Precond<T>::factor_band_steps()
 {
+    if (var_mode_) return var_factor_steps();
     const PartInfo pi{n, P};
     const int64_t w = 2 * k + 1;
     // timing events: 2 before the coupling blocks, 3 before and 4 after the
@@ -592,7 +899,8 @@ void Precond<T>::apply(const double* v, double* z)
         ProfScope ps(prof, "permute");
         launch_permute_scale<T>(n, v, row_perm_.get(), scale_ ? row_scale_.get() : nullptr, false, vp_.get(), stream);
     }
-    solve_band(vp_.get(), zp_.get());
+    if (var_mode_) var_solve(vp_.get(), n, zp_.get(), n, 1, nullptr);
+    else solve_band(vp_.get(), zp_.get());
     ProfScope ps(prof, "permute");
     launch_permute_scale<T>(n, zp_.get(), col_reord_.get(), scale_ ? col_scale_.get() : nullptr, true, z, stream);
 }
@@ -636,19 +944,27 @@ void Precond<T>::solve_band(const double* v, double* z)
 template <class T>
 void Precond<T>::reserve_multi(int nr)
 {
-    if (nr <= nr_cap_) return;
-    const size_t m = (size_t)nr;
-    gm_.resize(m * n);
-    scr0m_.resize(m * n);
-    if (spike_active()) {
-        corrm_.resize(m * 2 * k * (P - 1));
-        if (lu_ul()) scr1m_.resize(m * n);
+    if (nr > nr_cap_) {
+        const size_t m = (size_t)nr;
+        gm_.resize(m * n);
+        scr0m_.resize(m * n);
+        if (spike_active()) {
+            corrm_.resize(m * 2 * k * (P - 1));
+            if (lu_ul()) scr1m_.resize(m * n);
+        }
+        if (reorder_) {
+            vpm_.resize(m * n);
+            zpm_.resize(m * n);
+        }
+        if (var_mode_ && spike_active()) {
+            gdm_.resize(m * n);
+            vqm_.resize(m * n);
+        }
+        nr_cap_ = nr;
     }
-    if (reorder_) {
-        vpm_.resize(m * n);
-        zpm_.resize(m * n);
-    }
-    nr_cap_ = nr;
+    // the job table apply_multi's variable-bandwidth sweeps use (input and
+    // output strides n), built here, before any graph capture
+    if (var_mode_ && nr > 1) (void)var_jobs(nr, n, n, scr0m_.get());
 }
 
 template <class T>
@@ -680,7 +996,8 @@ void Precond<T>::apply_multi(const double* V, int64_t ldv, double* Z, int64_t ld
         launch_permute_scale<T>(n, V, row_perm_.get(), scale_ ? row_scale_.get() : nullptr, false, vpm_.get(), stream,
                                 nr, ldv, n, gate);
     }
-    solve_band_multi(vpm_.get(), n, zpm_.get(), n, nr, gate);
+    if (var_mode_) var_solve(vpm_.get(), n, zpm_.get(), n, nr, gate);
+    else solve_band_multi(vpm_.get(), n, zpm_.get(), n, nr, gate);
     ProfScope ps(prof, "permute");
     launch_permute_scale<T>(n, zpm_.get(), col_reord_.get(), scale_ ? col_scale_.get() : nullptr, true, Z, stream, nr,
                             n, ldz, gate);

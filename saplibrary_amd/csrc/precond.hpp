@@ -171,7 +171,11 @@ public:
     bool variable_mode() const { return var_mode_; }
 
     // diagnostics
-    const DevBuf<T>& factors() const { return B_; }
+    // the factored band: (2k+1) n entries, or on the variable-bandwidth path
+    // the per-partition bands (band_offsets().back() entries)
+    const DevBuf<T>& factors() const { return var_mode_ ? Bv_ : B_; }
+    const std::vector<int64_t>& band_offsets() const { return vboff_; }
+    const std::vector<int>& band_widths() const { return vks_; }
     const DevBuf<T>& factors_ul() const { return Bul_; }
     const DevBuf<T>& spikes() const { return WV_; }
     const DevBuf<T>& schur() const { return S_; }
@@ -192,13 +196,43 @@ private:
     // factorizations both read; otherwise both factor in place
     DevBuf<T> Bin_;
     bool sep_in_ = false;
-    T* band_in() const { return sep_in_ ? Bin_.get() : B_.get(); }
-    size_t band_in_bytes() const { return (sep_in_ ? Bin_.size() : B_.size()) * sizeof(T); }
+    T* band_in() const { return var_mode_ ? Bv_.get() : sep_in_ ? Bin_.get() : B_.get(); }
+    size_t band_in_bytes() const { return (var_mode_ ? Bv_.size() : sep_in_ ? Bin_.size() : B_.size()) * sizeof(T); }
     // reordering: row permutation (DB then RCM), column reordering, scalings
     bool reorder_ = false, scale_ = false;
     // variable-bandwidth setup: the preconditioner of the variable-bandwidth
     // path is factored LU_only with every pivot boosted
     bool var_mode_ = false;
+    // variable-bandwidth storage: the partitions' own bands in second-level
+    // order (partition p at vboff_[p], half-bandwidth vks_[p]), then the
+    // coupling blocks B_i and C_{i+1} (k x k column-major each, first-level
+    // order); second-level permutation sp_ (first-level row -> row) and its
+    // inverse sr_; the packed spike columns (offDiagPerms) and their counts
+    DevBuf<T> Bv_;
+    std::vector<int> vks_;
+    std::vector<int64_t> vboff_;
+    DevBuf<int> vks_d_, sp_, sr_, vperm_r_, vperm_l_, vwidth_;
+    DevBuf<int64_t> vboff_d_;
+    DevBuf<VarSpikeRangeH> vrange_;
+    int vmaxw_ = 0;
+    std::vector<T> vcoupling_h_;   // host copy of the coupling blocks (uploaded by setup)
+    T* voffB() const { return Bv_.get() + vboff_.back(); }
+    T* voffC() const { return voffB() + (size_t)k * k * (P - 1); }
+    DevBuf<double> gd_, vq_, gdm_, vqm_;   // first-pass result, purified right-hand side (second-level order)
+    // the full-sweep job tables of the partition bands: [0] for one column,
+    // [1] for apply_multi's columns (built by reserve_multi, so that a
+    // captured apply_multi uploads nothing)
+    struct VJobKey { int nr; int64_t ldv, ldz; const void* scr; };
+    struct VJobs {
+        DevBuf<SweepJob> d;
+        size_t n = 0, lds = 0;
+        bool ok = false;
+        VJobKey key{0, 0, 0, nullptr};
+    } vjobs_[2];
+    VJobs& var_jobs(int nr, int64_t ldv, int64_t ldz, T* scr);
+    int var_factor_steps();
+    void var_sweeps(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate, T* scr);
+    void var_solve(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate);
     DevBuf<int> row_perm_, col_reord_;
     DevBuf<T> row_scale_, col_scale_;
     DevBuf<double> vp_, zp_;
@@ -209,7 +243,7 @@ private:
     const int* src_ci_ = nullptr;
     DevBuf<int64_t> upd_map_;               // input entry -> band slot (-1: dropped)
     DevBuf<T> upd_scale_;                   // DB scale factor per input entry
-    void begin_setup(int64_t n_, int k_);
+    void begin_setup(int64_t n_, int k_, bool const_band = true);
     int factor_band();
     int factor_band_steps();
     void solve_band(const double* v, double* z);

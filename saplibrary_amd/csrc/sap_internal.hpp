@@ -77,6 +77,12 @@ struct PartInfo {
         if (p < r) { s = p * (q + 1); len = q + 1; }
         else       { s = p * q + r;   len = q; }
     }
+    // the partition holding row j
+    __host__ __device__ inline int part_of(int64_t j) const
+    {
+        const int64_t q = n / P, r = n % P;
+        return j < r * (q + 1) ? (int)(j / (q + 1)) : (int)(r + (j - r * (q + 1)) / q);
+    }
 };
 
 // Kernel-family timer (HIP events on the solver stream), used when profiling.
@@ -223,6 +229,9 @@ struct FactorJob {          // one banded LU (dir = +1) or UL (dir = -1) of a pa
     int     boost;          // boost pivots
     int     boost_last;     // boost also the partition's final pivot
     int     buf;            // 0: main factor band, 1: UL band
+    int     kb;             // the job's half-bandwidth (0: the launch's k)
+    int64_t boff;           // element offset of the job's band in its buffer
+                            // (a partition stored in a band of its own width)
 };
 // max_len: longest partition among the jobs.  Bin: the unfactored band the
 // jobs read (the factors go to B0 / B1), or null for in-place factorization;
@@ -238,6 +247,11 @@ bool band_factor_resident_fits(int k, int64_t max_len);
 template <class T>
 bool launch_band_factor_resident(T* B0, T* B1, const T* Bin, int k, const FactorJob* d_jobs, int njobs,
                                  int64_t max_len, hipStream_t st);
+// Which factorization kernel launch_band_factor uses for half-bandwidth k
+// (jobs of one launch with their own widths, FactorJob::kb, must share it;
+// the launch then takes the largest of their widths as its k).
+template <class T>
+int band_factor_kernel_class(int k, int64_t max_len);
 template <class T>
 void launch_zero_pivot_check(const T* B, int k, int64_t row0, int64_t rows, int* d_flag, hipStream_t st);
 template <class T>
@@ -245,6 +259,24 @@ void launch_extract_coupling(const T* B, int k, PartInfo pi, T* offB, T* offC, h
 template <class T>
 void launch_spike_tips(const T* B, const T* Bul, int k, PartInfo pi, int fact_lu_ul,
                        const T* offB, const T* offC, T* WV, hipStream_t st);
+// var.hip: the variable-bandwidth path (per-partition bands in second-level
+// order; see there)
+template <class T>
+void launch_csr_to_varband(PartInfo pi, const int* rp, const int* ci, const T* val, const int* ks,
+                           const int64_t* boff, T* Bv, hipStream_t st);
+struct VarSpikeRangeH {
+    int lo, mn;   // first row of the forward elimination, lowest tip row (partition-local, second level)
+};
+template <class T>
+void launch_var_spikes(PartInfo pi, int k, const T* Bv, const int* ks, const int64_t* boff, const int* sp,
+                       const T* offB, const T* offC, const int* perms_r, const int* perms_l, const int* widths,
+                       const void* ranges, int max_width, T* WV, hipStream_t st);
+template <class T>
+void launch_var_gather(PartInfo pi, int k, const int* sp, const double* gd, int64_t ldgd, T* g, int nr,
+                       const int* gate, hipStream_t st);
+template <class T>
+void launch_var_purify(PartInfo pi, int k, const int* sr, const double* v, int64_t ldv, const T* corr, double* out,
+                       int64_t ldo, int nr, const int* gate, hipStream_t st);
 template <class T>
 void launch_reduced_factor(const T* WV, int k, int nb, T* S, T* Sinv, int safe, hipStream_t st);
 

@@ -1621,6 +1621,31 @@ int sap_get_factors(const sap_solver* h, double* h_out, int64_t count)
     return guarded(const_cast<sap_solver*>(h), [&] {
         auto grab = [&](auto& p) {
             using T = typename std::remove_reference<decltype(*p.factors().get())>::type;
+            if (p.variable_mode()) {
+                // the partitions' own bands, second-level order, one after
+                // the other (the device copy pads each to an aligned start)
+                const std::vector<int64_t>& off = p.band_offsets();
+                const std::vector<int>& ks = p.band_widths();
+                sap_amd::PartInfo pi{S.n, p.P};
+                int64_t total = 0;
+                for (int q = 0; q < p.P; ++q) {
+                    int64_t s, len;
+                    pi.bounds(q, s, len);
+                    total += len * (2 * (int64_t)ks[q] + 1);
+                }
+                if (count < total) throw Error(SAP_ERR_BADARG, "output too small");
+                std::vector<T> tmp((size_t)off.back());
+                SAP_HIP_CHECK(hipMemcpy(tmp.data(), p.factors().get(), sizeof(T) * tmp.size(), hipMemcpyDeviceToHost));
+                int64_t o = 0;
+                for (int q = 0; q < p.P; ++q) {
+                    int64_t s, len;
+                    pi.bounds(q, s, len);
+                    const int64_t m = len * (2 * (int64_t)ks[q] + 1);
+                    for (int64_t i = 0; i < m; ++i) h_out[o + i] = (double)tmp[off[q] + i];
+                    o += m;
+                }
+                return;
+            }
             const int64_t w = 2 * (int64_t)S.k + 1;
             if (count < w * S.n) throw Error(SAP_ERR_BADARG, "output too small");
             std::vector<T> tmp((size_t)w * S.n);
