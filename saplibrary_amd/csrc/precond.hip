@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 
 namespace sap_amd {
 
@@ -83,6 +84,15 @@ void upload(DevBuf<V>& d, const std::vector<V>& h, hipStream_t st)
 
 namespace {
 
+// Out-of-memory for the SaP-C buffers can be forced from the host
+// (SAP_AMD_FAIL_SPIKE_ALLOC=1 in the environment), so the fallback below
+// is testable without exhausting the device.
+bool spike_alloc_forced_to_fail()
+{
+    const char* e = std::getenv("SAP_AMD_FAIL_SPIKE_ALLOC");
+    return e && e[0] == '1';
+}
+
 int clamp_partitions(int64_t n, int k, int requested)
 {
     const int64_t rows = std::max<int64_t>(k + 1, 2 * (int64_t)k);
@@ -92,11 +102,33 @@ int clamp_partitions(int64_t n, int k, int requested)
 
 }  // namespace
 
+// The reference builds the truncated-SPIKE buffers inside try blocks and,
+// on std::bad_alloc, factors the block-diagonal preconditioner instead
+// (sap/precond.h:1242-1265, :1281-1291, :1325-1346); the same here for the
+// device allocations of the SaP-C path.
+template <class T>
+template <class A, class R>
+bool Precond<T>::try_spike_buffers(A alloc, R release)
+{
+    if (block_fallback_) return false;
+    try {
+        if (spike_alloc_forced_to_fail()) throw Error(SAP_ERR_OOM, "SaP-C buffers (forced)");
+        alloc();
+        return true;
+    } catch (const Error& e) {
+        if (e.code != SAP_ERR_OOM) throw;
+        release();
+        block_fallback_ = true;
+        return false;
+    }
+}
+
 template <class T>
 void Precond<T>::begin_setup(int64_t n_, int k_, bool const_band)
 {
     n = n_;
     k = k_;
+    block_fallback_ = false;
     nr_cap_ = 0;   // apply_multi's buffers are sized again for this setup
     nbuf_ = 0;     // the sweeps' staging depth is tuned again for this k
     time_toBanded = time_offDiags = time_bandLU = time_bandUL = time_assembly = time_fullLU = 0;
@@ -119,16 +151,38 @@ void Precond<T>::begin_setup(int64_t n_, int k_, bool const_band)
     }
     // LU_UL factors the UL of partitions 1..P-1 on a second copy of the band
     // (kept across setups of the same size)
-    const bool need_ul = const_band && cfg.factMethod == SAP_LU_UL && cfg.precondType == SAP_Spike && P > 1 && k > 0;
-    if (need_ul) alloc_band(Bul_);
-    else Bul_.release();
+    bool need_ul = const_band && cfg.factMethod == SAP_LU_UL && cfg.precondType == SAP_Spike && P > 1 && k > 0;
     // the first partition is the longest (PartInfo gives the n mod P extra
     // rows to the first partitions)
     int64_t s0 = 0, len0 = n;
     PartInfo{n, P}.bounds(0, s0, len0);
     sep_in_ = need_ul && band_factor_resident_fits<T>(k, len0);
-    if (sep_in_) alloc_band(Bin_);
-    else Bin_.release();
+    // the SaP-C buffers of the constant-band path, before anything is
+    // written to the bands (the variable-bandwidth path allocates its own)
+    if (const_band && cfg.precondType == SAP_Spike && P > 1 && k > 0) {
+        const size_t kk = (size_t)k * k;
+        const bool ok = try_spike_buffers(
+            [&] {
+                if (need_ul) {
+                    alloc_band(Bul_);
+                    if (sep_in_) alloc_band(Bin_);
+                    scr1_.resize(n);
+                }
+                offB_.resize(kk * (P - 1));
+                offC_.resize(kk * (P - 1));
+                WV_.resize(2 * kk * (P - 1));
+                S_.resize(kk * (P - 1));
+                Sinv_.resize(kk * (P - 1));
+                corr_.resize((size_t)2 * k * (P - 1));
+            },
+            [&] {
+                Bul_.release(); Bin_.release(); scr1_.release();
+                offB_.release(); offC_.release(); WV_.release(); S_.release(); Sinv_.release(); corr_.release();
+            });
+        if (!ok) need_ul = sep_in_ = false;
+    }
+    if (!need_ul) Bul_.release();
+    if (!sep_in_) Bin_.release();
 }
 
 template <class T>
@@ -589,15 +643,26 @@ int Precond<T>::var_factor_steps()
     tm_.span(&time_bandLU, 3, 4);
     actual_nnz = vboff_[P] - count_band_zeros<T>(Bv_.get(), vboff_[P], (const T*)nullptr, 0, cnt_.get(), stream);
 
+    const size_t kk = (size_t)k * k;
+    if (spike_active())
+        (void)try_spike_buffers(
+            [&] {
+                g_.resize(n);
+                gd_.resize(n);
+                vq_.resize(n);
+                WV_.resize(2 * kk * (P - 1));
+                S_.resize(kk * (P - 1));
+                Sinv_.resize(kk * (P - 1));
+                corr_.resize((size_t)2 * k * (P - 1));
+            },
+            [&] {
+                WV_.release(); S_.release(); Sinv_.release(); corr_.release(); g_.release(); gd_.release();
+                vq_.release();
+            });
     if (!spike_active()) {
         WV_.release(); S_.release(); Sinv_.release(); corr_.release(); g_.release(); gd_.release(); vq_.release();
         return 0;
     }
-    const size_t kk = (size_t)k * k;
-    g_.resize(n);
-    gd_.resize(n);
-    vq_.resize(n);
-    WV_.resize(2 * kk * (P - 1));
     tm_.mark(5, stream);
     {
         ProfScope ps(prof, "spike_tips");
@@ -606,9 +671,6 @@ int Precond<T>::var_factor_steps()
                              vperm_r_.get(), vperm_l_.get(), vwidth_.get(), vrange_.get(), vmaxw_, WV_.get(), stream);
     }
     tm_.mark(6, stream);
-    S_.resize(kk * (P - 1));
-    Sinv_.resize(kk * (P - 1));
-    corr_.resize((size_t)2 * k * (P - 1));
     {
         ProfScope ps(prof, "reduced_factor");
         launch_reduced_factor<T>(WV_.get(), k, P - 1, S_.get(), Sinv_.get(), cfg.safeFactorization, stream);
@@ -620,8 +682,8 @@ int Precond<T>::var_factor_steps()
 }
 
 // Full sweeps (L, D^{-1}, U') of every partition on its own band, second-
-// level order in and out, one launch over all partitions when every width
-// takes the staged sweep.
+// level order in and out: one launch over all partitions per sweep variant
+// their widths need.
 template <class T>
 typename Precond<T>::VJobs& Precond<T>::var_jobs(int nr, int64_t ldv, int64_t ldz, T* scr)
 {
@@ -630,25 +692,32 @@ typename Precond<T>::VJobs& Precond<T>::var_jobs(int nr, int64_t ldv, int64_t ld
     if (key.nr == J.key.nr && key.ldv == J.key.ldv && key.ldz == J.key.ldz && key.scr == J.key.scr && J.n > 0)
         return J;
     const PartInfo pi{n, P};
-    std::vector<SweepJob> jobs;
-    size_t lds = 0;
-    bool ok = true;
-    for (int p = 0; p < P && ok; ++p) {
+    // the jobs grouped by sweep variant (one launch each)
+    std::vector<SweepJob> by_mode[3];
+    for (int p = 0; p < P; ++p) {
         int64_t s, len;
         pi.bounds(p, s, len);
         const T* Bp = Bv_.get() + vboff_[p];
-        ok = sweep_jobs_append<T>(jobs, lds, Bp, Bp, vks_[p], PartInfo{len, 1}, 0, nullptr, scr + nr * s, s, ldv, s,
-                                  ldz, nr, 0, 0);
+        std::vector<SweepJob> one;
+        size_t lds = 0;
+        int mode = 0;
+        if (!sweep_jobs_append<T>(one, lds, Bp, Bp, vks_[p], PartInfo{len, 1}, 0, nullptr, scr + nr * s, s, ldv, s,
+                                  ldz, nr, 0, 0, &mode))
+            throw Error(SAP_ERR_HIP, "variable-bandwidth sweep job table");
+        by_mode[mode].insert(by_mode[mode].end(), one.begin(), one.end());
+        J.lds[mode] = std::max(J.lds[mode], lds);
     }
-    J.ok = ok;
-    J.lds = lds;
-    J.n = ok ? jobs.size() : 1;
-    if (ok) {
-        J.d.resize(jobs.size());
-        SAP_HIP_CHECK(hipMemcpyAsync(J.d.get(), jobs.data(), jobs.size() * sizeof(SweepJob), hipMemcpyHostToDevice,
-                                     stream));
-        SAP_HIP_CHECK(hipStreamSynchronize(stream));   // the host table is released on return
+    std::vector<SweepJob> jobs;
+    for (int m = 0; m < 3; ++m) {
+        J.start[m] = jobs.size();
+        J.count[m] = by_mode[m].size();
+        jobs.insert(jobs.end(), by_mode[m].begin(), by_mode[m].end());
     }
+    J.n = jobs.size();
+    J.d.resize(jobs.size());
+    SAP_HIP_CHECK(hipMemcpyAsync(J.d.get(), jobs.data(), jobs.size() * sizeof(SweepJob), hipMemcpyHostToDevice,
+                                 stream));
+    SAP_HIP_CHECK(hipStreamSynchronize(stream));   // the host table is released on return
     J.key = key;
     return J;
 }
@@ -656,25 +725,10 @@ typename Precond<T>::VJobs& Precond<T>::var_jobs(int nr, int64_t ldv, int64_t ld
 template <class T>
 void Precond<T>::var_sweeps(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate, T* scr)
 {
-    const PartInfo pi{n, P};
     const VJobs& J = var_jobs(nr, ldv, ldz, scr);
-    if (J.ok) {
-        launch_sweep_full_jobs<T>(J.d.get(), (int)J.n, J.lds, V, Z, gate, stream);
-        return;
-    }
-    // a width the staged sweep does not take: one launch per partition
-    Rhs rh;
-    rh.nr = nr;
-    rh.ldv = ldv;
-    rh.ldo = ldz;
-    rh.gate = gate;
-    for (int p = 0; p < P; ++p) {
-        int64_t s, len;
-        pi.bounds(p, s, len);
-        const T* Bp = Bv_.get() + vboff_[p];
-        launch_sweep_full<T>(Bp, Bp, vks_[p], PartInfo{len, 1}, 0, V + s, nullptr, Z + s, nullptr, scr + nr * s,
-                             stream, rh);
-    }
+    for (int m = 2; m >= 0; --m)
+        if (J.count[m])
+            launch_sweep_full_jobs<T>(J.d.get() + J.start[m], (int)J.count[m], J.lds[m], V, Z, gate, stream, m);
 }
 
 // getSRev's variable-bandwidth branch (sap/precond.h:1457-1470, then the
@@ -745,7 +799,8 @@ int Precond<T>::factor_band_steps()
     // no stored pivot can be below 1e-7, so the check is skipped.
     const bool check_pivots = !cfg.safeFactorization && !(boost && boost_last);
 
-    if (cfg.precondType == SAP_Block || P == 1) {
+    const size_t kk = (size_t)k * k;
+    if (!spike_active()) {
         std::vector<FactorJob> jobs;
         for (int p = 0; p < P; ++p) {
             int64_t s, len;
@@ -763,10 +818,6 @@ int Precond<T>::factor_band_steps()
     }
 
     // (2) coupling blocks, extracted before factorization
-    const size_t kk = (size_t)k * k;
-    offB_.resize(kk * (P - 1));
-    offC_.resize(kk * (P - 1));
-    WV_.resize(2 * kk * (P - 1));
     tm_.mark(2, stream);
     {
         ProfScope ps(prof, "extract_coupling");
@@ -868,9 +919,6 @@ int Precond<T>::factor_band_steps()
     tm_.mark(6, stream);
 
     // (5) reduced blocks: S_i = I - W V_i, LU, inverse
-    S_.resize(kk * (P - 1));
-    Sinv_.resize(kk * (P - 1));
-    corr_.resize((size_t)2 * k * (P - 1));
     {
         ProfScope ps(prof, "reduced_factor");
         launch_reduced_factor<T>(WV_.get(), k, P - 1, S_.get(), Sinv_.get(), cfg.safeFactorization, stream);

@@ -166,8 +166,11 @@ public:
     // Error(SAP_ERR_ILLEGAL_UPDATE) before a setup.
     bool update(const double* d_entries, int64_t count, int* rc);
 
-    bool spike_active() const { return cfg.precondType == SAP_Spike && P > 1 && k > 0; }
-    bool lu_ul() const { return cfg.factMethod == SAP_LU_UL && !var_mode_; }
+    bool spike_active() const { return cfg.precondType == SAP_Spike && P > 1 && k > 0 && !block_fallback_; }
+    bool lu_ul() const { return cfg.factMethod == SAP_LU_UL && !var_mode_ && !block_fallback_; }
+    // the last setup could not allocate the SaP-C buffers and built the
+    // block-diagonal (SaP-D) preconditioner instead (sap/precond.h:1255-1265)
+    bool block_fallback() const { return block_fallback_; }
     bool variable_mode() const { return var_mode_; }
 
     // diagnostics
@@ -203,6 +206,11 @@ private:
     // variable-bandwidth setup: the preconditioner of the variable-bandwidth
     // path is factored LU_only with every pivot boosted
     bool var_mode_ = false;
+    bool block_fallback_ = false;
+    // allocates the SaP-C buffers with alloc(); on an out-of-memory error
+    // releases them with release() and switches this setup to Block
+    template <class A, class R>
+    bool try_spike_buffers(A alloc, R release);
     // variable-bandwidth storage: the partitions' own bands in second-level
     // order (partition p at vboff_[p], half-bandwidth vks_[p]), then the
     // coupling blocks B_i and C_{i+1} (k x k column-major each, first-level
@@ -225,8 +233,8 @@ private:
     struct VJobKey { int nr; int64_t ldv, ldz; const void* scr; };
     struct VJobs {
         DevBuf<SweepJob> d;
-        size_t n = 0, lds = 0;
-        bool ok = false;
+        size_t n = 0;                       // jobs in d (0: not built)
+        size_t start[3] = {0, 0, 0}, count[3] = {0, 0, 0}, lds[3] = {0, 0, 0};   // per sweep variant
         VJobKey key{0, 0, 0, nullptr};
     } vjobs_[2];
     VJobs& var_jobs(int nr, int64_t ldv, int64_t ldz, T* scr);

@@ -315,15 +315,17 @@ struct SweepJob {
 };
 // Appends the jobs of one system (nr columns at voff + r*ldv / ooff +
 // r*ldo, scr and corr holding nr consecutive blocks, gates gate0 + r) and
-// raises lds to what they need; false when its width does not take the
-// staged sweep (nothing appended).
+// raises lds to what they need.  Without mode_out: false when its width
+// does not take the staged sweep (nothing appended); with mode_out: any
+// width, *mode_out the sweep variant its jobs need (jobs of one launch
+// share it).  False for k = 0.
 template <class T>
 bool sweep_jobs_append(std::vector<SweepJob>& jobs, size_t& lds, const T* B, const T* Bul, int k, PartInfo pi,
                        int last_is_ul, const T* corr, T* scr, int64_t voff, int64_t ldv, int64_t ooff, int64_t ldo,
-                       int nr, int gate0, int nbuf_req);
+                       int nr, int gate0, int nbuf_req, int* mode_out = nullptr);
 template <class T>
 void launch_sweep_full_jobs(const SweepJob* d_jobs, int njobs, size_t lds, const double* V, double* Z,
-                            const int* gate, hipStream_t st);
+                            const int* gate, hipStream_t st, int mode = 2);
 template <class T>
 void launch_sweep_interface(const T* B, const T* Bul, int k, PartInfo pi, const double* v, T* g, T* scr0, T* scr1,
                             hipStream_t st, const Rhs& rh = Rhs());

@@ -783,11 +783,10 @@ __global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_full_kerne
 // a table (one partition of one system for one right-hand side; null B:
 // padding), with that system's band, width, staging depth and LDS ring;
 // the input column is V + voff and the output Z + ooff.
-template <class T>
-__global__ __launch_bounds__(SW_THREADS) void sweep_full_jobs_kernel(const SweepJob* __restrict__ jobs,
-                                                                     const double* __restrict__ V,
-                                                                     double* __restrict__ Z,
-                                                                     const int* __restrict__ gate)
+template <class T, int MODE>
+__global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_full_jobs_kernel(
+    const SweepJob* __restrict__ jobs, const double* __restrict__ V, double* __restrict__ Z,
+    const int* __restrict__ gate)
 {
     T* pend = lds_ring<T>();
     __shared__ T xblk[NBS];
@@ -797,7 +796,7 @@ __global__ __launch_bounds__(SW_THREADS) void sweep_full_jobs_kernel(const Sweep
     if (!J.B) return;
     if (gate && !gate[J.gate]) return;
     const PartInfo pi{J.n, J.P};
-    sweep_full_body<T, 2>((const T*)J.B, (const T*)J.Bul, J.k, pi, J.last_is_ul, J.part, V + J.voff,
+    sweep_full_body<T, MODE>((const T*)J.B, (const T*)J.Bul, J.k, pi, J.last_is_ul, J.part, V + J.voff,
                           (const T*)J.corr, Z + J.ooff, nullptr, (T*)J.scr, J.ring_mask, J.nbuf, pend, xblk, xblk2,
                           acc2);
 }
@@ -1064,12 +1063,14 @@ void launch_diag_divide(const T* B, int64_t n, const double* v, double* out, hip
 template <class T>
 bool sweep_jobs_append(std::vector<SweepJob>& jobs, size_t& lds, const T* B, const T* Bul, int k, PartInfo pi,
                        int last_is_ul, const T* corr, T* scr, int64_t voff, int64_t ldv, int64_t ooff, int64_t ldo,
-                       int nr, int gate0, int nbuf_req)
+                       int nr, int gate0, int nbuf_req, int* mode_out)
 {
     // many workgroups: the fewest staging buffers, so more fit on a CU
     (void)nbuf_req;
     const SweepCfg c = sweep_cfg<T>(k, 4);
-    if (c.mode != 2 || k == 0) return false;
+    if (k == 0) return false;
+    if (mode_out) *mode_out = c.mode;
+    else if (c.mode != 2) return false;
     lds = std::max(lds, c.lds);
     // the kernel's (job, right-hand side) order per system: the nr
     // workgroups of a partition 8 ids apart (one XCD), padded to whole
@@ -1100,13 +1101,24 @@ bool sweep_jobs_append(std::vector<SweepJob>& jobs, size_t& lds, const T* B, con
     return true;
 }
 
+template <class T, int MODE>
+void launch_sweep_full_jobs_t(const SweepJob* d_jobs, int njobs, size_t lds, const double* V, double* Z,
+                              const int* gate, hipStream_t st)
+{
+    set_lds_limit((const void*)sweep_full_jobs_kernel<T, MODE>, lds);
+    hipLaunchKernelGGL((sweep_full_jobs_kernel<T, MODE>), dim3(njobs), dim3(MODE == 2 ? SW_THREADS : 256), lds, st,
+                       d_jobs, V, Z, gate);
+    SAP_LAUNCH_CHECK();
+}
+
 template <class T>
 void launch_sweep_full_jobs(const SweepJob* d_jobs, int njobs, size_t lds, const double* V, double* Z,
-                            const int* gate, hipStream_t st)
+                            const int* gate, hipStream_t st, int mode)
 {
-    set_lds_limit((const void*)sweep_full_jobs_kernel<T>, lds);
-    hipLaunchKernelGGL(sweep_full_jobs_kernel<T>, dim3(njobs), dim3(SW_THREADS), lds, st, d_jobs, V, Z, gate);
-    SAP_LAUNCH_CHECK();
+    if (njobs <= 0) return;
+    if (mode == 2)      launch_sweep_full_jobs_t<T, 2>(d_jobs, njobs, lds, V, Z, gate, st);
+    else if (mode == 1) launch_sweep_full_jobs_t<T, 1>(d_jobs, njobs, lds, V, Z, gate, st);
+    else                launch_sweep_full_jobs_t<T, 0>(d_jobs, njobs, lds, V, Z, gate, st);
 }
 
 template <class T>
@@ -1119,9 +1131,9 @@ int sweep_max_nbuf(int k)
 #define SAP_INST(T)                                                                                               \
     template int sweep_max_nbuf<T>(int);                                                                           \
     template bool sweep_jobs_append<T>(std::vector<SweepJob>&, size_t&, const T*, const T*, int, PartInfo, int,    \
-                                       const T*, T*, int64_t, int64_t, int64_t, int64_t, int, int, int);            \
+                                       const T*, T*, int64_t, int64_t, int64_t, int64_t, int, int, int, int*);      \
     template void launch_sweep_full_jobs<T>(const SweepJob*, int, size_t, const double*, double*, const int*,      \
-                                            hipStream_t);                                                          \
+                                            hipStream_t, int);                                                     \
     template void launch_sweep_interface<T>(const T*, const T*, int, PartInfo, const double*, T*, T*, T*,          \
                                             hipStream_t, const Rhs&);                                              \
     template void launch_interface_solve<T>(const T*, const T*, const T*, const T*, int, PartInfo, const T*, T*,   \

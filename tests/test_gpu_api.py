@@ -12,6 +12,12 @@
     within 2x of the oracle's), BiCGStab(1) converging (iteration count
     within a quarter step), MINRES with an indefinite preconditioner (-10)
     and with M = A (10, "beta2 = 0 ...").
+  * Out of device memory for the SaP-C buffers (forced from the host with
+    SAP_AMD_FAIL_SPIKE_ALLOC=1): setup builds the block-diagonal (SaP-D)
+    preconditioner instead, as the reference does on std::bad_alloc
+    (sap/precond.h:1242-1265, :1281-1291, :1325-1346), so M^{-1} v equals
+    an explicit Block setup's bit for bit, on the banded (LU_UL, LU_only)
+    and the variable-bandwidth CSR paths, and the solve still converges.
   * The C++ interface (include/sap/solver.hpp) through a compiled program:
     setup, solve with MVBanded and a user functor, update(), CSR setup with
     reordering and SpmvCusp, Illegal_solve before setup.
@@ -174,3 +180,54 @@ def test_cpp_interface_program_runs(sap):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "ALL PASSED" in r.stdout
+
+
+@pytest.mark.parametrize("fact", ["LU_UL", "LU_only"])
+def test_spike_buffers_out_of_memory_fall_back_to_block(sap, oracle, fact, monkeypatch):
+    n, k, P = 20000, 40, 8
+    band = oracle.gen_banded(n, k, 1.0, 401)
+    A = sap.BandedMatrix(torch.from_numpy(band).cuda())
+    fm = sap.LU_UL if fact == "LU_UL" else sap.LU_only
+    v = torch.from_numpy(oracle.gen_vector(n, 402)).cuda()
+    blk = sap.Solver(P, sap.Options(factMethod=fm, precondType=sap.Block))
+    blk.setup(A)
+    zb = torch.empty_like(v)
+    blk.precond_apply(v, zb)
+    monkeypatch.setenv("SAP_AMD_FAIL_SPIKE_ALLOC", "1")
+    g = sap.Solver(P, sap.Options(factMethod=fm, precondType=sap.Spike, relTol=1e-10, refine=8))
+    g.setup(A)
+    monkeypatch.delenv("SAP_AMD_FAIL_SPIKE_ALLOC")
+    z = torch.empty_like(v)
+    g.precond_apply(v, z)
+    torch.cuda.synchronize()
+    assert torch.equal(z, zb)
+    b = torch.from_numpy(oracle.band_spmv(band, oracle.gen_vector(n, 403))).cuda()
+    x = torch.zeros_like(b)
+    assert g.solve(None, b, x), g.getMonitorMessage()
+    r = b.cpu().numpy() - oracle.band_spmv(band, x.cpu().numpy())
+    assert np.linalg.norm(r) / np.linalg.norm(b.cpu().numpy()) <= 1e-10
+    # a later setup with memory available builds SaP-C again
+    g.setup(A)
+    g.precond_apply(v, z)
+    torch.cuda.synchronize()
+    assert not torch.equal(z, zb)
+
+
+def test_spike_buffers_out_of_memory_variable_bandwidth(sap, oracle, monkeypatch):
+    from tests.csr_cases import sparse_band
+    n, P = 2400, 3
+    rp, ci, val = sparse_band(n, 90, 8, 57)
+    A = sap.CsrMatrix.from_host(rp, ci, val)
+    v = torch.from_numpy(oracle.gen_vector(n, 404)).cuda()
+    blk = sap.Solver(P, sap.Options(precondType=sap.Block))
+    blk.setup(A)
+    zb = torch.empty_like(v)
+    blk.precond_apply(v, zb)
+    monkeypatch.setenv("SAP_AMD_FAIL_SPIKE_ALLOC", "1")
+    g = sap.Solver(P, sap.Options())
+    g.setup(A)
+    monkeypatch.delenv("SAP_AMD_FAIL_SPIKE_ALLOC")
+    z = torch.empty_like(v)
+    g.precond_apply(v, z)
+    torch.cuda.synchronize()
+    assert torch.equal(z, zb)
