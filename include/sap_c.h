@@ -99,7 +99,7 @@ typedef struct sap_options {
     int    safeFactorization;   /* default 0                                       */
     int    variableBandwidth;   /* default 1                                       */
     int    trackReordering;     /* default 0                                       */
-    int    useBCR;              /* default 0; only 0 is supported                  */
+    int    useBCR;              /* default 0; 1: block cyclic reduction (Block)    */
     int    ilu_level;           /* default -1; only -1 is supported                */
     /* ---- additions ---- */
     int    precValueType;       /* SAP_PREC_FLOAT (default) or SAP_PREC_DOUBLE     */

@@ -151,7 +151,8 @@ void Precond<T>::begin_setup(int64_t n_, int k_, bool const_band)
     }
     // LU_UL factors the UL of partitions 1..P-1 on a second copy of the band
     // (kept across setups of the same size)
-    bool need_ul = const_band && cfg.factMethod == SAP_LU_UL && cfg.precondType == SAP_Spike && P > 1 && k > 0;
+    bool need_ul = const_band && cfg.factMethod == SAP_LU_UL && cfg.precondType == SAP_Spike && P > 1 && k > 0 &&
+                   !cfg.bcr;
     // the first partition is the longest (PartInfo gives the n mod P extra
     // rows to the first partitions)
     int64_t s0 = 0, len0 = n;
@@ -159,7 +160,7 @@ void Precond<T>::begin_setup(int64_t n_, int k_, bool const_band)
     sep_in_ = need_ul && band_factor_resident_fits<T>(k, len0);
     // the SaP-C buffers of the constant-band path, before anything is
     // written to the bands (the variable-bandwidth path allocates its own)
-    if (const_band && cfg.precondType == SAP_Spike && P > 1 && k > 0) {
+    if (const_band && cfg.precondType == SAP_Spike && P > 1 && k > 0 && !cfg.bcr) {
         const size_t kk = (size_t)k * k;
         const bool ok = try_spike_buffers(
             [&] {
@@ -361,7 +362,8 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
     //     in-place reduced sweeps, sap/device/sweep_band_var.cuh:21-191, then
     //     race), so the banded path's clamp n / max(k+1, 2k) is used here
     //     too, and the assembly below uses the same partition count.
-    var_mode_ = rc.variable && R.k > 0 && clamp_partitions(n_, R.k, cfg.numPartitions) > 1;
+    // (BCR runs on the constant-width band)
+    var_mode_ = rc.variable && !cfg.bcr && R.k > 0 && clamp_partitions(n_, R.k, cfg.numPartitions) > 1;
     begin_setup(n_, R.k, !var_mode_);
     const PartInfo pi{n, P};
 
@@ -767,10 +769,155 @@ void Precond<T>::var_solve(const double* V, int64_t ldv, double* Z, int64_t ldz,
     var_sweeps(vq, n, Z, ldz, nr, gate, scr);
 }
 
+// Block cyclic reduction of every partition (the reference's useBCR setup,
+// sap/precond.h:981-1210): blocks of at least 2k rows, level l eliminating
+// the blocks s, 3s, 5s, ... (s = 2^l) of every partition together, each
+// partition's block 0 factored last.  Like the reference's BCR setup it
+// builds the block-diagonal preconditioner (no spikes); pivots are boosted
+// by the Block rules, and there is no zero-pivot check (the reference's BCR
+// branch returns before it).
+template <class T>
+int Precond<T>::bcr_factor_steps()
+{
+    const PartInfo pi{n, P};
+    cnt_.resize(1);
+    scr0_.resize(n);
+    split_ = false;
+    offB_.release(); offC_.release(); WV_.release(); S_.release(); Sinv_.release(); corr_.release();
+    scr1_.release();
+    bblk_.clear();
+    blev_.clear();
+    bfinal_.clear();
+    std::vector<int> first(P), nb(P);
+    int64_t max_len = 0;
+    for (int p = 0; p < P; ++p) {
+        int64_t s, len;
+        pi.bounds(p, s, len);
+        nb[p] = (int)std::max<int64_t>(1, len / (2 * (int64_t)k));
+        first[p] = (int)bblk_.size();
+        const PartInfo bi{len, nb[p]};
+        for (int j = 0; j < nb[p]; ++j) {
+            int64_t bs, bl;
+            bi.bounds(j, bs, bl);
+            bblk_.push_back({s + bs, (int)bl, j > 0, j < nb[p] - 1});
+            max_len = std::max(max_len, bl);
+        }
+        bfinal_.push_back(first[p]);
+    }
+    for (int st = 1;; st *= 2) {
+        BcrLevel L;
+        for (int p = 0; p < P; ++p)
+            for (int e = st; e < nb[p]; e += 2 * st)
+                L.el.push_back({first[p] + e, first[p] + e - st, e + st < nb[p] ? first[p] + e + st : -1});
+        if (L.el.empty()) break;
+        blev_.push_back(std::move(L));
+    }
+    const size_t nblk = bblk_.size(), kk = (size_t)k * k;
+    upload(bblk_d_, bblk_, stream);
+    for (BcrLevel& L : blev_) upload(L.d, L.el, stream);
+    bCL_.resize(nblk * kk);
+    bCU_.resize(nblk * kk);
+    bUA_.resize(nblk * kk);
+    bLC_.resize(nblk * kk);
+    bw_.resize(n);
+    bt_.resize(n);
+    bjobs_[0].nr = bjobs_[1].nr = 0;   // the sweep job tables are built again for these blocks
+    const int boost = (k >= 70) || cfg.safeFactorization;
+    const int boost_last = (k >= 70);
+    tm_.mark(3, stream);
+    {
+        ProfScope ps(prof, "band_factor");
+        launch_bcr_extract<T>(B_.get(), k, bblk_d_.get(), (int)nblk, bCL_.get(), bCU_.get(), stream);
+        for (const BcrLevel& L : blev_) {
+            std::vector<FactorJob> jobs;
+            for (const BcrElim& E : L.el) jobs.push_back({bblk_[E.e].start, bblk_[E.e].len, 1, boost, boost_last, 0});
+            factor(jobs);
+            launch_bcr_level_setup<T>(B_.get(), k, bblk_d_.get(), L.d.get(), (int)L.el.size(), max_len, bCL_.get(),
+                                      bCU_.get(), bUA_.get(), bLC_.get(), stream);
+        }
+        std::vector<FactorJob> jobs;
+        for (int b : bfinal_) jobs.push_back({bblk_[b].start, bblk_[b].len, 1, boost, boost_last, 0});
+        factor(jobs);
+    }
+    tm_.mark(4, stream);
+    tm_.span(&time_bandLU, 3, 4);
+    const int64_t w = 2 * k + 1;
+    actual_nnz = (int64_t)w * n - count_band_zeros<T>(B_.get(), (int64_t)w * n, (const T*)nullptr, 0, cnt_.get(),
+                                                      stream);
+    bcr_ = true;
+    return 0;
+}
+
+// The sweep jobs of the BCR apply: stage l < L the blocks eliminated at
+// level l, stage L every partition's last block; input and output columns n
+// apart.
+template <class T>
+typename Precond<T>::BcrJobs& Precond<T>::bcr_jobs(int nr, T* scr)
+{
+    BcrJobs& J = bjobs_[nr == 1 ? 0 : 1];
+    if (J.nr == nr && J.scr == scr) return J;
+    std::vector<SweepJob> jobs;
+    size_t lds = 0;
+    int mode = 0;
+    J.start.clear();
+    J.count.clear();
+    auto add = [&](int b) {
+        mode = sweep_range_job_append<T>(jobs, lds, B_.get(), k, bblk_[b].start, bblk_[b].len, scr, n, n, n, nr);
+    };
+    for (const BcrLevel& L : blev_) {
+        J.start.push_back(jobs.size());
+        for (const BcrElim& E : L.el) add(E.e);
+        J.count.push_back(jobs.size() - J.start.back());
+    }
+    J.start.push_back(jobs.size());
+    for (int b : bfinal_) add(b);
+    J.count.push_back(jobs.size() - J.start.back());
+    J.d.resize(jobs.size());
+    SAP_HIP_CHECK(hipMemcpyAsync(J.d.get(), jobs.data(), jobs.size() * sizeof(SweepJob), hipMemcpyHostToDevice,
+                                 stream));
+    SAP_HIP_CHECK(hipStreamSynchronize(stream));   // the host table is released on return
+    J.lds = lds;
+    J.mode = mode;
+    J.nr = nr;
+    J.scr = scr;
+    return J;
+}
+
+// The BCR apply (updateRHS, sap/precond.h:6485-6612): forward through the
+// levels, the last blocks, back-substitution through the levels.
+template <class T>
+void Precond<T>::bcr_solve(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate)
+{
+    T* scr = nr == 1 ? scr0_.get() : scr0m_.get();
+    double* w = nr == 1 ? bw_.get() : bwm_.get();
+    double* t = nr == 1 ? bt_.get() : btm_.get();
+    const BcrJobs& J = bcr_jobs(nr, scr);
+    const int L = (int)blev_.size();
+    auto sweep = [&](int stage, const double* in, double* out) {
+        launch_sweep_full_jobs<T>(J.d.get() + J.start[stage], (int)J.count[stage], J.lds, in, out, gate, stream,
+                                  J.mode);
+    };
+    ProfScope ps(prof, "sweep_full");
+    for (int r = 0; r < nr; ++r) launch_copy(n, V + r * ldv, w + r * n, stream);
+    for (int l = 0; l < L; ++l) {
+        sweep(l, w, t);
+        launch_bcr_gemv<T>(0, k, bblk_d_.get(), blev_[l].d.get(), (int)blev_[l].el.size(), bUA_.get(), bLC_.get(), t,
+                           n, w, n, nr, gate, stream);
+    }
+    sweep(L, w, Z);
+    for (int l = L - 1; l >= 0; --l) {
+        launch_bcr_gemv<T>(1, k, bblk_d_.get(), blev_[l].d.get(), (int)blev_[l].el.size(), bCL_.get(), bCU_.get(), Z,
+                           ldz, w, n, nr, gate, stream);
+        sweep(l, w, Z);
+    }
+}
+
 template <class T>
 int Precond<T>::factor_band_steps()
 {
+    bcr_ = false;
     if (var_mode_) return var_factor_steps();
+    if (cfg.bcr && k > 0 && cfg.precondType != SAP_None) return bcr_factor_steps();
     const PartInfo pi{n, P};
     const int64_t w = 2 * k + 1;
     // timing events: 2 before the coupling blocks, 3 before and 4 after the
@@ -965,6 +1112,10 @@ void Precond<T>::solve_band(const double* v, double* z)
         launch_diag_divide<T>(B_.get(), n, v, z, stream);
         return;
     }
+    if (bcr_) {
+        bcr_solve(v, n, z, n, 1, nullptr);
+        return;
+    }
     if (spike_active()) {
         if (lu_ul()) {
             ProfScope ps(prof, "sweep_interface");
@@ -1013,13 +1164,20 @@ void Precond<T>::reserve_multi(int nr)
     // the job table apply_multi's variable-bandwidth sweeps use (input and
     // output strides n), built here, before any graph capture
     if (var_mode_ && nr > 1) (void)var_jobs(nr, n, n, scr0m_.get());
+    if (bcr_ && nr > 1) {
+        if (bwm_.size() < (size_t)nr * n) {
+            bwm_.resize((size_t)nr * n);
+            btm_.resize((size_t)nr * n);
+        }
+        (void)bcr_jobs(nr, scr0m_.get());
+    }
 }
 
 template <class T>
 bool Precond<T>::append_sweep_jobs(std::vector<SweepJob>& jobs, size_t& lds, int64_t voff, int64_t ooff, int nr,
                                    int gate0)
 {
-    if (cfg.precondType == SAP_None || reorder_ || spike_active() || k == 0) return false;
+    if (cfg.precondType == SAP_None || reorder_ || spike_active() || k == 0 || bcr_) return false;
     reserve_multi(nr);
     return sweep_jobs_append<T>(jobs, lds, B_.get(), B_.get(), k, PartInfo{n, P}, 0, nullptr, scr0m_.get(), voff, n,
                                 ooff, n, nr, gate0, nbuf_);
@@ -1064,6 +1222,11 @@ void Precond<T>::solve_band_multi(const double* V, int64_t ldv, double* Z, int64
     if (k == 0) {
         ProfScope ps(prof, "diag_divide");
         launch_diag_divide<T>(B_.get(), n, V, Z, stream, rh);
+        return;
+    }
+    if (bcr_) {
+        if (ldz != n) throw Error(SAP_ERR_BADARG, "block cyclic reduction: output columns must be n apart");
+        bcr_solve(V, ldv, Z, ldz, nr, gate);
         return;
     }
     if (spike_active()) {

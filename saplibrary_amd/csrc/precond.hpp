@@ -14,6 +14,7 @@ struct PrecondConfig {
     int precondType = SAP_Spike;
     int safeFactorization = 0;
     int autotune = 0;         // time the tunable launch parameters on first use (autotune.hpp)
+    int bcr = 0;              // useBCR: block cyclic reduction of the partitions (block-diagonal preconditioner)
 };
 
 // Reordering options of a CSR setup, as sap::Solver forwards them to
@@ -166,8 +167,11 @@ public:
     // Error(SAP_ERR_ILLEGAL_UPDATE) before a setup.
     bool update(const double* d_entries, int64_t count, int* rc);
 
-    bool spike_active() const { return cfg.precondType == SAP_Spike && P > 1 && k > 0 && !block_fallback_; }
-    bool lu_ul() const { return cfg.factMethod == SAP_LU_UL && !var_mode_ && !block_fallback_; }
+    bool spike_active() const
+    {
+        return cfg.precondType == SAP_Spike && P > 1 && k > 0 && !block_fallback_ && !cfg.bcr;
+    }
+    bool lu_ul() const { return cfg.factMethod == SAP_LU_UL && !var_mode_ && !block_fallback_ && !cfg.bcr; }
     // the last setup could not allocate the SaP-C buffers and built the
     // block-diagonal (SaP-D) preconditioner instead (sap/precond.h:1255-1265)
     bool block_fallback() const { return block_fallback_; }
@@ -241,6 +245,31 @@ private:
     int var_factor_steps();
     void var_sweeps(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate, T* scr);
     void var_solve(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate);
+    // block cyclic reduction (cfg.bcr): the blocks of every partition, the
+    // eliminations of each level, each partition's last block, the
+    // couplings (CL, CU, k x k per block, updated level by level) and the
+    // neighbours' couplings each eliminated block saw (UA, LC)
+    bool bcr_ = false;
+    std::vector<BcrBlock> bblk_;
+    DevBuf<BcrBlock> bblk_d_;
+    struct BcrLevel {
+        std::vector<BcrElim> el;
+        DevBuf<BcrElim> d;
+    };
+    std::vector<BcrLevel> blev_;
+    std::vector<int> bfinal_;
+    DevBuf<T> bCL_, bCU_, bUA_, bLC_;
+    DevBuf<double> bw_, bt_, bwm_, btm_;
+    struct BcrJobs {
+        DevBuf<SweepJob> d;
+        std::vector<size_t> start, count;   // per stage: levels 0..L-1, then the last blocks
+        size_t lds = 0;
+        int mode = 0, nr = 0;
+        const void* scr = nullptr;
+    } bjobs_[2];
+    int bcr_factor_steps();
+    BcrJobs& bcr_jobs(int nr, T* scr);
+    void bcr_solve(const double* V, int64_t ldv, double* Z, int64_t ldz, int nr, const int* gate);
     DevBuf<int> row_perm_, col_reord_;
     DevBuf<T> row_scale_, col_scale_;
     DevBuf<double> vp_, zp_;

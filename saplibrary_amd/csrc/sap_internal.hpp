@@ -259,6 +259,24 @@ void launch_extract_coupling(const T* B, int k, PartInfo pi, T* offB, T* offC, h
 template <class T>
 void launch_spike_tips(const T* B, const T* Bul, int k, PartInfo pi, int fact_lu_ul,
                        const T* offB, const T* offC, T* WV, hipStream_t st);
+// bcr.hip: block cyclic reduction of the partitions (see there)
+struct BcrBlock {
+    int64_t start;           // first row
+    int len;                 // rows (>= 2k, or the whole partition)
+    int has_left, has_right; // neighbours inside the partition
+};
+struct BcrElim {
+    int e, a, c;             // eliminated block, its left and right neighbours (c = -1: none)
+};
+template <class T>
+void launch_bcr_extract(const T* B, int k, const BcrBlock* blk, int nblk, T* CL, T* CU, hipStream_t st);
+template <class T>
+void launch_bcr_level_setup(T* B, int k, const BcrBlock* blk, const BcrElim* el, int nel, int64_t max_len, T* CL,
+                            T* CU, T* UA, T* LC, hipStream_t st);
+template <class T>
+void launch_bcr_gemv(int mode, int k, const BcrBlock* blk, const BcrElim* el, int nel, const T* M0, const T* M1,
+                     const double* src, int64_t lds, double* wv, int64_t ldw, int nr, const int* gate,
+                     hipStream_t st);
 // var.hip: the variable-bandwidth path (per-partition bands in second-level
 // order; see there)
 template <class T>
@@ -312,7 +330,15 @@ struct SweepJob {
     int64_t voff, ooff;
     int64_t n;
     int P, k, part, gate, last_is_ul, ring_mask, nbuf, pad_;
+    int64_t s0, len0;     // len0 > 0: band rows [s0, s0 + len0) instead of partition `part`
 };
+// Appends nr jobs (column r: input V + r*ldv, output Z + r*ldo, scratch
+// scr + r*n, gate r) sweeping band rows [s0, s0 + len0) of B (half-bandwidth
+// k, n rows) as one system; returns the sweep variant (jobs of one launch
+// share it) and raises lds to what it needs.
+template <class T>
+int sweep_range_job_append(std::vector<SweepJob>& jobs, size_t& lds, const T* B, int k, int64_t s0, int64_t len0,
+                           T* scr, int64_t n, int64_t ldv, int64_t ldo, int nr);
 // Appends the jobs of one system (nr columns at voff + r*ldv / ooff +
 // r*ldo, scr and corr holding nr consecutive blocks, gates gate0 + r) and
 // raises lds to what they need.  Without mode_out: false when its width

@@ -230,6 +230,7 @@ public:
         c.precondType = opt.precondType;
         c.safeFactorization = opt.safeFactorization;
         c.autotune = opt.autotune;
+        c.bcr = opt.useBCR;
         if (opt.precValueType == SAP_PREC_DOUBLE) {
             pd = std::make_unique<Precond<double>>();
             pd->cfg = c; pd->stream = stream; pd->prof = &prof;
@@ -1300,7 +1301,7 @@ int guarded_batch(sap_batch* h, F&& f)
 int check_options(const sap_options* o, std::string& why)
 {
     if (o->gpuCount != 1) { why = "gpuCount > 1 is not supported (one GPU)"; return SAP_ERR_UNSUPPORTED; }
-    if (o->useBCR) { why = "useBCR is not supported"; return SAP_ERR_UNSUPPORTED; }
+    if (o->useBCR && o->isSPD) { why = "BCR does not support SPD matrices"; return SAP_ERR_UNSUPPORTED; }
     if (o->ilu_level >= 0) { why = "ILU preconditioning is not supported"; return SAP_ERR_UNSUPPORTED; }
     if (o->saveMem) { why = "saveMem (SPD half-band storage) is not supported"; return SAP_ERR_UNSUPPORTED; }
     if (o->solverType != SAP_BiCGStab1 && o->solverType != SAP_BiCGStab2 && o->solverType != SAP_MINRES) {

@@ -683,10 +683,14 @@ __device__ __forceinline__ void sweep_full_body(const T* __restrict__ B, const T
                                                 int last_is_ul, int p, const double* __restrict__ v,
                                                 const T* __restrict__ corr, double* __restrict__ outd,
                                                 T* __restrict__ outt, T* __restrict__ scr, int ring_mask, int nbuf,
-                                                T* pend, T* xblk, T* xblk2, T* acc2, int half = -1)
+                                                T* pend, T* xblk, T* xblk2, T* acc2, int half = -1,
+                                                int64_t s_ovr = 0, int64_t len_ovr = 0)
 {
+    // len_ovr > 0: the rows [s_ovr, s_ovr + len_ovr) of the band instead of
+    // partition p (a block of a partition)
     int64_t s, len;
-    pi.bounds(p, s, len);
+    if (len_ovr > 0) { s = s_ovr; len = len_ovr; }
+    else pi.bounds(p, s, len);
     // half 0: rows [s + len/2, s + len) from the LU factors (stage 2 from
     // the bottom up to the middle); half 1: rows [s, s + len/2) from the UL
     // factors (stage 2 from the top down to the middle); -1: all rows
@@ -798,7 +802,7 @@ __global__ __launch_bounds__(MODE == 2 ? SW_THREADS : 256) void sweep_full_jobs_
     const PartInfo pi{J.n, J.P};
     sweep_full_body<T, MODE>((const T*)J.B, (const T*)J.Bul, J.k, pi, J.last_is_ul, J.part, V + J.voff,
                           (const T*)J.corr, Z + J.ooff, nullptr, (T*)J.scr, J.ring_mask, J.nbuf, pend, xblk, xblk2,
-                          acc2);
+                          acc2, -1, J.s0, J.len0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1101,6 +1105,35 @@ bool sweep_jobs_append(std::vector<SweepJob>& jobs, size_t& lds, const T* B, con
     return true;
 }
 
+template <class T>
+int sweep_range_job_append(std::vector<SweepJob>& jobs, size_t& lds, const T* B, int k, int64_t s0, int64_t len0,
+                           T* scr, int64_t n, int64_t ldv, int64_t ldo, int nr)
+{
+    const SweepCfg c = sweep_cfg<T>(k, 4);
+    lds = std::max(lds, c.lds);
+    for (int r = 0; r < nr; ++r) {
+        SweepJob J{};
+        J.B = B;
+        J.Bul = B;
+        J.corr = nullptr;
+        J.scr = scr + (int64_t)r * n;
+        J.voff = r * ldv;
+        J.ooff = r * ldo;
+        J.n = n;
+        J.P = 1;
+        J.k = k;
+        J.part = 0;
+        J.gate = r;
+        J.last_is_ul = 0;
+        J.ring_mask = ring_size(k) - 1;
+        J.nbuf = c.nbuf;
+        J.s0 = s0;
+        J.len0 = len0;
+        jobs.push_back(J);
+    }
+    return c.mode;
+}
+
 template <class T, int MODE>
 void launch_sweep_full_jobs_t(const SweepJob* d_jobs, int njobs, size_t lds, const double* V, double* Z,
                               const int* gate, hipStream_t st)
@@ -1134,6 +1167,8 @@ int sweep_max_nbuf(int k)
                                        const T*, T*, int64_t, int64_t, int64_t, int64_t, int, int, int, int*);      \
     template void launch_sweep_full_jobs<T>(const SweepJob*, int, size_t, const double*, double*, const int*,      \
                                             hipStream_t, int);                                                     \
+    template int sweep_range_job_append<T>(std::vector<SweepJob>&, size_t&, const T*, int, int64_t, int64_t, T*,  \
+                                           int64_t, int64_t, int64_t, int);                                        \
     template void launch_sweep_interface<T>(const T*, const T*, int, PartInfo, const double*, T*, T*, T*,          \
                                             hipStream_t, const Rhs&);                                              \
     template void launch_interface_solve<T>(const T*, const T*, const T*, const T*, int, PartInfo, const T*, T*,   \

@@ -67,8 +67,9 @@ def test_struct_layouts_match_header(sap):
 
 
 def test_create_rejects_unsupported_options(sap):
-    # useBCR / ILU / several GPUs are not part of this build: rejected at create
-    for kw in ({"useBCR": 1}, {"ilu_level": 0}, {"gpuCount": 2}, {"solverType": sap.GMRES_C}):
+    # ILU / several GPUs / the CUSP Krylov solvers are not part of this build,
+    # and BCR does not take SPD matrices (as in the reference): rejected at create
+    for kw in ({"useBCR": 1, "isSPD": 1}, {"ilu_level": 0}, {"gpuCount": 2}, {"solverType": sap.GMRES_C}):
         with pytest.raises(sap.SapError):
             sap.Solver(4, sap.Options(**kw))
 
