@@ -141,18 +141,19 @@ def main():
     x = torch.zeros_like(b)
     torch.cuda.synchronize()
 
-    def options(plain):
+    def options(plain, bcr=0):
         # autotune: the warm-up step times the conversion tile widths and
         # sweep staging depths for this k (the results do not change)
         if plain:
             return sap.Options(precondType=sap.Block, factMethod=sap.LU_UL, precValueType=sap.PREC_DOUBLE,
                                relTol=1e-10, profile=1, autotune=1)
         return sap.Options(precondType=sap.Spike if spike else sap.Block, factMethod=sap.LU_UL,
-                           precValueType=sap.PREC_FLOAT, relTol=1e-10, refine=8, profile=1, autotune=1)
+                           precValueType=sap.PREC_FLOAT, relTol=1e-10, refine=8, profile=1, autotune=1,
+                           useBCR=bcr)
 
-    def run(plain, steps, warmup, label):
+    def run(plain, steps, warmup, label, bcr=0):
         # one solver object: buffers allocated by the warm-up setup are reused
-        s = sap.Solver(P, options(plain))
+        s = sap.Solver(P, options(plain, bcr))
         times, last = [], None
         for it in range(warmup + steps):
             if it == warmup:
@@ -217,6 +218,18 @@ def main():
             gbs = algo_b / (prof[fam][1] / prof[fam][0] / 1e3) / 1e9
             stream_fracs[fam] = {"GB/s": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4),
                                  "frac_of_stream_read": round(gbs / read_gbs, 4)}
+
+    # the SaP-D workload with block cyclic reduction of the partitions
+    # (useBCR) instead of their sequential banded LU
+    bcr = None
+    if not spike:
+        btimes, (bs, bst, bok) = run(False, args.steps, args.warmup, "SaP-D fp32 BCR", bcr=1)
+        bprof = bs.profile()
+        bcr = {"ms": round(sum(btimes) / len(btimes), 3), "converged": bool(bok),
+               "setup_ms": round(bst.timeSetup, 3), "solve_ms": round(bst.timeSolve, 3),
+               "iterations": bst.numIterations,
+               "kernels": {kn: {"launches_per_step": round(l / args.steps, 2), "ms_per_step": round(t / args.steps, 3)}
+                           for kn, (l, t) in bprof.items()}}
 
     plain_ms = None
     if not args.no_plain:
@@ -291,6 +304,7 @@ def main():
         "kernels": kernels,
         "roofline": roof,
         "krylov": krylov,
+        "bcr": bcr,
         "stream_read_gbs": round(read_gbs, 1),
         "hbm_kernels": stream_fracs,
         "cpu_baseline": cpu,
