@@ -124,7 +124,9 @@ APPLY_CASES = FACTOR_CASES + [(20000, 100, 16, "LU_UL", "Spike"), (5000, 0, 4, "
                               (3000, 10, 3, "LU_UL", "None"),
                               # k above the pipelined-sweep limit (224 FP32, 208 FP64) and between them
                               (5000, 300, 4, "LU_UL", "Spike"), (8000, 210, 8, "LU_UL", "Spike"),
-                              (7000, 230, 3, "LU_only", "Spike")]
+                              (7000, 230, 3, "LU_only", "Spike"),
+                              # wide FP64 bands (the staging buffers no longer fit the LDS: plain sweeps)
+                              (9000, 350, 4, "LU_only", "Block"), (7000, 380, 3, "LU_UL", "Spike")]
 
 
 @pytest.mark.parametrize("n,k,P,fact,precond", APPLY_CASES)
