@@ -18,6 +18,52 @@ double Precond<T>::elapsed_ms(hipEvent_t a, hipEvent_t b)
 }
 
 template <class T>
+Precond<T>::~Precond()
+{
+    try {
+        finish_pending();
+    } catch (...) {
+    }
+    if (spk_done_) (void)hipEventDestroy(spk_done_);
+}
+
+template <class T>
+void Precond<T>::finish_pending()
+{
+    if (!spk_open_) return;
+    spk_open_ = spk_wait_ = false;
+    SAP_HIP_CHECK(hipStreamSynchronize(side2_.s));
+    tm2_.resolve();
+    actual_nnz = (2 * (int64_t)k + 1) * n - (int64_t)*side2_.host;
+}
+
+template <class T>
+void Precond<T>::wait_spikes()
+{
+    if (!spk_wait_) return;
+    SAP_HIP_CHECK(hipStreamWaitEvent(stream, spk_done_, 0));
+    spk_wait_ = false;
+}
+
+namespace {
+
+// profile scopes of work queued on another stream record their events there
+struct ProfOnStream {
+    Profiler* p;
+    hipStream_t saved = nullptr;
+    ProfOnStream(Profiler* p_, hipStream_t s) : p(p_)
+    {
+        if (p) { saved = p->stream; p->stream = s; }
+    }
+    ~ProfOnStream()
+    {
+        if (p) p->stream = saved;
+    }
+};
+
+}  // namespace
+
+template <class T>
 void Precond<T>::factor(const std::vector<FactorJob>& jobs)
 {
     if (jobs.empty()) return;
@@ -126,6 +172,7 @@ bool Precond<T>::try_spike_buffers(A alloc, R release)
 template <class T>
 void Precond<T>::begin_setup(int64_t n_, int k_, bool const_band)
 {
+    finish_pending();
     n = n_;
     k = k_;
     block_fallback_ = false;
@@ -247,6 +294,13 @@ void Precond<T>::tune_sweeps()
     std::vector<int> cand;
     for (int b = 4; b <= nmax; ++b) cand.push_back(b);
     if (cand.size() < 2) return;
+    // already tuned for this width: no timing runs (and no allocation, whose
+    // release would wait for the setup work still on the second stream)
+    int known = 0;
+    if (Autotuner::instance().find(TUNE_SWEEP_NBUF, k, (int)sizeof(T), &known)) {
+        nbuf_ = known;
+        return;
+    }
     DevBuf<double> tmp(2 * (size_t)n);
     launch_fill(n, 1.0, tmp.get(), stream);
     const bool on = prof && prof->enabled;
@@ -544,6 +598,7 @@ bool Precond<T>::update(const double* d_entries, int64_t count, int* rc)
         throw Error(SAP_ERR_ILLEGAL_UPDATE, "Illegal call to update() with reordering tracking disabled.");
     if (count != src_count_) return false;
     if (cfg.precondType == SAP_None) return true;
+    finish_pending();
     const double t0 = std::chrono::duration<double, std::milli>(
         std::chrono::steady_clock::now().time_since_epoch()).count();
     time_transfer = 0;
@@ -1042,39 +1097,39 @@ int Precond<T>::factor_band_steps()
         }
     }
 
-    // non-zeros of the factored band (sap/precond.h:1352): the LU part and,
-    // for LU_UL, the UL-factored last partition.  The count reads the factors
-    // only, so it runs on a second stream beside the spike tips and the
-    // reduced blocks.
+    // (4) spike tips and (5) reduced blocks S_i = I - W V_i (LU, inverse),
+    // then the non-zeros of the factored band (sap/precond.h:1352: the LU
+    // part and, for LU_UL, the UL-factored last partition), on a second
+    // stream after the factorization; the setup returns without waiting
+    // (finish_pending, wait_spikes)
+    hipStream_t s2 = side2_.after(stream);
+    if (!spk_done_) SAP_HIP_CHECK(hipEventCreateWithFlags(&spk_done_, hipEventDisableTiming));
+    tm2_.clear();
+    tm2_.mark(5, s2);
     {
-        hipStream_t side = side_.after(stream);
-        if (lu_ul())
-            count_band_zeros_async<T>(B_.get(), (int64_t)w * s_last, Bul_.get() + (size_t)w * s_last,
-                                      (int64_t)w * l_last, cnt_.get(), side_.host, side, 1024);
-        else
-            count_band_zeros_async<T>(B_.get(), (int64_t)w * n, (const T*)nullptr, 0, cnt_.get(), side_.host, side,
-                                      1024);
-    }
-
-    // (4) spike tips
-    tm_.mark(5, stream);
-    {
+        ProfOnStream po(prof, s2);
         ProfScope ps(prof, "spike_tips");
-        launch_spike_tips<T>(B_.get(), Bul_.get(), k, pi, lu_ul() ? 1 : 0, offB_.get(), offC_.get(), WV_.get(),
-                             stream);
+        launch_spike_tips<T>(B_.get(), Bul_.get(), k, pi, lu_ul() ? 1 : 0, offB_.get(), offC_.get(), WV_.get(), s2);
     }
-    tm_.mark(6, stream);
-
-    // (5) reduced blocks: S_i = I - W V_i, LU, inverse
+    tm2_.mark(6, s2);
     {
+        ProfOnStream po(prof, s2);
         ProfScope ps(prof, "reduced_factor");
-        launch_reduced_factor<T>(WV_.get(), k, P - 1, S_.get(), Sinv_.get(), cfg.safeFactorization, stream);
+        launch_reduced_factor<T>(WV_.get(), k, P - 1, S_.get(), Sinv_.get(), cfg.safeFactorization, s2);
     }
-    tm_.mark(7, stream);
-    tm_.span(&time_assembly, 5, 6);
-    tm_.span(&time_fullLU, 6, 7);
-    SAP_HIP_CHECK(hipStreamSynchronize(side_.s));
-    actual_nnz = (int64_t)w * n - (int64_t)*side_.host;
+    tm2_.mark(7, s2);
+    SAP_HIP_CHECK(hipEventRecord(spk_done_, s2));
+    if (lu_ul())
+        count_band_zeros_async<T>(B_.get(), (int64_t)w * s_last, Bul_.get() + (size_t)w * s_last,
+                                  (int64_t)w * l_last, cnt_.get(), side2_.host, s2, 1024);
+    else
+        count_band_zeros_async<T>(B_.get(), (int64_t)w * n, (const T*)nullptr, 0, cnt_.get(), side2_.host, s2,
+                                  1024);
+    tm2_.span(&time_assembly, 5, 6);
+    tm2_.span(&time_fullLU, 6, 7);
+    time_assembly = time_fullLU = 0;
+    actual_nnz = 0;
+    spk_wait_ = spk_open_ = true;
     return 0;
 }
 
@@ -1124,6 +1179,7 @@ void Precond<T>::solve_band(const double* v, double* z)
             ProfScope ps(prof, "sweep_interface");
             launch_sweep_full<T>(B_.get(), B_.get(), k, pi, 0, v, nullptr, nullptr, g_.get(), scr0_.get(), stream, rh);
         }
+        wait_spikes();
         {
             ProfScope ps(prof, "interface_solve");
             launch_interface_solve<T>(WV_.get(), Sinv_.get(), offB_.get(), offC_.get(), k, pi, g_.get(), corr_.get(),
@@ -1143,6 +1199,9 @@ void Precond<T>::solve_band(const double* v, double* z)
 template <class T>
 void Precond<T>::reserve_multi(int nr)
 {
+    // (a batched solve may capture apply_multi into a graph: the main stream
+    // is ordered after the deferred setup work here, outside the capture)
+    wait_spikes();
     if (nr > nr_cap_) {
         const size_t m = (size_t)nr;
         gm_.resize(m * n);
@@ -1237,6 +1296,7 @@ void Precond<T>::solve_band_multi(const double* V, int64_t ldv, double* Z, int64
             ProfScope ps(prof, "sweep_interface");
             launch_sweep_full<T>(B_.get(), B_.get(), k, pi, 0, V, nullptr, nullptr, gm_.get(), scr0m_.get(), stream, rh);
         }
+        wait_spikes();
         {
             ProfScope ps(prof, "interface_solve");
             launch_interface_solve<T>(WV_.get(), Sinv_.get(), offB_.get(), offC_.get(), k, pi, gm_.get(), corrm_.get(),

@@ -166,6 +166,17 @@ public:
     // their order).  Returns false when count does not match; throws
     // Error(SAP_ERR_ILLEGAL_UPDATE) before a setup.
     bool update(const double* d_entries, int64_t count, int* rc);
+    // A SaP-C setup of the constant band leaves the spike tips, the reduced
+    // blocks and the nonzero count running on a second stream (the first
+    // application's interface sweep needs only the factors and runs beside
+    // them; its reduced solve waits for them).  finish_pending() waits for
+    // that work and fills the statistics it produces (time_assembly,
+    // time_fullLU, actual_nnz); every other user of those buffers calls it.
+    void finish_pending();
+    ~Precond();
+    Precond() = default;
+    Precond(const Precond&) = delete;
+    Precond& operator=(const Precond&) = delete;
 
     bool spike_active() const
     {
@@ -195,6 +206,13 @@ private:
     DevBuf<unsigned long long> cnt_;
     SideStream side_;
     SetupTimer tm_;
+    // the deferred spike / reduced-block work (see finish_pending)
+    SideStream side2_;
+    SetupTimer tm2_;
+    hipEvent_t spk_done_ = nullptr;   // recorded after the reduced blocks
+    bool spk_wait_ = false;           // the main stream is not yet ordered after spk_done_
+    bool spk_open_ = false;           // its statistics are not yet read
+    void wait_spikes();
     // the conversion that preceded factor_band() also wrote Bul_ (the copy
     // the UL factorizations overwrite), so no device copy is needed
     bool bul_filled_ = false;

@@ -243,12 +243,26 @@ public:
     }
     void set_stream(hipStream_t s)
     {
+        settle();   // the deferred setup work is ordered on the old stream
         stream = s;
         prof.stream = s;
         if (pf) pf->stream = s;
         if (pd) pd->stream = s;
     }
     int num_partitions() const { return pf ? pf->P : (pd ? pd->P : 0); }
+    // waits for the setup work left on the preconditioner's second stream
+    // and copies the statistics it produces
+    void settle()
+    {
+        auto fin = [&](auto& p) {
+            p.finish_pending();
+            stats.time_assembly = p.time_assembly;
+            stats.time_fullLU = p.time_fullLU;
+            stats.actual_nnz = (int)std::min<int64_t>(p.actual_nnz, INT_MAX);
+        };
+        if (pf) fin(*pf);
+        else if (pd) fin(*pd);
+    }
 
     // ---------------------------------------------------------- vectors
     void reserve_vectors(int count)
@@ -1552,6 +1566,11 @@ const char* sap_batch_last_error(const sap_batch* h) { return h ? h->err.c_str()
 int sap_get_stats(const sap_solver* h, sap_stats* out)
 {
     if (!h || !out) return SAP_ERR_BADARG;
+    const int rc = guarded(const_cast<sap_solver*>(h), [&] {
+        h->s->settle();
+        return 0;
+    });
+    if (rc != 0) return rc;
     *out = h->s->stats;
     return SAP_OK;
 }
@@ -1620,6 +1639,7 @@ int sap_get_factors(const sap_solver* h, double* h_out, int64_t count)
     Solver& S = *h->s;
     if (!S.setup_done) return SAP_ERR_ILLEGAL_SOLVE;
     return guarded(const_cast<sap_solver*>(h), [&] {
+        S.settle();
         auto grab = [&](auto& p) {
             using T = typename std::remove_reference<decltype(*p.factors().get())>::type;
             if (p.variable_mode()) {
@@ -1672,6 +1692,7 @@ int sap_get_spikes(const sap_solver* h, double* h_wv, double* h_schur, int64_t c
     Solver& S = *h->s;
     if (!S.setup_done) return SAP_ERR_ILLEGAL_SOLVE;
     return guarded(const_cast<sap_solver*>(h), [&] {
+        S.settle();
         auto grab = [&](auto& p) {
             using T = typename std::remove_reference<decltype(*p.spikes().get())>::type;
             const size_t nwv = p.spikes().size(), ns = p.schur().size();
