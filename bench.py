@@ -73,7 +73,7 @@ def algorithmic(name, n, k, P, sp, spike):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c1_sapc_fp32_pmc_traffic.md")
 FAMILY_KERNEL = {"band_factor": "band_factor_resident_kernel", "sweep_full": "sweep_full_kernel",
                  "sweep_interface": "sweep_interface_kernel", "spmv": "band_spmv_kernel",
-                 "band_to_colmajor": "band_to_colmajor_cols_kernel"}
+                 "band_to_colmajor": "band_to_colmajor_pipe_kernel"}
 
 
 def measured_traffic(family):
