@@ -303,7 +303,11 @@ __device__ __forceinline__ void lower_sweep_fast(const BandView<T>& V, int64_t l
 constexpr int SW_PROD = 3;           // producer wavefronts
 constexpr int SW_STREAMER = 4;
 constexpr int SW_PROD0 = 5;
-constexpr int SW_NI_MAX = 12;        // load instructions per producer per block (k <= 284 FP32, 335 FP64)
+// load instructions per producer per block: up to 12 in FP32 (k <= 284),
+// 16 in FP64 (k <= 382, the wide FP64 bands; FP32 keeps its per-lane
+// offset tables small)
+template <class T>
+constexpr int sw_ni_max() { return sizeof(T) == 4 ? 12 : 16; }
 constexpr int SW_FAR = 192;          // far rows of waves 1-3 per pass (one row per lane)
 constexpr int SW_THREADS = 512;
 constexpr int SW_DIN = 8;            // right-hand side steps in flight
@@ -393,13 +397,14 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
     // every block up to a uniform shift: seg_lo(c0 + j) = seg_lo(j) +
     // dir * w * c0, and w * c0 is a multiple of E.  So the offsets (for block
     // 0) and columns are tabulated once per lane.
-    int64_t poff[SW_NI_MAX];
-    int pcol[SW_NI_MAX];
+    constexpr int NIM = sw_ni_max<T>();
+    int64_t poff[NIM];
+    int pcol[NIM];
     const int64_t first0 = seg_lo(0) & ~(int64_t)(E - 1);
     if (wave >= SW_PROD0) {
         const int lane = threadIdx.x & 63;
 #pragma unroll
-        for (int i = 0; i < SW_NI_MAX; ++i) {
+        for (int i = 0; i < NIM; ++i) {
             const int rnd = i * SW_PROD + (wave - SW_PROD0);   // 64-chunk round of this block
             const int g = rnd * 64 + lane;
             const int j = g / G.nch, ch = g - j * G.nch;
@@ -414,7 +419,7 @@ __device__ __forceinline__ void lower_sweep_lds(const BandView<T>& V, int64_t le
         const int64_t shift = b * blk_step;
         const int64_t rem = len - b * BS;                // columns of this block inside the view
 #pragma unroll
-        for (int i = 0; i < SW_NI_MAX; ++i) {
+        for (int i = 0; i < NIM; ++i) {
             if (i < G.ni) {
                 const int rnd = i * SW_PROD + (wave - SW_PROD0);
                 // columns past the end re-read the block's first chunk
@@ -974,7 +979,12 @@ SweepCfg sweep_cfg(int k, int nbuf_req = 0)
     const size_t guard = 256 * sizeof(T);   // masked reads past the last buffer stay inside the allocation
     int nbuf = (int)std::min<size_t>(8, (LDS_MAX - ring - stage - guard) / bb);
     nbuf = std::min(nbuf, 63 / G.ni + 3);
-    if (nbuf < 4 || G.ni > SW_NI_MAX) return {wide ? 0 : 1, 0, ring, 256};
+    // at least four block buffers (loads two blocks ahead); wide FP64 bands,
+    // whose blocks of 16 columns no longer fit four times, run with three
+    // (the next block's loads then complete within the step that issues
+    // them, still far faster than the plain blocked sweep)
+    const int min_nbuf = (sizeof(T) == 8 && wide) ? 3 : 4;
+    if (nbuf < min_nbuf || G.ni > sw_ni_max<T>()) return {wide ? 0 : 1, 0, ring, 256};
     if (nbuf_req >= 4 && nbuf_req < nbuf) nbuf = nbuf_req;
     return {2, nbuf, ring + stage + nbuf * bb + guard, SW_THREADS};
 }
