@@ -141,3 +141,26 @@ def test_bcr_reordered_csr_matches_oracle_block(sap, oracle):
     z = torch.empty(n, dtype=torch.float64, device="cuda")
     g.precond_apply(dev(v), z)
     np.testing.assert_allclose(host(z), expect, rtol=0, atol=1e-9 * np.abs(expect).max())
+
+
+def test_bcr_update_equals_fresh_setup(sap, oracle):
+    """update() with block cyclic reduction reruns the reduction on the new
+    values (same blocks, levels and kernels), so it equals a fresh setup of
+    those values bit for bit."""
+    n, k, P = 12000, 24, 3
+    band1 = oracle.gen_banded(n, k, 1.0, 530)
+    band2 = oracle.gen_banded(n, k, 1.1, 531)
+    o = dict(useBCR=1, precondType=sap.Block, trackReordering=1)
+    g = sap.Solver(P, sap.Options(**o))
+    g.setup(sap.BandedMatrix(dev(band1)))
+    assert g.update(dev(band2))
+    fresh = sap.Solver(P, sap.Options(**o))
+    fresh.setup(sap.BandedMatrix(dev(band2)))
+    np.testing.assert_array_equal(g.factors(), fresh.factors())
+    v = dev(oracle.gen_vector(n, 532))
+    z1 = torch.empty_like(v)
+    z2 = torch.empty_like(v)
+    g.precond_apply(v, z1)
+    fresh.precond_apply(v, z2)
+    torch.cuda.synchronize()
+    assert torch.equal(z1, z2)
