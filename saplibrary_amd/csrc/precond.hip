@@ -471,31 +471,17 @@ int Precond<T>::setup_csr(int64_t n_, int64_t nnz, const int* d_rp, const int* d
         upload(sr_, VB.second_reorder, stream);
         upload(vperm_r_, VB.perms_right, stream);
         upload(vperm_l_, VB.perms_left, stream);
-        // packed column counts and spike solve ranges per (boundary, side)
+        // packed column counts per (boundary, side), and the spike columns
         std::vector<int> widths(2 * (P - 1));
-        std::vector<VarSpikeRangeH> ranges(2 * (P - 1));
-        vmaxw_ = 0;
+        vcols_h_.clear();
         for (int i = 0; i < P - 1; ++i) {
-            int64_t s0, l0, s1, l1;
-            pi.bounds(i, s0, l0);
-            pi.bounds(i + 1, s1, l1);
             widths[2 * i] = VB.widths_right[i];
             widths[2 * i + 1] = VB.widths_left[i];
-            vmaxw_ = std::max({vmaxw_, VB.widths_right[i], VB.widths_left[i]});
-            // V_i: forward from the first second-level row holding a
-            // coupling entry (first_rows, sap/graph.h:994-1007), backward
-            // down to the lowest of the last k rows (sap/precond.h:5650)
-            int mn0 = (int)l0, mn1 = (int)l1;
-            for (int t2 = 0; t2 < k; ++t2) {
-                mn0 = std::min(mn0, VB.second_perm[s1 - k + t2] - (int)s0);
-                mn1 = std::min(mn1, VB.second_perm[s1 + t2] - (int)s1);
-            }
-            ranges[2 * i] = {VB.first_rows[i] - (int)s0, mn0};
-            // W_{i+1}: the coupling rows are its first k rows
-            ranges[2 * i + 1] = {mn1, mn1};
+            for (int q = 0; q < VB.widths_right[i]; ++q) vcols_h_.push_back({i, 0, q});
+            for (int q = 0; q < VB.widths_left[i]; ++q) vcols_h_.push_back({i, 1, q});
         }
         upload(vwidth_, widths, stream);
-        upload(vrange_, ranges, stream);
+        upload(vcols_, vcols_h_, stream);
         // the coupling blocks, column-major k x k after the bands
         const size_t kk = (size_t)k * k;
         Bv_.resize((size_t)vboff_[P] + 2 * kk * (P - 1) + 8);
@@ -636,6 +622,67 @@ int Precond<T>::factor_band()
     return rc;
 }
 
+// Spike tips of the variable-bandwidth path: every packed coupling column
+// a full-length right-hand side of its partition, solved by the staged
+// job-table sweep on the partition's band (one job per column, in chunks of
+// columns), tip rows gathered into WV.
+template <class T>
+void Precond<T>::var_spikes()
+{
+    const PartInfo pi{n, P};
+    const int ncol = (int)vcols_h_.size();
+    if (ncol == 0) return;
+    int64_t maxlen = 0;
+    for (int p = 0; p < P; ++p) {
+        int64_t s, len;
+        pi.bounds(p, s, len);
+        maxlen = std::max(maxlen, len);
+    }
+    const int64_t stride = (maxlen + 63) / 64 * 64;
+    // columns per chunk: about 1 GiB of FP64 right-hand sides, solutions and
+    // T scratch at a time
+    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(ncol, ((int64_t)1 << 30) /
+                                                                             (stride * (16 + (int64_t)sizeof(T)))));
+    DevBuf<double> V((size_t)stride * chunk), Z((size_t)stride * chunk);
+    DevBuf<T> scr((size_t)stride * chunk);
+    DevBuf<SweepJob> jd(chunk);
+    for (int c0 = 0; c0 < ncol; c0 += chunk) {
+        const int nc = std::min(chunk, ncol - c0);
+        launch_var_rhs<T>(pi, k, sp_.get(), voffB(), voffC(), vperm_r_.get(), vperm_l_.get(), vwidth_.get(),
+                          vcols_.get() + c0, nc, stride, V.get(), stream);
+        std::vector<SweepJob> by_mode[3];
+        size_t lds[3] = {0, 0, 0};
+        for (int j = 0; j < nc; ++j) {
+            const VarSpikeCol& c = vcols_h_[c0 + j];
+            const int p = c.i + c.side;
+            int64_t s, len;
+            pi.bounds(p, s, len);
+            std::vector<SweepJob> one;
+            size_t l = 0;
+            const int mode = sweep_range_job_append<T>(one, l, Bv_.get() + vboff_[p], vks_[p], 0, len,
+                                                       scr.get() + (int64_t)j * stride, len, 0, 0, 1);
+            one[0].voff = one[0].ooff = (int64_t)j * stride;
+            by_mode[mode].push_back(one[0]);
+            lds[mode] = std::max(lds[mode], l);
+        }
+        std::vector<SweepJob> jobs;
+        size_t start[3];
+        for (int m = 0; m < 3; ++m) {
+            start[m] = jobs.size();
+            jobs.insert(jobs.end(), by_mode[m].begin(), by_mode[m].end());
+        }
+        SAP_HIP_CHECK(hipMemcpyAsync(jd.get(), jobs.data(), jobs.size() * sizeof(SweepJob), hipMemcpyHostToDevice,
+                                     stream));
+        for (int m = 2; m >= 0; --m)
+            if (!by_mode[m].empty())
+                launch_sweep_full_jobs<T>(jd.get() + start[m], (int)by_mode[m].size(), lds[m], V.get(), Z.get(),
+                                          nullptr, stream, m);
+        launch_var_tips<T>(pi, k, sp_.get(), vperm_r_.get(), vperm_l_.get(), vwidth_.get(), vcols_.get() + c0, nc,
+                           stride, Z.get(), WV_.get(), stream);
+        SAP_HIP_CHECK(hipStreamSynchronize(stream));   // the job table is rewritten, the buffers released
+    }
+}
+
 // Variable-bandwidth setup after the fill (sap/precond.h:4546-4631 LU of the
 // partitions in their own bands, :5540-5790 spikes, :2549-2611 reduced
 // blocks): every pivot boosted, as in the reference's variable-bandwidth
@@ -724,8 +771,7 @@ int Precond<T>::var_factor_steps()
     {
         ProfScope ps(prof, "spike_tips");
         SAP_HIP_CHECK(hipMemsetAsync(WV_.get(), 0, WV_.size() * sizeof(T), stream));
-        launch_var_spikes<T>(pi, k, Bv_.get(), vks_d_.get(), vboff_d_.get(), sp_.get(), voffB(), voffC(),
-                             vperm_r_.get(), vperm_l_.get(), vwidth_.get(), vrange_.get(), vmaxw_, WV_.get(), stream);
+        var_spikes();
     }
     tm_.mark(6, stream);
     {

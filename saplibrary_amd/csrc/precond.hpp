@@ -243,8 +243,9 @@ private:
     std::vector<int64_t> vboff_;
     DevBuf<int> vks_d_, sp_, sr_, vperm_r_, vperm_l_, vwidth_;
     DevBuf<int64_t> vboff_d_;
-    DevBuf<VarSpikeRangeH> vrange_;
-    int vmaxw_ = 0;
+    std::vector<VarSpikeCol> vcols_h_;   // the packed coupling columns, one spike solve each
+    DevBuf<VarSpikeCol> vcols_;
+    void var_spikes();
     std::vector<T> vcoupling_h_;   // host copy of the coupling blocks (uploaded by setup)
     T* voffB() const { return Bv_.get() + vboff_.back(); }
     T* voffC() const { return voffB() + (size_t)k * k * (P - 1); }

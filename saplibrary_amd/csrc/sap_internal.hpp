@@ -282,13 +282,16 @@ void launch_bcr_gemv(int mode, int k, const BcrBlock* blk, const BcrElim* el, in
 template <class T>
 void launch_csr_to_varband(PartInfo pi, const int* rp, const int* ci, const T* val, const int* ks,
                            const int64_t* boff, T* Bv, hipStream_t st);
-struct VarSpikeRangeH {
-    int lo, mn;   // first row of the forward elimination, lowest tip row (partition-local, second level)
+struct VarSpikeCol {
+    int i, side, q;   // boundary, 0: V_i (right spike) / 1: W_{i+1} (left spike), packed column
 };
 template <class T>
-void launch_var_spikes(PartInfo pi, int k, const T* Bv, const int* ks, const int64_t* boff, const int* sp,
-                       const T* offB, const T* offC, const int* perms_r, const int* perms_l, const int* widths,
-                       const void* ranges, int max_width, T* WV, hipStream_t st);
+void launch_var_rhs(PartInfo pi, int k, const int* sp, const T* offB, const T* offC, const int* perms_r,
+                    const int* perms_l, const int* widths, const VarSpikeCol* cols, int ncol, int64_t stride,
+                    double* V, hipStream_t st);
+template <class T>
+void launch_var_tips(PartInfo pi, int k, const int* sp, const int* perms_r, const int* perms_l, const int* widths,
+                     const VarSpikeCol* cols, int ncol, int64_t stride, const double* Z, T* WV, hipStream_t st);
 template <class T>
 void launch_var_gather(PartInfo pi, int k, const int* sp, const double* gd, int64_t ldgd, T* g, int nr,
                        const int* gate, hipStream_t st);

@@ -38,96 +38,63 @@ __global__ void csr_to_varband_kernel(PartInfo pi, const int* __restrict__ rp, c
 
 // Spike tips of the variable-bandwidth path (calculateSpikes_var,
 // sap/precond.h:5540-5790, with fwdElim/bckElim_*Spike_per_partition of
-// sap/device/sweep_band_var.cuh).  Workgroup (i, side, chunk):
+// sap/device/sweep_band_var.cuh), one column of a coupling block per job:
 //   side 0: V_i = bottom k rows (first-level order) of A_i^{-1} [0; B_i]
 //           over the packed columns of B_i (offDiagPerms_right: the columns
 //           of B_i that hold an entry, widths_right of them);
 //   side 1: W_{i+1} = top k rows of A_{i+1}^{-1} [C_{i+1}; 0] over the
 //           packed columns of C_{i+1} (offDiagPerms_left, packed from the
 //           right end).
-// The solve runs on the partition's own band in second-level order: the
-// right-hand side rows are the second-level images of the coupling rows,
-// the forward elimination starts at the first row it can change (lo), the
-// backward substitution stops at the lowest tip row (mn); the tip rows are
-// gathered back into first-level order and the packed columns unpacked into
-// the k x k blocks of WV (WV[(2i + side) k^2 + col k + t], zero columns
-// where the coupling block has none).  RC right-hand sides per workgroup,
-// working vectors in global scratch (rows [0, n_p) x RC, row-major).
-constexpr int VRC = 32;
+// var_rhs_kernel scatters each packed column to the second-level rows of
+// its coupling rows (a full-length right-hand side of the partition), the
+// partition's full sweep (the staged job-table sweep, one job per column)
+// solves it on the partition's own band, and var_tips_kernel gathers the
+// tip rows back into first-level order and unpacks the column into the
+// k x k block of WV (WV[(2i + side) k^2 + col k + t]; WV zero-filled, so
+// columns without entries stay zero).
+__device__ inline void var_col(const VarSpikeCol& c, int k, const int* __restrict__ perms_r,
+                               const int* __restrict__ perms_l, const int* __restrict__ widths, int& col)
+{
+    col = c.side == 0 ? perms_r[(size_t)c.i * k + c.q]
+                      : perms_l[(size_t)c.i * k + (k - widths[2 * c.i + 1]) + c.q];
+}
 
 template <class T>
-__global__ __launch_bounds__(256) void var_spike_kernel(PartInfo pi, int k, const T* __restrict__ Bv,
-                                                        const int* __restrict__ ks, const int64_t* __restrict__ boff,
-                                                        const int* __restrict__ sp, const T* __restrict__ offB,
-                                                        const T* __restrict__ offC, const int* __restrict__ perms_r,
-                                                        const int* __restrict__ perms_l,
-                                                        const int* __restrict__ widths,
-                                                        const VarSpikeRangeH* __restrict__ ranges, T* __restrict__ WV,
-                                                        T* __restrict__ scratch, int64_t scratch_stride)
+__global__ void var_rhs_kernel(PartInfo pi, int k, const int* __restrict__ sp, const T* __restrict__ offB,
+                               const T* __restrict__ offC, const int* __restrict__ perms_r,
+                               const int* __restrict__ perms_l, const int* __restrict__ widths,
+                               const VarSpikeCol* __restrict__ cols, int64_t stride, double* __restrict__ V)
 {
-    const int i = blockIdx.x, side = blockIdx.y;
-    const int width = widths[2 * i + side];
-    const int q0 = blockIdx.z * VRC;
-    const int nrhs = min(VRC, width - q0);
-    if (nrhs <= 0) return;
-    const int p = i + side;
-    int64_t s, len64;
-    pi.bounds(p, s, len64);
-    const int len = (int)len64;
-    const int kp = ks[p];
-    const int64_t w = 2 * kp + 1;
-    const T* __restrict__ F = Bv + boff[p];
-    const VarSpikeRangeH rg = ranges[2 * i + side];
-    const int r0 = min(rg.lo, rg.mn);
-    const size_t kk = (size_t)k * k;
-    T* __restrict__ Y = scratch + (size_t)((2 * i + side) * gridDim.z + blockIdx.z) * scratch_stride;
-    // packed column q0 + r of this side: its column in the k x k block
-    auto col_of = [&](int r) -> int {
-        return side == 0 ? perms_r[(size_t)i * k + q0 + r] : perms_l[(size_t)i * k + (k - width) + q0 + r];
-    };
-    // first-level row t of the coupling rows -> second-level local row
-    const int64_t trow0 = side == 0 ? s + len - k : s;
-    auto tip_row = [&](int t) -> int { return sp[trow0 + t] - (int)s; };
+    const VarSpikeCol c = cols[blockIdx.x];
+    const int p = c.i + c.side;
+    int64_t s, len;
+    pi.bounds(p, s, len);
+    double* v = V + (int64_t)blockIdx.x * stride;
+    for (int64_t r = threadIdx.x; r < len; r += blockDim.x) v[r] = 0.0;
+    __syncthreads();
+    int col;
+    var_col(c, k, perms_r, perms_l, widths, col);
+    const T* src = (c.side == 0 ? offB : offC) + (size_t)c.i * k * k + (size_t)col * k;
+    const int64_t trow0 = c.side == 0 ? s + len - k : s;   // first coupling row (first-level order)
+    for (int t = threadIdx.x; t < k; t += blockDim.x) v[sp[trow0 + t] - s] = (double)src[t];
+}
 
-    for (int64_t e = threadIdx.x; e < (int64_t)(len - r0) * VRC; e += blockDim.x)
-        Y[(int64_t)r0 * VRC + e] = (T)0;
-    __syncthreads();
-    const T* __restrict__ src = (side == 0 ? offB : offC) + (size_t)i * kk;
-    for (int e = threadIdx.x; e < k * nrhs; e += blockDim.x) {
-        const int t = e % k, r = e / k;
-        Y[(int64_t)tip_row(t) * VRC + r] = src[(size_t)col_of(r) * k + t];
-    }
-    __syncthreads();
-    // L (unit lower): column j updates rows j+1 .. j+kp
-    for (int j = rg.lo; j < len - 1; ++j) {
-        const int nt = min(kp, len - 1 - j);
-        const T* col = F + (int64_t)j * w + kp;
-        for (int e = threadIdx.x; e < nt * nrhs; e += blockDim.x) {
-            const int dt = 1 + e % nt, r = e / nt;
-            Y[(int64_t)(j + dt) * VRC + r] -= col[dt] * Y[(int64_t)j * VRC + r];
-        }
-        __syncthreads();
-    }
-    for (int e = threadIdx.x; e < (len - r0) * nrhs; e += blockDim.x) {
-        const int t = r0 + e / nrhs, r = e % nrhs;
-        Y[(int64_t)t * VRC + r] /= F[(int64_t)t * w + kp];
-    }
-    __syncthreads();
-    // U' (unit upper): column j updates rows max(j-kp, mn) .. j-1
-    for (int j = len - 1; j > rg.mn; --j) {
-        const int nt = min(kp, j - rg.mn);
-        const T* col = F + (int64_t)j * w + kp;
-        for (int e = threadIdx.x; e < nt * nrhs; e += blockDim.x) {
-            const int dt = 1 + e % nt, r = e / nt;
-            Y[(int64_t)(j - dt) * VRC + r] -= col[-dt] * Y[(int64_t)j * VRC + r];
-        }
-        __syncthreads();
-    }
-    T* dst = WV + (size_t)(2 * i + side) * kk;
-    for (int e = threadIdx.x; e < k * nrhs; e += blockDim.x) {
-        const int t = e % k, r = e / k;
-        dst[(size_t)col_of(r) * k + t] = Y[(int64_t)tip_row(t) * VRC + r];
-    }
+template <class T>
+__global__ void var_tips_kernel(PartInfo pi, int k, const int* __restrict__ sp, const int* __restrict__ perms_r,
+                                const int* __restrict__ perms_l, const int* __restrict__ widths,
+                                const VarSpikeCol* __restrict__ cols, int64_t stride, const double* __restrict__ Z,
+                                T* __restrict__ WV)
+{
+    const VarSpikeCol c = cols[blockIdx.x];
+    const int p = c.i + c.side;
+    int64_t s, len;
+    pi.bounds(p, s, len);
+    const double* z = Z + (int64_t)blockIdx.x * stride;
+    int col;
+    var_col(c, k, perms_r, perms_l, widths, col);
+    T* dst = WV + (size_t)(2 * c.i + c.side) * k * k + (size_t)col * k;
+    const int64_t trow0 = c.side == 0 ? s + len - k : s;
+    for (int t = threadIdx.x; t < k; t += blockDim.x) dst[t] = (T)z[sp[trow0 + t] - s];
 }
 
 // Interface values for the reduced solve: g (first-level positions, type T)
@@ -196,22 +163,25 @@ void launch_csr_to_varband(PartInfo pi, const int* rp, const int* ci, const T* v
     SAP_LAUNCH_CHECK();
 }
 
-// ranges: 2 (P-1) (lo, mn) pairs; widths: 2 (P-1) packed column counts
-// (right of boundary i at 2i, left at 2i + 1).  WV must be zero-filled.
 template <class T>
-void launch_var_spikes(PartInfo pi, int k, const T* Bv, const int* ks, const int64_t* boff, const int* sp,
-                       const T* offB, const T* offC, const int* perms_r, const int* perms_l, const int* widths,
-                       const void* ranges, int max_width, T* WV, hipStream_t st)
+void launch_var_rhs(PartInfo pi, int k, const int* sp, const T* offB, const T* offC, const int* perms_r,
+                    const int* perms_l, const int* widths, const VarSpikeCol* cols, int ncol, int64_t stride,
+                    double* V, hipStream_t st)
 {
-    if (pi.P < 2 || max_width <= 0) return;
-    const unsigned chunks = (unsigned)((max_width + VRC - 1) / VRC);
-    const int64_t maxlen = pi.n / pi.P + 1;
-    const int64_t stride = maxlen * VRC;
-    DevBuf<T> scratch((size_t)stride * 2 * (pi.P - 1) * chunks);
-    hipLaunchKernelGGL(var_spike_kernel<T>, dim3(pi.P - 1, 2, chunks), dim3(256), 0, st, pi, k, Bv, ks, boff, sp, offB,
-                       offC, perms_r, perms_l, widths, (const VarSpikeRangeH*)ranges, WV, scratch.get(), stride);
+    if (ncol <= 0) return;
+    hipLaunchKernelGGL(var_rhs_kernel<T>, dim3(ncol), dim3(256), 0, st, pi, k, sp, offB, offC, perms_r, perms_l,
+                       widths, cols, stride, V);
     SAP_LAUNCH_CHECK();
-    SAP_HIP_CHECK(hipStreamSynchronize(st));   // scratch is released on return
+}
+
+template <class T>
+void launch_var_tips(PartInfo pi, int k, const int* sp, const int* perms_r, const int* perms_l, const int* widths,
+                     const VarSpikeCol* cols, int ncol, int64_t stride, const double* Z, T* WV, hipStream_t st)
+{
+    if (ncol <= 0) return;
+    hipLaunchKernelGGL(var_tips_kernel<T>, dim3(ncol), dim3(256), 0, st, pi, k, sp, perms_r, perms_l, widths, cols,
+                       stride, Z, WV);
+    SAP_LAUNCH_CHECK();
 }
 
 template <class T>
@@ -237,9 +207,10 @@ void launch_var_purify(PartInfo pi, int k, const int* sr, const double* v, int64
 #define SAP_INST(T)                                                                                               \
     template void launch_csr_to_varband<T>(PartInfo, const int*, const int*, const T*, const int*, const int64_t*, \
                                            T*, hipStream_t);                                                       \
-    template void launch_var_spikes<T>(PartInfo, int, const T*, const int*, const int64_t*, const int*, const T*,  \
-                                       const T*, const int*, const int*, const int*, const void*, int, T*,          \
-                                       hipStream_t);                                                               \
+    template void launch_var_rhs<T>(PartInfo, int, const int*, const T*, const T*, const int*, const int*,        \
+                                    const int*, const VarSpikeCol*, int, int64_t, double*, hipStream_t);          \
+    template void launch_var_tips<T>(PartInfo, int, const int*, const int*, const int*, const int*,               \
+                                     const VarSpikeCol*, int, int64_t, const double*, T*, hipStream_t);           \
     template void launch_var_gather<T>(PartInfo, int, const int*, const double*, int64_t, T*, int, const int*,     \
                                        hipStream_t);                                                               \
     template void launch_var_purify<T>(PartInfo, int, const int*, const double*, int64_t, const T*, double*,       \
