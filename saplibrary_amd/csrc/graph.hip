@@ -78,11 +78,10 @@ struct DbState {
     std::vector<int> match;         // match[column] = CSR row
     std::vector<int> rev_entry;     // rev_entry[row] = entry index of its matched edge
     std::vector<char> matched, rev_matched;
-    // Dijkstra scratch kept between augmentations (m_DB_*)
-    std::vector<char> inB, visited;
+    // Dijkstra scratch kept between augmentations (m_DB_*; the per-column
+    // part lives in DbCol records during the augmentations)
     std::vector<int> Blist;
-    std::vector<double> dval;
-    std::vector<int> prev, irn;
+    std::vector<int> prev;
 
     DbState(int n_, const std::vector<int>& rp_, const std::vector<int>& ci_) : n(n_), rp(rp_), ci(ci_) {}
 };
@@ -230,94 +229,115 @@ void db_init(DbState<T>& S, bool first_stage_only)
     }
 }
 
+// Per-column state of the augmenting-path search, one record per column:
+// relaxing an edge reads the column's dual, distance and flags together
+// (one cache line instead of one per array).  Filled from DbState before
+// the augmentations and copied back after them.
+struct DbCol {
+    double u;        // column dual
+    double dval;     // tentative distance (LOC_INFINITY when not reached)
+    int match;       // matched CSR row
+    int irn;         // entry through which the column was last reached
+    char inB, visited, matched, pad;
+};
+
 // Shortest augmenting path from CSR row `init` (find_shortest_aug_path,
 // sap/graph.h:2045-2173): Dijkstra over reduced costs, then augmentation and
-// dual update.
+// dual update.  The queue is a binary heap in `heap` driven by
+// std::push_heap / std::pop_heap with the reference's comparator, which is
+// std::priority_queue's own implementation, so entries leave in the same
+// order; the vector is reused across augmentations.
 template <class T>
-void db_augment(DbState<T>& S, int init)
+void db_augment(DbState<T>& S, std::vector<DbCol>& C, std::vector<NodeKey<double>>& heap, int init)
 {
     const auto& rp = S.rp;
     const auto& ci = S.ci;
     const auto& c = S.c;
-    std::priority_queue<NodeKey<double>, std::vector<NodeKey<double>>, KeyGreater<double>> Q;
+    const KeyGreater<double> greater;
+    heap.clear();
     double lsp = 0.0, lsap = LOC_INFINITY;
     int cur = init, isap = -1, ksap = -1, b_cnt = 0;
     S.prev[init] = -1;
     while (true) {
+        const double vcur = S.v[cur];
         for (int i = rp[cur]; i < rp[cur + 1]; ++i) {
             const int col = ci[i];
-            if (S.inB[col]) continue;
+            DbCol& cc = C[col];
+            if (cc.inB) continue;
             if (c[i] > LOC_INFINITY / 2.0) continue;
-            const double reduced = c[i] - S.u[col] - S.v[cur];
+            const double reduced = c[i] - cc.u - vcur;
             if (reduced + 1e-10 < 0) throw Error(SAP_ERR_NEG_DB_WEIGHT, "Negative reduced weight in DB.");
             const double d_new = lsp + reduced;
             if (d_new < lsap) {
-                if (!S.matched[col]) {
+                if (!cc.matched) {
                     lsap = d_new;
                     isap = col;
                     ksap = i;
-                    S.match[isap] = cur;
-                } else if (d_new < S.dval[col]) {
-                    S.dval[col] = d_new;
-                    S.prev[S.match[col]] = cur;
-                    Q.push({col, d_new});
-                    S.irn[col] = i;
+                    cc.match = cur;
+                } else if (d_new < cc.dval) {
+                    cc.dval = d_new;
+                    S.prev[cc.match] = cur;
+                    heap.push_back({col, d_new});
+                    std::push_heap(heap.begin(), heap.end(), greater);
+                    cc.irn = i;
                 }
             }
         }
         NodeKey<double> top{0, 0.0};
         bool found = false;
-        while (!Q.empty()) {
-            top = Q.top();
-            Q.pop();
-            if (S.visited[top.node]) continue;
+        while (!heap.empty()) {
+            top = heap.front();
+            std::pop_heap(heap.begin(), heap.end(), greater);
+            heap.pop_back();
+            if (C[top.node].visited) continue;
             found = true;
             break;
         }
         if (!found) break;
         const int t = top.node;
-        S.visited[t] = 1;
+        C[t].visited = 1;
         lsp = top.key;
         if (lsap <= lsp) {
-            S.visited[t] = 0;
-            S.dval[t] = LOC_INFINITY;
+            C[t].visited = 0;
+            C[t].dval = LOC_INFINITY;
             break;
         }
-        S.inB[t] = 1;
+        C[t].inB = 1;
         S.Blist[b_cnt++] = t;
-        cur = S.match[t];
+        cur = C[t].match;
     }
     if (lsap < LOC_INFINITY / 2.0) {
-        S.matched[isap] = 1;
-        cur = S.match[isap];
+        C[isap].matched = 1;
+        cur = C[isap].match;
         S.v[cur] = c[ksap];
         while (S.prev[cur] >= 0) {
-            S.match[isap] = cur;
+            C[isap].match = cur;
             int next_k = S.rev_entry[cur];
             const int next_i = ci[next_k];
-            next_k = S.irn[next_i];
+            next_k = C[next_i].irn;
             S.rev_entry[cur] = ksap;
             cur = S.prev[cur];
             isap = next_i;
             ksap = next_k;
         }
-        S.match[isap] = cur;
+        C[isap].match = cur;
         S.rev_entry[cur] = ksap;
         S.rev_matched[cur] = 1;
         for (int i = 0; i < b_cnt; ++i) {
-            const int row = S.Blist[i];
-            const int jv = S.match[row];
+            const int col = S.Blist[i];
+            DbCol& cb = C[col];
+            const int jv = cb.match;
             const int tk = S.rev_entry[jv];
-            S.u[row] += S.dval[row] - lsap;
-            S.v[jv] = c[tk] - S.u[row];
-            S.dval[row] = LOC_INFINITY;
-            S.visited[row] = 0;
-            S.inB[row] = 0;
+            cb.u += cb.dval - lsap;
+            S.v[jv] = c[tk] - cb.u;
+            cb.dval = LOC_INFINITY;
+            cb.visited = 0;
+            cb.inB = 0;
         }
-        while (!Q.empty()) {
-            S.dval[Q.top().node] = LOC_INFINITY;
-            Q.pop();
-        }
+        // the columns still queued get their distance reset (the order of
+        // the resets does not matter)
+        for (const auto& q : heap) C[q.node].dval = LOC_INFINITY;
+        heap.clear();
     }
 }
 
@@ -518,14 +538,20 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
         DbState<T> S(n, rp, ci);
         db_costs(S, val);
         db_init(S, o.dbFirstStageOnly);
-        S.inB.assign(n, 0);
-        S.visited.assign(n, 0);
         S.Blist.assign(n, 0);
-        S.dval.assign(n, LOC_INFINITY);
         S.prev.assign(n, 0);
-        S.irn.assign(n, 0);
-        for (int i = 0; i < n; ++i)
-            if (!S.rev_matched[i]) db_augment(S, i);
+        {
+            std::vector<DbCol> cols(n);
+            for (int i = 0; i < n; ++i) cols[i] = {S.u[i], LOC_INFINITY, S.match[i], 0, 0, 0, S.matched[i], 0};
+            std::vector<NodeKey<double>> heap;
+            for (int i = 0; i < n; ++i)
+                if (!S.rev_matched[i]) db_augment(S, cols, heap, i);
+            for (int i = 0; i < n; ++i) {
+                S.u[i] = cols[i].u;
+                S.match[i] = cols[i].match;
+                S.matched[i] = cols[i].matched;
+            }
+        }
         for (int i = 0; i < n; ++i)
             if (!S.matched[i]) throw Error(SAP_ERR_SINGULAR, "Singular matrix found");
         std::vector<double> cs(n), rs(n);
