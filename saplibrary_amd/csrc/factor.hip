@@ -127,18 +127,23 @@ __global__ __launch_bounds__(256) void band_factor_kernel(T* __restrict__ B0, T*
 }
 
 // ---------------------------------------------------------------------------
-// MFMA variant of the blocked banded LU (32-column panels), used for
-// k <= 224 (one panel row per thread).  Per panel:
+// MFMA variant of the blocked banded LU (32-column panels), for the bands the
+// register-resident kernel does not take, up to k = 480 (one or two panel
+// rows per thread).  Per panel:
 //   1. every thread holds one or two rows of the (32 + k) x 32 panel in
 //      registers and the panel is factored column by column (boosting at
 //      pivot time); the pivot row is broadcast through LDS, one barrier per
 //      column;
-//   2. each thread also holds one column of the 32 x k block of U rows to
-//      the right of the panel and applies the panel's row operations to it,
-//      one column step behind (multipliers shared through LDS);
-//   3. the k x k trailing window in global memory receives the rank-32
+//   2. each thread also holds one or two columns of the 32 x k block of U
+//      rows to the right of the panel and applies the panel's row
+//      operations to them, one column step behind (multipliers shared
+//      through LDS);
+//   3. the k x k trailing window in global memory (L2) receives the rank-32
 //      update -L_below * U_right on the matrix cores
-//      (v_mfma_f32_32x32x2f32, or v_mfma_f64_16x16x4f64 in FP64).
+//      (v_mfma_f32_32x32x2f32, or v_mfma_f64_16x16x4f64 in FP64), tiles
+//      held transposed so that their loads and stores run along columns,
+//      four tiles per wavefront in flight; -L_below comes from LDS or, for
+//      the widest FP64 bands, from the band itself.
 // Factors are written as L (unit), D, U' (divided), like the LDS variant.
 // ---------------------------------------------------------------------------
 template <class T>
@@ -177,18 +182,21 @@ struct Mfma<double> {
 
 constexpr int MNB = 32;   // panel width of the MFMA factorization
 
+// lg: the -L_below operand is read from the band (written one step
+// earlier by the same workgroup) instead of being staged in LDS, which
+// leaves the LDS to the U block alone (the wide bands).
 template <class T>
 struct MfmaLds {
     int kp, ls_stride, us_stride;
     size_t ls_off, us_off, ld_off, prow_off, piv_off, bytes;
-    __host__ __device__ MfmaLds(int k)
+    __host__ __device__ MfmaLds(int k, bool lg = false)
     {
         constexpr int TILE = sizeof(T) == 4 ? 32 : 16;
         kp = ((k + TILE - 1) / TILE) * TILE;
         ls_stride = MNB + 1;
         us_stride = kp + 4;
         ls_off = 0;
-        us_off = ls_off + (size_t)kp * ls_stride;
+        us_off = ls_off + (lg ? 0 : (size_t)kp * ls_stride);
         ld_off = us_off + (size_t)MNB * us_stride;
         prow_off = ld_off + (size_t)MNB * (MNB + 1);
         piv_off = prow_off + 2 * MNB;
@@ -196,7 +204,9 @@ struct MfmaLds {
     }
 };
 
-template <class T, int RPT>
+// RPT: panel rows (and U-block columns) per thread, so k + 32 <= 256 RPT.
+// LG: -L_below read from the band in the trailing update (MfmaLds lg).
+template <class T, int RPT, bool LG>
 __global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B0, T* __restrict__ B1, int k_all,
                                                                const FactorJob* __restrict__ jobs)
 {
@@ -214,7 +224,7 @@ __global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B
 
     extern __shared__ unsigned char smem_raw[];
     T* sm = reinterpret_cast<T*>(smem_raw);
-    const MfmaLds<T> lay(k);
+    const MfmaLds<T> lay(k, LG);
     T* Ls = sm + lay.ls_off;       // -L_below [kp][MNB+1]
     T* Us = sm + lay.us_off;       // U_right [MNB][kp+4]
     T* lcol = sm + lay.ld_off;     // multipliers of one panel column for rows < MNB, [2][MNB]
@@ -305,7 +315,7 @@ __global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B
                     }
                 }
             }
-            if (r >= nb && r - nb < lay.kp) {
+            if (!LG && r >= nb && r - nb < lay.kp) {
 #pragma unroll
                 for (int c = 0; c < MNB; ++c) Ls[(r - nb) * lay.ls_stride + c] = (r < H) ? -row[q][c] : (T)0;
             }
@@ -320,34 +330,77 @@ __global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B
             }
         }
         // Ls rows not covered above (panel rows beyond the threads) are padding
-        for (int r = 256 * RPT - nb + tid; r < lay.kp; r += 256) {
+        if (!LG) {
+            for (int r = 256 * RPT - nb + tid; r < lay.kp; r += 256) {
 #pragma unroll
-            for (int c = 0; c < MNB; ++c) Ls[r * lay.ls_stride + c] = (T)0;
+                for (int c = 0; c < MNB; ++c) Ls[r * lay.ls_stride + c] = (T)0;
+            }
         }
         __syncthreads();
 
-        // (4) trailing window -= L_below * U_right on the matrix cores
+        // (4) trailing window -= L_below * U_right on the matrix cores, TG
+        //     tiles per wavefront at a time: the window (and, LG, the
+        //     -L_below) loads of the group are issued before its products, so
+        //     their latencies overlap; each tile's products run in the same
+        //     order as one tile at a time.
         const int nt = (Mw + TILE - 1) / TILE;
-        for (int e = wave; e < nt * nt; e += 4) {
-            const int ti = e / nt, tj = e % nt;
-            typename MF::acc_t acc;
-            const int cc = tj * TILE + MF::col(lane);
-            // rows j0+nb+ii of column j0+nb+cc: step dir from the column's row j0+nb
-            T* tp = base + aidx(j0 + nb, j0 + nb + cc);
+        constexpr int TG = 4, NK = MNB / MF::KSTEP;
+        for (int e0 = wave; e0 < nt * nt; e0 += 4 * TG) {
+            typename MF::acc_t acc[TG];
+            T av[LG ? TG : 1][NK];
+            int tis[TG], tjs[TG];
 #pragma unroll
-            for (int r = 0; r < MF::NACC; ++r) {
-                const int ii = ti * TILE + MF::row(lane, r);
-                acc[r] = (ii < Mw && cc < Mw && ii - cc <= k && cc - ii <= k) ? tp[dir * ii] : (T)0;
+            for (int g = 0; g < TG; ++g) {
+                const int e = e0 + 4 * g;
+                tis[g] = e < nt * nt ? e / nt : nt;   // nt: no rows (every entry masked)
+                tjs[g] = e < nt * nt ? e % nt : 0;
+                // the tile is held transposed (acc^T += U^T (-L)^T): register r
+                // of lane l holds window row ii = col(l), column cc = row(l, r),
+                // so each load reads whole runs of consecutive rows of a column
+                const int ii = tis[g] * TILE + MF::col(lane);
+                const T* tp = base + aidx(j0 + nb + ii, j0 + nb + tjs[g] * TILE);
+#pragma unroll
+                for (int r = 0; r < MF::NACC; ++r) {
+                    const int cc = tjs[g] * TILE + MF::row(lane, r);
+                    acc[g][r] = (ii < Mw && cc < Mw && ii - cc <= k && cc - ii <= k)
+                                    ? tp[(int64_t)MF::row(lane, r) * cstep] : (T)0;
+                }
+                if constexpr (LG) {
+                    // A = -L(j0 + nb + ia, j0 + kk + ka) from the band: row ia
+                    // of L_below, panel column kk + ka (zero outside the band,
+                    // past the partition's end and past the panel)
+                    const int ia = tis[g] * TILE + MF::ab_mn(lane), ka = MF::ab_k(lane);
+                    const T* lp = base + aidx(j0 + nb + min(ia, Mw - 1), j0 + ka);
+#pragma unroll
+                    for (int q = 0; q < NK; ++q) {
+                        const int c = q * MF::KSTEP + ka;
+                        const bool ok = ia < Mw && c < nb && nb + ia - c <= k;
+                        av[g][q] = ok ? -lp[(int64_t)(q * MF::KSTEP) * cstep] : (T)0;
+                    }
+                }
             }
-            const T* la = Ls + (ti * TILE + MF::ab_mn(lane)) * lay.ls_stride + MF::ab_k(lane);
-            const T* ub = Us + MF::ab_k(lane) * lay.us_stride + tj * TILE + MF::ab_mn(lane);
 #pragma unroll
-            for (int kk = 0; kk < MNB; kk += MF::KSTEP)
-                acc = MF::mma(la[kk], ub[kk * lay.us_stride], acc);
+            for (int g = 0; g < TG; ++g) {
+                if (e0 + 4 * g >= nt * nt) break;   // uniform over the wavefront
+                const T* ub = Us + MF::ab_k(lane) * lay.us_stride + tjs[g] * TILE + MF::ab_mn(lane);
+                // A = U_right^T (from the staged U block), B = (-L_below)^T
+                if constexpr (LG) {
 #pragma unroll
-            for (int r = 0; r < MF::NACC; ++r) {
-                const int ii = ti * TILE + MF::row(lane, r);
-                if (ii < Mw && cc < Mw && ii - cc <= k && cc - ii <= k) tp[dir * ii] = acc[r];
+                    for (int q = 0; q < NK; ++q)
+                        acc[g] = MF::mma(ub[q * MF::KSTEP * lay.us_stride], av[g][q], acc[g]);
+                } else {
+                    const T* la = Ls + (tis[g] * TILE + MF::ab_mn(lane)) * lay.ls_stride + MF::ab_k(lane);
+#pragma unroll
+                    for (int q = 0; q < NK; ++q)
+                        acc[g] = MF::mma(ub[q * MF::KSTEP * lay.us_stride], la[q * MF::KSTEP], acc[g]);
+                }
+                const int ii = tis[g] * TILE + MF::col(lane);
+                T* tp = base + aidx(j0 + nb + ii, j0 + nb + tjs[g] * TILE);
+#pragma unroll
+                for (int r = 0; r < MF::NACC; ++r) {
+                    const int cc = tjs[g] * TILE + MF::row(lane, r);
+                    if (ii < Mw && cc < Mw && ii - cc <= k && cc - ii <= k) tp[(int64_t)MF::row(lane, r) * cstep] = acc[g][r];
+                }
             }
         }
         __syncthreads();
@@ -1150,21 +1203,34 @@ constexpr size_t LDS_LIMIT = 160 * 1024 - 1024;   // leave room for static LDS
 
 }  // namespace
 
-template <class T, int RPT>
+template <class T, int RPT, bool LG>
 void launch_factor_mfma(T* B0, T* B1, int k, const FactorJob* jobs, int njobs, hipStream_t st)
 {
-    const size_t lds = MfmaLds<T>(k).bytes;
-    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_factor_mfma_kernel<T, RPT>,
+    const size_t lds = MfmaLds<T>(k, LG).bytes;
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_factor_mfma_kernel<T, RPT, LG>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((band_factor_mfma_kernel<T, RPT>), dim3(njobs), dim3(256), lds, st, B0, B1, k, jobs);
+    hipLaunchKernelGGL((band_factor_mfma_kernel<T, RPT, LG>), dim3(njobs), dim3(256), lds, st, B0, B1, k, jobs);
     SAP_LAUNCH_CHECK();
+}
+
+// The matrix-core blocked kernel's variant for half-bandwidth k: 100 (one
+// panel row per thread, -L_below in LDS), 101 (two rows per thread), 102
+// (two rows per thread, -L_below from the band); -1 when none applies.
+template <class T>
+int factor_mfma_variant(int k)
+{
+    if (k + MNB <= 256 && MfmaLds<T>(k).bytes <= LDS_LIMIT) return 100;
+    if (k + MNB <= 512 && MfmaLds<T>(k).bytes <= LDS_LIMIT) return 101;
+    if (k + MNB <= 512 && MfmaLds<T>(k, true).bytes <= LDS_LIMIT) return 102;
+    return -1;
 }
 
 template <class T>
 int band_factor_kernel_class(int k, int64_t max_len)
 {
     if (band_factor_resident_fits<T>(k, max_len)) return (k + 32 + 15) / 16;   // resident, by window tiles
-    if (MfmaLds<T>(k).bytes <= LDS_LIMIT && k + MNB <= 256) return 100;
+    const int mv = factor_mfma_variant<T>(k);
+    if (mv > 0) return mv;
     if (factor_lds_bytes<T>(k, 32) <= LDS_LIMIT) return 132;
     if (factor_lds_bytes<T>(k, 16) <= LDS_LIMIT) return 116;
     if (factor_lds_bytes<T>(k, 8) <= LDS_LIMIT) return 108;
@@ -1178,8 +1244,12 @@ void launch_band_factor(T* B0, T* B1, const T* Bin, int k, const FactorJob* d_jo
     if (njobs <= 0) return;
     if (launch_band_factor_resident<T>(B0, B1, Bin, k, d_jobs, njobs, max_len, st)) return;
     if (Bin) throw Error(SAP_ERR_HIP, "separate-input factorization needs the register-resident kernel");
-    const bool fits = MfmaLds<T>(k).bytes <= LDS_LIMIT;
-    if (fits && k + MNB <= 256) { launch_factor_mfma<T, 1>(B0, B1, k, d_jobs, njobs, st); return; }
+    switch (factor_mfma_variant<T>(k)) {
+    case 100: launch_factor_mfma<T, 1, false>(B0, B1, k, d_jobs, njobs, st); return;
+    case 101: launch_factor_mfma<T, 2, false>(B0, B1, k, d_jobs, njobs, st); return;
+    case 102: launch_factor_mfma<T, 2, true>(B0, B1, k, d_jobs, njobs, st); return;
+    default: break;
+    }
     if (factor_lds_bytes<T>(k, 32) <= LDS_LIMIT)      launch_factor_nb<T, 32>(B0, B1, k, d_jobs, njobs, st);
     else if (factor_lds_bytes<T>(k, 16) <= LDS_LIMIT) launch_factor_nb<T, 16>(B0, B1, k, d_jobs, njobs, st);
     else if (factor_lds_bytes<T>(k, 8) <= LDS_LIMIT)  launch_factor_nb<T, 8>(B0, B1, k, d_jobs, njobs, st);

@@ -104,6 +104,27 @@ def test_factors_match_oracle(sap, oracle, n, k, P, fact, precond, prec_double):
             np.testing.assert_allclose(s[i].T, Sc, rtol=0, atol=tol * max(1.0, np.abs(Sc).max()))
 
 
+# Bands wider than the register-resident factorization takes: the blocked
+# matrix-core kernel with two panel rows per thread (k + 32 > 256), -L_below
+# staged in LDS (FP32, and FP64 up to k = 256) or read back from the band
+# (FP64 from k = 272 on, where the LDS holds only the U block).
+WIDE_FACTOR_CASES = [(4000, 230, 3, "LU_UL", "Block"), (3000, 300, 2, "LU_only", "Block"),
+                     (2400, 460, 2, "LU_UL", "Block")]
+
+
+@pytest.mark.parametrize("n,k,P,fact,precond", WIDE_FACTOR_CASES)
+@pytest.mark.parametrize("prec_double", [True, False])
+def test_wide_band_factors_match_oracle(sap, oracle, n, k, P, fact, precond, prec_double):
+    band = oracle.gen_banded(n, k, 0.8, 25)
+    gpu, cpu = make_pair(sap, oracle, P, fact, precond, prec_double)
+    gpu.setup(sap.BandedMatrix(dev(band)))
+    assert cpu.setup(band) == 0
+    assert gpu.numPartitions == cpu.num_partitions == P
+    Fg, Fc = gpu.factors(), cpu.factors()
+    tol = 1e-10 if prec_double else 2e-4
+    np.testing.assert_allclose(Fg, Fc, rtol=0, atol=tol * np.abs(Fc).max())
+
+
 @pytest.mark.parametrize("n,k,P,fact,precond", FACTOR_CASES)
 @pytest.mark.parametrize("prec_double", [True, False])
 def test_actual_nnz_counts_the_factored_band(sap, oracle, n, k, P, fact, precond, prec_double):
