@@ -341,6 +341,87 @@ void db_augment(DbState<T>& S, std::vector<DbCol>& C, std::vector<NodeKey<double
     }
 }
 
+// Every augmentation of the DB matching (the loop over unmatched rows of
+// Graph::DB, sap/graph.h:1256-1263), run on a relabelled copy of the
+// matrix for locality: node i (row i and column i of the square pattern)
+// gets its position in a breadth-first order of the pattern (restarted at
+// the lowest unnumbered node), and the rows are stored in that order with
+// each row's entries in their original order.  The search never compares
+// node labels (the heap orders by distance, and pushes happen in entry
+// order), the rows are still visited in their original order, and the
+// same operations run on the same values, so the matching, the duals and
+// every tie are those of the original labelling; a randomly permuted
+// matrix stops sending every relaxed edge to a random cache line.
+template <class T>
+void db_augment_all(DbState<T>& S)
+{
+    const int n = S.n;
+    const auto& rp = S.rp;
+    const auto& ci = S.ci;
+    const size_t nnz = S.c.size();
+    std::vector<int> pos(n, -1), inv(n);
+    {
+        int head = 0, tail = 0, seed = 0;
+        while (tail < n) {
+            if (head == tail) {
+                while (pos[seed] >= 0) ++seed;
+                pos[seed] = tail;
+                inv[tail++] = seed;
+            }
+            const int r = inv[head++];
+            for (int e = rp[r]; e < rp[r + 1]; ++e) {
+                const int j = ci[e];
+                if (pos[j] < 0) {
+                    pos[j] = tail;
+                    inv[tail++] = j;
+                }
+            }
+        }
+    }
+    std::vector<int> rp2(n + 1), ci2(nnz), emap(nnz), einv(nnz);
+    rp2[0] = 0;
+    for (int r2 = 0; r2 < n; ++r2) {
+        const int r = inv[r2];
+        const int len = rp[r + 1] - rp[r];
+        rp2[r2 + 1] = rp2[r2] + len;
+        for (int t = 0; t < len; ++t) {
+            const int e = rp[r] + t, e2 = rp2[r2] + t;
+            ci2[e2] = pos[ci[e]];
+            emap[e] = e2;
+            einv[e2] = e;
+        }
+    }
+    DbState<T> R(n, rp2, ci2);
+    R.c.resize(nnz);
+    for (size_t e = 0; e < nnz; ++e) R.c[emap[e]] = S.c[e];
+    R.v.resize(n);
+    R.rev_entry.resize(n);
+    R.rev_matched.resize(n);
+    for (int r = 0; r < n; ++r) {
+        R.v[pos[r]] = S.v[r];
+        R.rev_entry[pos[r]] = emap[S.rev_entry[r]];
+        R.rev_matched[pos[r]] = S.rev_matched[r];
+    }
+    R.Blist.assign(n, 0);
+    R.prev.assign(n, 0);
+    std::vector<DbCol> cols(n);
+    for (int j = 0; j < n; ++j) cols[pos[j]] = {S.u[j], LOC_INFINITY, pos[S.match[j]], 0, 0, 0, S.matched[j], 0};
+    std::vector<NodeKey<double>> heap;
+    for (int i = 0; i < n; ++i)
+        if (!R.rev_matched[pos[i]]) db_augment(R, cols, heap, pos[i]);
+    for (int j = 0; j < n; ++j) {
+        const DbCol& cj = cols[pos[j]];
+        S.u[j] = cj.u;
+        S.match[j] = inv[cj.match];
+        S.matched[j] = cj.matched;
+    }
+    for (int r = 0; r < n; ++r) {
+        S.v[r] = R.v[pos[r]];
+        S.rev_entry[r] = einv[R.rev_entry[pos[r]]];
+        S.rev_matched[r] = R.rev_matched[pos[r]];
+    }
+}
+
 // ------------------------------------------------------------------- RCM
 // Symmetric adjacency without self loops, each (i, j) edge inserted as
 // (i, j) and (j, i) in entry order, then grouped by row stably
@@ -538,20 +619,7 @@ Reordering<T> reorder_csr(int n, const int* rp_in, const int* ci_in, const doubl
         DbState<T> S(n, rp, ci);
         db_costs(S, val);
         db_init(S, o.dbFirstStageOnly);
-        S.Blist.assign(n, 0);
-        S.prev.assign(n, 0);
-        {
-            std::vector<DbCol> cols(n);
-            for (int i = 0; i < n; ++i) cols[i] = {S.u[i], LOC_INFINITY, S.match[i], 0, 0, 0, S.matched[i], 0};
-            std::vector<NodeKey<double>> heap;
-            for (int i = 0; i < n; ++i)
-                if (!S.rev_matched[i]) db_augment(S, cols, heap, i);
-            for (int i = 0; i < n; ++i) {
-                S.u[i] = cols[i].u;
-                S.match[i] = cols[i].match;
-                S.matched[i] = cols[i].matched;
-            }
-        }
+        db_augment_all(S);
         for (int i = 0; i < n; ++i)
             if (!S.matched[i]) throw Error(SAP_ERR_SINGULAR, "Singular matrix found");
         std::vector<double> cs(n), rs(n);
