@@ -128,16 +128,15 @@ __global__ __launch_bounds__(256) void band_factor_kernel(T* __restrict__ B0, T*
 
 // ---------------------------------------------------------------------------
 // MFMA variant of the blocked banded LU (32-column panels), for the bands the
-// register-resident kernel does not take, up to k = 480 (one or two panel
-// rows per thread).  Per panel:
-//   1. every thread holds one or two rows of the (32 + k) x 32 panel in
+// register-resident kernel does not take, up to k = 480 (256 or 512
+// threads, one panel row each).  Per panel:
+//   1. every thread holds one row of the (32 + k) x 32 panel in
 //      registers and the panel is factored column by column (boosting at
 //      pivot time); the pivot row is broadcast through LDS, one barrier per
 //      column;
-//   2. each thread also holds one or two columns of the 32 x k block of U
-//      rows to the right of the panel and applies the panel's row
-//      operations to them, one column step behind (multipliers shared
-//      through LDS);
+//   2. each thread also holds one column of the 32 x k block of U rows to
+//      the right of the panel and applies the panel's row operations to it,
+//      one column step behind (multipliers shared through LDS);
 //   3. the k x k trailing window in global memory (L2) receives the rank-32
 //      update -L_below * U_right on the matrix cores
 //      (v_mfma_f32_32x32x2f32, or v_mfma_f64_16x16x4f64 in FP64), tiles
@@ -198,16 +197,17 @@ struct MfmaLds {
         ls_off = 0;
         us_off = ls_off + (lg ? 0 : (size_t)kp * ls_stride);
         ld_off = us_off + (size_t)MNB * us_stride;
-        prow_off = ld_off + (size_t)MNB * (MNB + 1);
+        prow_off = ld_off + (size_t)2 * MNB;
         piv_off = prow_off + 2 * MNB;
         bytes = (piv_off + MNB) * sizeof(T);
     }
 };
 
-// RPT: panel rows (and U-block columns) per thread, so k + 32 <= 256 RPT.
-// LG: -L_below read from the band in the trailing update (MfmaLds lg).
-template <class T, int RPT, bool LG>
-__global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B0, T* __restrict__ B1, int k_all,
+// NT threads (256 or 512), one panel row and one U-block column each, so
+// k + 32 <= NT.  LG: -L_below read from the band in the trailing update
+// (MfmaLds lg).
+template <class T, int NT, bool LG>
+__global__ __launch_bounds__(NT) void band_factor_mfma_kernel(T* __restrict__ B0, T* __restrict__ B1, int k_all,
                                                                const FactorJob* __restrict__ jobs)
 {
     using MF = Mfma<T>;
@@ -233,19 +233,20 @@ __global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
+    constexpr int RPT = 1, NW = NT / 64;
 
     for (int64_t j0 = 0; j0 < len; j0 += MNB) {
         const int nb = (int)min((int64_t)MNB, len - j0);
         const int H = (int)min(len - j0, (int64_t)nb + k);
         const int Mw = (int)min(len - j0 - nb, (int64_t)k);
 
-        // (1) thread tid + 256 q owns panel row r = tid + 256 q (32 registers)
-        //     and U-block column c = tid + 256 q (rows j0..j0+31, 32 registers)
+        // (1) thread tid owns panel row r = tid (32 registers) and U-block
+        //     column c = tid (rows j0..j0+31, 32 registers)
         T row[RPT][MNB];
         T ucol[RPT][MNB];
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            const int r = tid + 256 * q;
+            const int r = tid + NT * q;
             const T* rp = base + aidx(j0 + r, j0);
 #pragma unroll
             for (int c = 0; c < MNB; ++c)
@@ -275,7 +276,7 @@ __global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B
                 T* lc = lcol + (jj & 1) * MNB;
 #pragma unroll
                 for (int q = 0; q < RPT; ++q) {
-                    const int r = tid + 256 * q;
+                    const int r = tid + NT * q;
                     T l = (T)0;
                     if (r > jj && r < H && r - jj <= k) {
                         l = row[q][jj] / piv;
@@ -302,7 +303,7 @@ __global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B
         //     (divided); stage -L_below and the undivided U block in LDS
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            const int r = tid + 256 * q;
+            const int r = tid + NT * q;
             if (r < H) {
                 const T pr_r = r < nb ? pivs[r] : (T)1;
                 T* rp = base + aidx(j0 + r, j0);
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B
         }
         // Ls rows not covered above (panel rows beyond the threads) are padding
         if (!LG) {
-            for (int r = 256 * RPT - nb + tid; r < lay.kp; r += 256) {
+            for (int r = NT - nb + tid; r < lay.kp; r += NT) {
 #pragma unroll
                 for (int c = 0; c < MNB; ++c) Ls[r * lay.ls_stride + c] = (T)0;
             }
@@ -339,19 +340,20 @@ __global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B
         __syncthreads();
 
         // (4) trailing window -= L_below * U_right on the matrix cores, TG
-        //     tiles per wavefront at a time: the window (and, LG, the
+        //     tiles per wavefront at a time (8 KB of FP64 or FP32 window): the
+        //     window (and, LG, the
         //     -L_below) loads of the group are issued before its products, so
         //     their latencies overlap; each tile's products run in the same
         //     order as one tile at a time.
         const int nt = (Mw + TILE - 1) / TILE;
-        constexpr int TG = 4, NK = MNB / MF::KSTEP;
-        for (int e0 = wave; e0 < nt * nt; e0 += 4 * TG) {
+        constexpr int TG = sizeof(T) == 4 ? 2 : 4, NK = MNB / MF::KSTEP;   // 32 x 32 FP32 tiles: two
+        for (int e0 = wave; e0 < nt * nt; e0 += NW * TG) {
             typename MF::acc_t acc[TG];
             T av[LG ? TG : 1][NK];
             int tis[TG], tjs[TG];
 #pragma unroll
             for (int g = 0; g < TG; ++g) {
-                const int e = e0 + 4 * g;
+                const int e = e0 + NW * g;
                 tis[g] = e < nt * nt ? e / nt : nt;   // nt: no rows (every entry masked)
                 tjs[g] = e < nt * nt ? e % nt : 0;
                 // the tile is held transposed (acc^T += U^T (-L)^T): register r
@@ -381,7 +383,7 @@ __global__ __launch_bounds__(256) void band_factor_mfma_kernel(T* __restrict__ B
             }
 #pragma unroll
             for (int g = 0; g < TG; ++g) {
-                if (e0 + 4 * g >= nt * nt) break;   // uniform over the wavefront
+                if (e0 + NW * g >= nt * nt) break;   // uniform over the wavefront
                 const T* ub = Us + MF::ab_k(lane) * lay.us_stride + tjs[g] * TILE + MF::ab_mn(lane);
                 // A = U_right^T (from the staged U block), B = (-L_below)^T
                 if constexpr (LG) {
@@ -1203,19 +1205,19 @@ constexpr size_t LDS_LIMIT = 160 * 1024 - 1024;   // leave room for static LDS
 
 }  // namespace
 
-template <class T, int RPT, bool LG>
+template <class T, int NT, bool LG>
 void launch_factor_mfma(T* B0, T* B1, int k, const FactorJob* jobs, int njobs, hipStream_t st)
 {
     const size_t lds = MfmaLds<T>(k, LG).bytes;
-    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_factor_mfma_kernel<T, RPT, LG>,
+    SAP_HIP_CHECK(hipFuncSetAttribute((const void*)band_factor_mfma_kernel<T, NT, LG>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((band_factor_mfma_kernel<T, RPT, LG>), dim3(njobs), dim3(256), lds, st, B0, B1, k, jobs);
+    hipLaunchKernelGGL((band_factor_mfma_kernel<T, NT, LG>), dim3(njobs), dim3(NT), lds, st, B0, B1, k, jobs);
     SAP_LAUNCH_CHECK();
 }
 
-// The matrix-core blocked kernel's variant for half-bandwidth k: 100 (one
-// panel row per thread, -L_below in LDS), 101 (two rows per thread), 102
-// (two rows per thread, -L_below from the band); -1 when none applies.
+// The matrix-core blocked kernel's variant for half-bandwidth k: 100 (256
+// threads, -L_below in LDS), 101 (512 threads), 102 (512 threads, -L_below
+// from the band); -1 when none applies.
 template <class T>
 int factor_mfma_variant(int k)
 {
@@ -1245,9 +1247,9 @@ void launch_band_factor(T* B0, T* B1, const T* Bin, int k, const FactorJob* d_jo
     if (launch_band_factor_resident<T>(B0, B1, Bin, k, d_jobs, njobs, max_len, st)) return;
     if (Bin) throw Error(SAP_ERR_HIP, "separate-input factorization needs the register-resident kernel");
     switch (factor_mfma_variant<T>(k)) {
-    case 100: launch_factor_mfma<T, 1, false>(B0, B1, k, d_jobs, njobs, st); return;
-    case 101: launch_factor_mfma<T, 2, false>(B0, B1, k, d_jobs, njobs, st); return;
-    case 102: launch_factor_mfma<T, 2, true>(B0, B1, k, d_jobs, njobs, st); return;
+    case 100: launch_factor_mfma<T, 256, false>(B0, B1, k, d_jobs, njobs, st); return;
+    case 101: launch_factor_mfma<T, 512, false>(B0, B1, k, d_jobs, njobs, st); return;
+    case 102: launch_factor_mfma<T, 512, true>(B0, B1, k, d_jobs, njobs, st); return;
     default: break;
     }
     if (factor_lds_bytes<T>(k, 32) <= LDS_LIMIT)      launch_factor_nb<T, 32>(B0, B1, k, d_jobs, njobs, st);
