@@ -356,8 +356,11 @@ struct Precond {
             /* LU_only (precond.h:1269-1292).  Every partition LU factored; the
              * spike tips come from the LU factors: V_i from the bottom blocks,
              * W_{i+1} from full sweeps (calculateSpikes_const, :5787-5949).
-             * The factors are post-divided before the sweeps (the reference
-             * leaves U undivided on this path). */
+             * The factors are post-divided before the sweeps, as the GPU
+             * library does on every path: the reference leaves U undivided
+             * here (bandLU_post_divide runs only for Block or P = 1,
+             * precond.h:4408-4417) while its sweeps assume a unit U'
+             * (:5152-5164). */
 #pragma omp parallel for schedule(dynamic, 1)
             for (int p = 0; p < P; ++p) {
                 int64_t s, len; part_bounds(n, P, p, s, len);
