@@ -359,6 +359,8 @@ void db_augment_all(DbState<T>& S)
     const auto& rp = S.rp;
     const auto& ci = S.ci;
     const size_t nnz = S.c.size();
+    // no entries: no row can be matched (the singular check follows)
+    if (nnz == 0) return;
     std::vector<int> pos(n, -1), inv(n);
     {
         int head = 0, tail = 0, seed = 0;

@@ -285,6 +285,10 @@ def test_product_reordering_rejects_singular(sap):
     with pytest.raises(sap.SapError) as ei:
         sap.reorder_csr(rp, ci, val)
     assert ei.value.reason == -5
+    # no entries at all
+    with pytest.raises(sap.SapError) as ei:
+        sap.reorder_csr(np.zeros(4, dtype=np.int32), np.zeros(0, dtype=np.int32), np.zeros(0))
+    assert ei.value.reason == -5
 
 
 # ------------------------------------------------ variable-bandwidth assembly
