@@ -207,6 +207,14 @@ def main():
     kernels = {kname: {"launches_per_step": round(l / steps_profiled, 2),
                        "ms_per_step": round(t / steps_profiled, 3)} for kname, (l, t) in prof.items()}
 
+    if roof is not None and name == "band_factor":
+        # one workgroup per (partition, LU or UL) job holds a CU for the whole
+        # launch: the peak of the CUs the launch occupies
+        jobs = 2 * (P - 1) if spike and P > 1 else P
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+        roof["workgroups"] = jobs
+        roof["frac_of_occupied_cus_peak"] = round(achieved / (peak * min(jobs, cus) / cus), 4)
+
     # the practical streaming-read bandwidth (the library's own read kernel)
     read_gbs = sap.stream_read_gbs()
     if roof is not None and roof["unit"] == "GB/s":
