@@ -454,8 +454,8 @@ def bench_c2(args, torch, sap):
     of width 300 under a symmetric permutation, 20% weak diagonals): host
     DB + RCM + drop-off, then SaP.  The specified recipe (1% drop-off, SaP-C
     with 32 partitions, FP32 factors) is timed as specified and reported
-    with its outcome; the configuration that converges on this matrix (no
-    drop-off, 8 partitions, FP64 factors) is timed beside it."""
+    with its outcome; the fastest configuration found that converges on this
+    matrix (no drop-off, 4 partitions, FP64 factors) is timed beside it."""
     from saplibrary_amd import workloads as W
     n = 500_000
     rp, ci, val = W.general_sparse(n)
@@ -465,7 +465,7 @@ def bench_c2(args, torch, sap):
     sap.csr_spmv(A, xt, b)
     out = {}
     for name, P, o in [("specified", 32, dict(dropOffFraction=0.01, maxNumIterations=30, refine=3)),
-                       ("converging", 8, dict(dropOffFraction=0.0, precValueType=sap.PREC_DOUBLE, refine=4))]:
+                       ("converging", 4, dict(dropOffFraction=0.0, precValueType=sap.PREC_DOUBLE, refine=4))]:
         times = []
         for it in range(args.warmup + max(1, min(args.steps, 2))):
             s = sap.Solver(P, sap.Options(relTol=1e-10, **o))

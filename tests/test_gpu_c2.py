@@ -11,7 +11,7 @@ symmetric permutation, 20% weak diagonals; saplibrary_amd.workloads).
     at 32 partitions (the oracle behaves the same on the reordered band,
     tests/test_c2_workload.py), so the solve stops with a failure code (its
     refinement limit) and the residual it reached.
-  * The configuration that converges (no drop-off, P=8, FP64 factors)
+  * The configurations that converge (no drop-off, P=4 or 8, FP64 factors)
     reaches a true relative residual <= 1e-10, checked with the library's
     CSR SpMV on the input matrix.
 """
@@ -73,10 +73,11 @@ def test_c2_specified_recipe_fails_honestly(sap, c2):
     assert np.isfinite(g.getStats().relResidualNorm)
 
 
-def test_c2_converging_configuration(sap, c2):
+@pytest.mark.parametrize("P", [4, 8])
+def test_c2_converging_configuration(sap, c2, P):
     rp, ci, val, A, b = c2
-    print("[c2] converging configuration: SaP-C P=8 FP64, no drop-off", flush=True)
-    g = sap.Solver(8, sap.Options(relTol=1e-10, precValueType=sap.PREC_DOUBLE, refine=4))
+    print(f"[c2] converging configuration: SaP-C P={P} FP64, no drop-off", flush=True)
+    g = sap.Solver(P, sap.Options(relTol=1e-10, precValueType=sap.PREC_DOUBLE, refine=4))
     g.setup(A)
     x = torch.zeros_like(b)
     assert g.solve(None, b, x), g.getMonitorMessage()
@@ -84,4 +85,4 @@ def test_c2_converging_configuration(sap, c2):
     sap.csr_spmv(A, x, r)
     rel = float(torch.linalg.norm(b - r) / torch.linalg.norm(b))
     assert rel <= 1e-10, rel
-    assert g.getStats().numPartitions == 8
+    assert g.getStats().numPartitions == P
